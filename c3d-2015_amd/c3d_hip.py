@@ -1,0 +1,249 @@
+"""Python front end of libc3d_hip.so (ctypes over the C-ABI in include/c3d_hip.h).
+
+Device buffers are passed as raw device pointers; torch tensors on a ROCm device are a
+convenient way to own them (see Ctx methods, which accept tensors or ints).  The library
+must be built (``make -C c3d-2015_amd``); loading fails loudly otherwise, and nothing here
+falls back to a CPU path.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libc3d_hip.so")
+
+(C3D_OK, C3D_ERR_ARG, C3D_ERR_HIP, C3D_ERR_U0, C3D_ERR_LADDER, C3D_ERR_UNSUPPORTED,
+ C3D_ERR_BUDGET) = range(7)
+FLAG_ISO_RESTART, FLAG_U0_NONPOS, FLAG_LADDER, FLAG_BUDGET = 1, 2, 4, 8
+
+_vp = C.c_void_p
+_i32 = C.c_int32
+_i64 = C.c_int64
+_d = C.c_double
+
+
+class ClosureOpts(C.Structure):
+    _fields_ = [("maxIter", _i32), ("useGaunt", _i32), ("useGauntSparse", _i32),
+                ("unbatched", _i32), ("tol", _d), ("tolGamma", _d), ("condHMax", _d)]
+
+
+def closure_opts(maxIter=100, tol=1e-8, tolGamma=1.1, condHMax=1e10, useGaunt=False,
+                 useGauntSparse=False, unbatched=False):
+    return ClosureOpts(maxIter, int(useGaunt), int(useGauntSparse), int(unbatched), tol,
+                       tolGamma, condHMax)
+
+
+# (name, restype, argtypes) for every symbol of include/c3d_hip.h
+SIGNATURES = [
+    ("solveFluxKinetic_hip", None, [C.c_int] * 5 + [_d] * 3 + [_vp] * 7 + [C.c_int]),
+    ("solveHInit_hip", None, [C.c_int] * 4 + [_vp] * 7 + [C.c_int]),
+    ("solveH_hip", None, [C.c_int] * 4 + [_vp, _vp, C.c_int, C.c_int]),
+    ("c3d_create", C.c_int, [C.POINTER(_vp), C.c_int, C.c_int]),
+    ("c3d_create_from_tables", C.c_int, [C.POINTER(_vp), C.c_int, C.c_int] + [_vp] * 7 +
+     [C.c_int, _vp, _vp, _vp]),
+    ("c3d_destroy", C.c_int, [_vp]),
+    ("c3d_tables_host", C.c_int, [C.c_int, C.c_int] + [_vp] * 9 + [C.c_int, _vp, _vp, _vp]),
+    ("c3d_sizes", C.c_int, [_vp] + [C.POINTER(C.c_int)] * 4),
+    ("c3d_get_tables", C.c_int, [_vp] * 12),
+    ("c3d_closure", C.c_int, [_vp, _i64, _vp, _vp, C.POINTER(ClosureOpts), _vp, _vp]),
+    ("c3d_set_pass_budget", C.c_int, [_vp, C.c_int]),
+    ("c3d_hessian", C.c_int, [_vp, _i64, _vp, _vp, C.c_int, C.c_int, _vp]),
+    ("c3d_dense_solve", C.c_int, [C.c_int, _i64] + [_vp] * 6 + [C.c_int, _vp]),
+    ("c3d_sync", C.c_int, [_vp, _vp]),
+    ("c3d_ghost_fill", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
+    ("c3d_flux_kinetic", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _d, _d, _d, _d,
+                                   C.c_int, _vp]),
+    ("c3d_flux_moment", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _d, _d, _d, _vp]),
+    ("c3d_euler", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _d, _vp]),
+    ("c3d_rk2_combine", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_int, _vp]),
+    ("c3d_update", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _d, _d, _d,
+                             _d, _d, C.c_int, C.c_int, C.POINTER(ClosureOpts), _vp, _vp]),
+    ("c3d_make_closure_batch", C.c_int, [_vp, _i64, C.c_uint64, _d, _d, _vp, _vp, _vp, _vp]),
+]
+
+_lib = None
+
+
+def load():
+    """Load libc3d_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C c3d-2015_amd`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(x):
+    """device pointer of a tensor / int / None"""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _stream(s):
+    if s is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(s, int):
+        return s
+    return s.cuda_stream
+
+
+class C3DError(RuntimeError):
+    def __init__(self, fn, rc):
+        super().__init__(f"{fn} returned {rc}")
+        self.rc = rc
+
+
+def check(fn, rc):
+    if rc != C3D_OK:
+        raise C3DError(fn, rc)
+    return rc
+
+
+class Ctx:
+    """A c3d context (tables resident on the current device)."""
+
+    def __init__(self, N=None, Q=None, tables=None):
+        L = load()
+        h = _vp()
+        if tables is None:
+            check("c3d_create", L.c3d_create(C.byref(h), N, Q))
+        else:
+            T = tables
+            nnz = int(T.nnz)
+            check("c3d_create_from_tables", L.c3d_create_from_tables(
+                C.byref(h), T.N, T.Q, T.w.ctypes.data, T.xi.ctypes.data, T.eta.ctypes.data,
+                T.mu.ctypes.data, T.M.ctypes.data, T.M2.ctypes.data, T.beta.ctypes.data, nnz,
+                T.betaVal.ctypes.data if nnz else None,
+                T.betaColInd.ctypes.data if nnz else None,
+                T.betaRowPtr.ctypes.data if nnz else None))
+        self.h = h
+        nm, nm2, nq, nnz = (C.c_int() for _ in range(4))
+        L.c3d_sizes(h, C.byref(nm), C.byref(nm2), C.byref(nq), C.byref(nnz))
+        self.nm, self.nm2, self.nq, self.nnz = nm.value, nm2.value, nq.value, nnz.value
+        self.L = L
+
+    def close(self):
+        if self.h:
+            self.L.c3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def closure(self, u, alpha, opts=None, stats=None, stream=None, ncells=None):
+        if opts is None:
+            opts = closure_opts()
+        n = ncells if ncells is not None else u.numel() // self.nm
+        return check("c3d_closure", self.L.c3d_closure(self.h, n, _p(u), _p(alpha),
+                                                       C.byref(opts), _p(stats),
+                                                       _stream(stream)))
+
+    def set_pass_budget(self, n):
+        return check("c3d_set_pass_budget", self.L.c3d_set_pass_budget(self.h, n))
+
+    def hessian(self, alpha, hs, stream=None):
+        n = alpha.numel() // self.nm
+        return check("c3d_hessian", self.L.c3d_hessian(self.h, n, _p(alpha), _p(hs), 0, 0,
+                                                       _stream(stream)))
+
+    def sync(self, stream=None):
+        return self.L.c3d_sync(self.h, _stream(stream))
+
+    def ghost_fill(self, nx, ny, nz, u, stream=None):
+        return check("c3d_ghost_fill", self.L.c3d_ghost_fill(self.h, nx, ny, nz, _p(u),
+                                                             _stream(stream)))
+
+    def flux_kinetic(self, nx, ny, nz, alpha, flux, dx, dy, dz, theta, isPn=0, stream=None):
+        return check("c3d_flux_kinetic", self.L.c3d_flux_kinetic(
+            self.h, nx, ny, nz, _p(alpha), _p(flux), dx, dy, dz, theta, int(isPn),
+            _stream(stream)))
+
+    def flux_moment(self, nx, ny, nz, u, flux, dx, dy, dz, stream=None):
+        return check("c3d_flux_moment", self.L.c3d_flux_moment(self.h, nx, ny, nz, _p(u),
+                                                               _p(flux), dx, dy, dz,
+                                                               _stream(stream)))
+
+    def euler(self, nx, ny, nz, u, flux, sigmaT, sigmaS, dt, stream=None):
+        return check("c3d_euler", self.L.c3d_euler(self.h, nx, ny, nz, _p(u), _p(flux),
+                                                   _p(sigmaT), _p(sigmaS), dt, _stream(stream)))
+
+    def rk2_combine(self, nx, ny, nz, u, uOld, kind, stream=None):
+        return check("c3d_rk2_combine", self.L.c3d_rk2_combine(self.h, nx, ny, nz, _p(u),
+                                                               _p(uOld), kind, _stream(stream)))
+
+    def update(self, nx, ny, nz, u, alpha, sigmaT, sigmaS, dx, dy, dz, theta, dt, solver=1,
+               pnUseKinetic=0, opts=None, stats=None, stream=None):
+        if opts is None:
+            opts = closure_opts()
+        return check("c3d_update", self.L.c3d_update(
+            self.h, nx, ny, nz, _p(u), _p(alpha), _p(sigmaT), _p(sigmaS), dx, dy, dz, theta, dt,
+            solver, pnUseKinetic, C.byref(opts), _p(stats), _stream(stream)))
+
+    def make_closure_batch(self, n, seed, smin, smax, u, alpha0, alphaStar=None, stream=None):
+        return check("c3d_make_closure_batch", self.L.c3d_make_closure_batch(
+            self.h, n, seed, smin, smax, _p(u), _p(alpha0), _p(alphaStar), _stream(stream)))
+
+
+STATS_FIELDS = [("r", "f8"), ("err", "f8"), ("gamma", "f8"), ("totalIter", "i4"),
+                ("iterGamma", "i4"), ("regIndex", "i4"), ("flags", "i4")]
+STATS_BYTES = 40  # sizeof(c3d_cell_stats)
+
+
+def alloc_stats(n, device):
+    """A zeroed device buffer for n c3d_cell_stats records."""
+    import torch
+    return torch.zeros(n * STATS_BYTES, dtype=torch.uint8, device=device)
+
+
+def stats_to_numpy(t):
+    """A device buffer of ncells c3d_cell_stats records -> numpy structured array."""
+    import numpy as np
+    raw = t.detach().cpu().contiguous().view(-1).numpy().view(np.uint8)
+    return raw.view(np.dtype(STATS_FIELDS))
+
+
+def dense_solve(nm, H, g, d, rcond, info, L, lapack_order=0, stream=None):
+    """Batched Cholesky + dpocon + Newton direction on device tensors."""
+    n = H.numel() // (nm * nm)
+    return check("c3d_dense_solve", load().c3d_dense_solve(
+        nm, n, _p(H), _p(g), _p(d), _p(rcond), _p(info), _p(L), int(lapack_order),
+        _stream(stream)))
+
+
+def tables_host(N, Q):
+    """The library's own tables (host only; no device needed)."""
+    import numpy as np
+    L = load()
+    nm, nm2, nq = (N + 1) ** 2, (2 * N + 1) ** 2, 2 * Q * Q
+    out = {k: np.zeros(nq) for k in ("w", "xi", "eta", "mu")}
+    out["M"] = np.zeros((nq, nm))
+    out["M2"] = np.zeros((nq, nm2))
+    out["beta"] = np.zeros((nm * nm, nm2))
+    out["pnOps"] = np.zeros((6, nm, nm))
+    nnz = C.c_int()
+    rowptr = np.zeros(nm * nm + 1, dtype=np.int32)
+    check("c3d_tables_host", L.c3d_tables_host(
+        N, Q, out["w"].ctypes.data, out["xi"].ctypes.data, out["eta"].ctypes.data,
+        out["mu"].ctypes.data, out["M"].ctypes.data, out["M2"].ctypes.data,
+        out["beta"].ctypes.data, out["pnOps"].ctypes.data, C.byref(nnz), 0, None, None,
+        rowptr.ctypes.data))
+    val = np.zeros(max(nnz.value, 1))
+    col = np.zeros(max(nnz.value, 1), dtype=np.int32)
+    check("c3d_tables_host", L.c3d_tables_host(
+        N, Q, None, None, None, None, None, None, None, None, C.byref(nnz), nnz.value,
+        val.ctypes.data, col.ctypes.data, None))
+    out["nnz"] = nnz.value
+    out["betaVal"], out["betaColInd"], out["betaRowPtr"] = val[:nnz.value], col[:nnz.value], rowptr
+    return out

@@ -1,0 +1,98 @@
+// c3d_internal.h -- context layout and kernel launchers shared by the HIP sources of
+// libc3d_hip.so.  Not part of the public ABI (include/c3d_hip.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+#include "../../include/c3d_hip.h"
+
+namespace c3d {
+
+constexpr int kWave = 64;
+constexpr int kChunk = 64;  // ordinates per LDS chunk in the closure sweeps
+
+// Host-side tables (init.cpp:423-548, 287-337, 199-234), built by tables.cpp or copied from
+// the caller.
+struct HostTables {
+    int N = 0, Q = 0, nm = 0, nm2 = 0, nq = 0, nnz = 0;
+    std::vector<double> w, xi, eta, mu, M, M2, beta, betaVal, pnOps;
+    std::vector<int32_t> betaColInd, betaRowPtr;
+};
+
+void build_tables(int N, int Q, HostTables &t);  // tables.cpp
+void build_pn_operators(HostTables &t);          // tables.cpp (needs w, xi, eta, mu, M)
+
+// Constants the closure needs, computed on the host once (libm), so that device and CPU see
+// the same bits.
+struct ClosureConsts {
+    double m0;         // M[0][0] = 1/(2 sqrt(pi)) up to rounding (fobj.cpp:123 relies on it)
+    double isoAlpha0;  // 2 sqrt(pi) log(0.5/sqrt(pi)), optBatch.cpp:117
+    double minf;       // sqrt((2p+1)/(2 pi)), p = floor(sqrt(nm))+1, optBatch.cpp:136-137
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace c3d
+
+struct c3d_ctx_s {
+    int device = 0;
+    int numCU = 0;
+    c3d::HostTables t;
+    c3d::ClosureConsts cc{};
+    int nqPad = 0;  // nq rounded up to kChunk
+    // device tables
+    double *d_w = nullptr, *d_xi = nullptr, *d_eta = nullptr, *d_mu = nullptr;
+    double *d_M = nullptr;     // [nqPad][nm], rows >= nq zero
+    double *d_wpad = nullptr;  // [nqPad], entries >= nq zero
+    double *d_M2 = nullptr;    // [nqPad][nm2]
+    double *d_beta = nullptr, *d_betaVal = nullptr, *d_pnOps = nullptr;
+    int32_t *d_betaColInd = nullptr, *d_betaRowPtr = nullptr;
+    // control words
+    unsigned long long *d_counter = nullptr;  // work queue head of the persistent closure
+    int *d_status = nullptr;                  // OR of per-cell error flags since c3d_sync
+    // scratch owned by the context, grown on demand
+    c3d::DevBuf ansatz, uOld, flux;
+    c3d::DevBuf closureState;  // per-cell Newton state, Hessians, worklists
+    int maxPasses = 100000;     // sweep passes per closure call before cells are reported
+    std::mutex mtx;
+};
+
+namespace c3d {
+
+hipError_t grow(DevBuf &b, size_t bytes);
+
+// closure_fp64.hip
+hipError_t launch_closure_fp64(c3d_ctx_s *ctx, int64_t ncells, double *u, double *alpha,
+                               const c3d_closure_opts &o, c3d_cell_stats *stats,
+                               hipStream_t s);
+hipError_t launch_dense_solve(int nm, int64_t n, const double *H, const double *g, double *d,
+                              double *rcond, int *info, double *L, int lapackOrder, hipStream_t s);
+hipError_t launch_hessian_fp64(c3d_ctx_s *ctx, int64_t ncells, const double *alpha,
+                               double *hs, hipStream_t s);
+
+// grid.hip
+hipError_t launch_ghost_fill(int nx, int ny, int nz, int nm, double *u, hipStream_t s);
+hipError_t launch_ansatz(c3d_ctx_s *ctx, int nx, int ny, int nz, const double *alpha,
+                         double *A, int isPn, hipStream_t s);
+hipError_t launch_stencil(c3d_ctx_s *ctx, int nx, int ny, int nz, const double *A,
+                          double *flux, double *u, const double *sigmaT,
+                          const double *sigmaS, double dt, double dx, double dy, double dz,
+                          double theta, int fuseEuler, int zeroGhosts, hipStream_t s);
+hipError_t launch_flux_moment(c3d_ctx_s *ctx, int nx, int ny, int nz, const double *u,
+                              double *flux, double dx, double dy, double dz, hipStream_t s);
+hipError_t launch_euler(int nx, int ny, int nz, int nm, double *u, const double *flux,
+                        const double *sigmaT, const double *sigmaS, double dt, hipStream_t s);
+hipError_t launch_rk2(int nx, int ny, int nz, int nm, double *u, double *uOld, int kind,
+                      hipStream_t s);
+hipError_t launch_make_batch(c3d_ctx_s *ctx, int64_t n, uint64_t seed, double smin,
+                             double smax, double *u, double *alpha0, double *alphaStar,
+                             hipStream_t s);
+
+}  // namespace c3d

@@ -1,0 +1,538 @@
+// capi.hip -- the extern "C" surface of libc3d_hip.so (declared in include/c3d_hip.h).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "c3d_internal.h"
+
+namespace c3d {
+
+hipError_t grow(DevBuf &b, size_t bytes)
+{
+    if (b.bytes >= bytes)
+        return hipSuccess;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        if (e != hipSuccess)
+            return e;
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e == hipSuccess)
+        b.bytes = bytes;
+    return e;
+}
+
+namespace {
+
+template <class T>
+hipError_t upload(T **dst, const T *src, size_t n)
+{
+    hipError_t e = hipMalloc((void **)dst, (n ? n : 1) * sizeof(T));
+    if (e != hipSuccess || n == 0)
+        return e;
+    return hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice);
+}
+
+int finish_ctx(c3d_ctx_s *ctx)
+{
+    HostTables &t = ctx->t;
+    if (hipGetDevice(&ctx->device) != hipSuccess)
+        return C3D_ERR_HIP;
+    if (hipDeviceGetAttribute(&ctx->numCU, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+        hipSuccess)
+        return C3D_ERR_HIP;
+    ctx->cc.m0 = t.M[0];
+    ctx->cc.isoAlpha0 = 2.0 * sqrt(M_PI) * log(0.5 / sqrt(M_PI));
+    {
+        int pnOrder = (int)floor(sqrt((double)t.nm)) + 1;
+        ctx->cc.minf = sqrt((2 * pnOrder + 1) / (2 * M_PI));
+    }
+    ctx->nqPad = (t.nq + kChunk - 1) / kChunk * kChunk;
+    std::vector<double> Mp((size_t)ctx->nqPad * t.nm, 0.0), wp(ctx->nqPad, 0.0);
+    std::memcpy(Mp.data(), t.M.data(), t.M.size() * sizeof(double));
+    std::memcpy(wp.data(), t.w.data(), t.w.size() * sizeof(double));
+    hipError_t e = hipSuccess;
+#define UP(dst, vec)                                                                           \
+    if (e == hipSuccess)                                                                       \
+    e = upload(&ctx->dst, vec.data(), vec.size())
+    UP(d_w, t.w);
+    UP(d_xi, t.xi);
+    UP(d_eta, t.eta);
+    UP(d_mu, t.mu);
+    UP(d_M, Mp);
+    UP(d_wpad, wp);
+    if (!t.M2.empty()) {
+        std::vector<double> M2p((size_t)ctx->nqPad * t.nm2, 0.0);
+        std::memcpy(M2p.data(), t.M2.data(), t.M2.size() * sizeof(double));
+        UP(d_M2, M2p);
+    }
+    UP(d_beta, t.beta);
+    UP(d_betaVal, t.betaVal);
+    UP(d_betaColInd, t.betaColInd);
+    UP(d_betaRowPtr, t.betaRowPtr);
+    UP(d_pnOps, t.pnOps);
+#undef UP
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&ctx->d_counter, sizeof(unsigned long long));
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&ctx->d_status, sizeof(int));
+    if (e == hipSuccess)
+        e = hipMemset(ctx->d_status, 0, sizeof(int));
+    if (e == hipSuccess)
+        e = hipDeviceSynchronize();
+    return e == hipSuccess ? C3D_OK : C3D_ERR_HIP;
+}
+
+bool valid_order(int N) { return N >= 1 && N <= 4; }
+
+int hip_status(hipError_t e) { return e == hipSuccess ? C3D_OK : C3D_ERR_HIP; }
+
+}  // namespace
+}  // namespace c3d
+
+using namespace c3d;
+
+extern "C" {
+
+int c3d_create(c3d_ctx *out, int moment_order, int quad_order)
+{
+    if (!out || !valid_order(moment_order) || quad_order < 1 || quad_order > 512)
+        return C3D_ERR_ARG;
+    *out = nullptr;
+    c3d_ctx_s *ctx = new c3d_ctx_s();
+    build_tables(moment_order, quad_order, ctx->t);
+    int rc = finish_ctx(ctx);
+    if (rc != C3D_OK) {
+        c3d_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return C3D_OK;
+}
+
+int c3d_create_from_tables(c3d_ctx *out, int moment_order, int quad_order, const double *w,
+                           const double *xi, const double *eta, const double *mu,
+                           const double *M, const double *M2, const double *beta, int nnz,
+                           const double *betaVal, const int32_t *betaColInd,
+                           const int32_t *betaRowPtr)
+{
+    if (!out || !valid_order(moment_order) || quad_order < 1 || !w || !xi || !eta || !mu || !M)
+        return C3D_ERR_ARG;
+    *out = nullptr;
+    c3d_ctx_s *ctx = new c3d_ctx_s();
+    HostTables &t = ctx->t;
+    t.N = moment_order;
+    t.Q = quad_order;
+    t.nm = (moment_order + 1) * (moment_order + 1);
+    t.nm2 = (2 * moment_order + 1) * (2 * moment_order + 1);
+    t.nq = 2 * quad_order * quad_order;
+    t.w.assign(w, w + t.nq);
+    t.xi.assign(xi, xi + t.nq);
+    t.eta.assign(eta, eta + t.nq);
+    t.mu.assign(mu, mu + t.nq);
+    t.M.assign(M, M + (size_t)t.nq * t.nm);
+    if (M2)
+        t.M2.assign(M2, M2 + (size_t)t.nq * t.nm2);
+    if (beta)
+        t.beta.assign(beta, beta + (size_t)t.nm * t.nm * t.nm2);
+    if (betaVal && betaColInd && betaRowPtr && nnz > 0) {
+        t.nnz = nnz;
+        t.betaVal.assign(betaVal, betaVal + nnz);
+        t.betaColInd.assign(betaColInd, betaColInd + nnz);
+        t.betaRowPtr.assign(betaRowPtr, betaRowPtr + t.nm * t.nm + 1);
+    }
+    build_pn_operators(t);
+    int rc = finish_ctx(ctx);
+    if (rc != C3D_OK) {
+        c3d_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return C3D_OK;
+}
+
+int c3d_destroy(c3d_ctx ctx)
+{
+    if (!ctx)
+        return C3D_OK;
+    void *ptrs[] = {ctx->d_w, ctx->d_xi, ctx->d_eta, ctx->d_mu, ctx->d_M, ctx->d_wpad,
+                    ctx->d_M2, ctx->d_beta, ctx->d_betaVal, ctx->d_pnOps, ctx->d_betaColInd,
+                    ctx->d_betaRowPtr, ctx->d_counter, ctx->d_status, ctx->ansatz.p,
+                    ctx->uOld.p, ctx->flux.p, ctx->closureState.p};
+    for (void *p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    delete ctx;
+    return C3D_OK;
+}
+
+int c3d_tables_host(int moment_order, int quad_order, double *w, double *xi, double *eta,
+                    double *mu, double *M, double *M2, double *beta, double *pnOps, int *nnz,
+                    int nnzCap, double *betaVal, int32_t *betaColInd, int32_t *betaRowPtr)
+{
+    if (!valid_order(moment_order) || quad_order < 1 || quad_order > 512)
+        return C3D_ERR_ARG;
+    HostTables t;
+    build_tables(moment_order, quad_order, t);
+    auto cp = [](auto *dst, const auto &v) {
+        if (dst && !v.empty())
+            std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    cp(w, t.w);
+    cp(xi, t.xi);
+    cp(eta, t.eta);
+    cp(mu, t.mu);
+    cp(M, t.M);
+    cp(M2, t.M2);
+    cp(beta, t.beta);
+    cp(pnOps, t.pnOps);
+    if (nnz)
+        *nnz = t.nnz;
+    if (betaRowPtr)
+        cp(betaRowPtr, t.betaRowPtr);
+    if (betaVal && betaColInd && nnzCap >= t.nnz) {
+        cp(betaVal, t.betaVal);
+        cp(betaColInd, t.betaColInd);
+    }
+    return C3D_OK;
+}
+
+int c3d_sizes(c3d_ctx ctx, int *nm, int *nm2, int *nq, int *nnz)
+{
+    if (!ctx)
+        return C3D_ERR_ARG;
+    if (nm)
+        *nm = ctx->t.nm;
+    if (nm2)
+        *nm2 = ctx->t.nm2;
+    if (nq)
+        *nq = ctx->t.nq;
+    if (nnz)
+        *nnz = ctx->t.nnz;
+    return C3D_OK;
+}
+
+int c3d_get_tables(c3d_ctx ctx, double *w, double *xi, double *eta, double *mu, double *M,
+                   double *M2, double *beta, double *betaVal, int32_t *betaColInd,
+                   int32_t *betaRowPtr, double *pnOps)
+{
+    if (!ctx)
+        return C3D_ERR_ARG;
+    const HostTables &t = ctx->t;
+    auto cp = [](auto *dst, const auto &v) {
+        if (dst && !v.empty())
+            std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    cp(w, t.w);
+    cp(xi, t.xi);
+    cp(eta, t.eta);
+    cp(mu, t.mu);
+    cp(M, t.M);
+    cp(M2, t.M2);
+    cp(beta, t.beta);
+    cp(betaVal, t.betaVal);
+    cp(betaColInd, t.betaColInd);
+    cp(betaRowPtr, t.betaRowPtr);
+    cp(pnOps, t.pnOps);
+    return C3D_OK;
+}
+
+int c3d_closure(c3d_ctx ctx, int64_t ncells, double *u, double *alpha,
+                const c3d_closure_opts *opts, c3d_cell_stats *stats, void *stream)
+{
+    if (!ctx || !opts || ncells < 0 || (ncells > 0 && (!u || !alpha)))
+        return C3D_ERR_ARG;
+    if (opts->useGaunt)
+        return C3D_ERR_UNSUPPORTED;
+    if (opts->maxIter < 1)
+        return C3D_ERR_ARG;
+    return hip_status(launch_closure_fp64(ctx, ncells, u, alpha, *opts, stats,
+                                          (hipStream_t)stream));
+}
+
+int c3d_set_pass_budget(c3d_ctx ctx, int max_passes)
+{
+    if (!ctx || max_passes < 1)
+        return C3D_ERR_ARG;
+    ctx->maxPasses = max_passes;
+    return C3D_OK;
+}
+
+int c3d_hessian(c3d_ctx ctx, int64_t ncells, const double *alpha, double *hs, int useGaunt,
+                int useGauntSparse, void *stream)
+{
+    (void)useGauntSparse;
+    if (!ctx || ncells < 0 || (ncells > 0 && (!alpha || !hs)))
+        return C3D_ERR_ARG;
+    if (useGaunt)
+        return C3D_ERR_UNSUPPORTED;
+    return hip_status(launch_hessian_fp64(ctx, ncells, alpha, hs, (hipStream_t)stream));
+}
+
+int c3d_dense_solve(int nm, int64_t n, const double *H, const double *g, double *d,
+                    double *rcond, int32_t *info, double *L, int lapackOrder, void *stream)
+{
+    if (n < 0 || (n > 0 && (!H || !g || !d || !rcond || !info || !L)))
+        return C3D_ERR_ARG;
+    if (nm != 4 && nm != 9 && nm != 16 && nm != 25)
+        return C3D_ERR_ARG;
+    return hip_status(launch_dense_solve(nm, n, H, g, d, rcond, info, L, lapackOrder,
+                                         (hipStream_t)stream));
+}
+
+int c3d_sync(c3d_ctx ctx, void *stream)
+{
+    if (!ctx)
+        return C3D_ERR_ARG;
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess)
+        return C3D_ERR_HIP;
+    int st = 0;
+    if (hipMemcpy(&st, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return C3D_ERR_HIP;
+    if (st != 0 && hipMemset(ctx->d_status, 0, sizeof(int)) != hipSuccess)
+        return C3D_ERR_HIP;
+    if (st & C3D_FLAG_U0_NONPOS)
+        return C3D_ERR_U0;
+    if (st & C3D_FLAG_LADDER)
+        return C3D_ERR_LADDER;
+    if (st & C3D_FLAG_BUDGET)
+        return C3D_ERR_BUDGET;
+    return C3D_OK;
+}
+
+int c3d_ghost_fill(c3d_ctx ctx, int nx, int ny, int nz, double *u, void *stream)
+{
+    if (!ctx || nx < 2 || ny < 2 || nz < 2 || !u)
+        return C3D_ERR_ARG;
+    return hip_status(launch_ghost_fill(nx, ny, nz, ctx->t.nm, u, (hipStream_t)stream));
+}
+
+int c3d_flux_kinetic(c3d_ctx ctx, int nx, int ny, int nz, const double *alpha, double *flux,
+                     double dx, double dy, double dz, double theta, int isPn, void *stream)
+{
+    if (!ctx || nx < 1 || ny < 1 || nz < 1 || !alpha || !flux)
+        return C3D_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    const size_t ng = (size_t)(nx + 4) * (ny + 4) * (nz + 4);
+    if (grow(ctx->ansatz, ng * ctx->t.nq * sizeof(double)) != hipSuccess)
+        return C3D_ERR_HIP;
+    hipStream_t s = (hipStream_t)stream;
+    double *A = (double *)ctx->ansatz.p;
+    hipError_t e = launch_ansatz(ctx, nx, ny, nz, alpha, A, isPn, s);
+    if (e == hipSuccess)
+        e = launch_stencil(ctx, nx, ny, nz, A, flux, nullptr, nullptr, nullptr, 0.0, dx, dy, dz,
+                           theta, 0, 0, s);
+    return hip_status(e);
+}
+
+int c3d_flux_moment(c3d_ctx ctx, int nx, int ny, int nz, const double *u, double *flux,
+                    double dx, double dy, double dz, void *stream)
+{
+    if (!ctx || nx < 1 || ny < 1 || nz < 1 || !u || !flux)
+        return C3D_ERR_ARG;
+    return hip_status(launch_flux_moment(ctx, nx, ny, nz, u, flux, dx, dy, dz,
+                                         (hipStream_t)stream));
+}
+
+int c3d_euler(c3d_ctx ctx, int nx, int ny, int nz, double *u, const double *flux,
+              const double *sigmaT, const double *sigmaS, double dt, void *stream)
+{
+    if (!ctx || nx < 1 || ny < 1 || nz < 1 || !u || !flux || !sigmaT || !sigmaS)
+        return C3D_ERR_ARG;
+    return hip_status(launch_euler(nx, ny, nz, ctx->t.nm, u, flux, sigmaT, sigmaS, dt,
+                                   (hipStream_t)stream));
+}
+
+int c3d_rk2_combine(c3d_ctx ctx, int nx, int ny, int nz, double *u, double *uOld, int kind,
+                    void *stream)
+{
+    if (!ctx || nx < 1 || ny < 1 || nz < 1 || !u || !uOld || (kind != 0 && kind != 1))
+        return C3D_ERR_ARG;
+    return hip_status(launch_rk2(nx, ny, nz, ctx->t.nm, u, uOld, kind, (hipStream_t)stream));
+}
+
+int c3d_update(c3d_ctx ctx, int nx, int ny, int nz, double *u, double *alpha,
+               const double *sigmaT, const double *sigmaS, double dx, double dy, double dz,
+               double theta, double dt, int solver, int pnUseKinetic,
+               const c3d_closure_opts *opts, c3d_cell_stats *stats, void *stream)
+{
+    if (!ctx || nx < 2 || ny < 2 || nz < 2 || !u || !sigmaT || !sigmaS)
+        return C3D_ERR_ARG;
+    if (solver == 1 && (!alpha || !opts || opts->useGaunt || opts->maxIter < 1))
+        return (opts && opts->useGaunt) ? C3D_ERR_UNSUPPORTED : C3D_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    const int nm = ctx->t.nm;
+    const size_t ng = (size_t)(nx + 4) * (ny + 4) * (nz + 4);
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = grow(ctx->uOld, ng * nm * sizeof(double));
+    const bool needAnsatz = solver == 1 || pnUseKinetic;
+    if (e == hipSuccess && needAnsatz)
+        e = grow(ctx->ansatz, ng * ctx->t.nq * sizeof(double));
+    if (e == hipSuccess && !needAnsatz)
+        e = grow(ctx->flux, ng * nm * sizeof(double));
+    if (e != hipSuccess)
+        return C3D_ERR_HIP;
+    double *uOld = (double *)ctx->uOld.p, *A = (double *)ctx->ansatz.p,
+           *flux = (double *)ctx->flux.p;
+    e = launch_rk2(nx, ny, nz, nm, u, uOld, 0, s);
+    for (int st = 0; st < 2 && e == hipSuccess; ++st) {
+        e = launch_ghost_fill(nx, ny, nz, nm, u, s);
+        if (e != hipSuccess)
+            break;
+        if (solver == 1) {
+            e = launch_closure_fp64(ctx, (int64_t)ng, u, alpha, *opts,
+                                    stats ? stats + st * ng : nullptr, s);
+            if (e == hipSuccess)
+                e = launch_ansatz(ctx, nx, ny, nz, alpha, A, 0, s);
+            if (e == hipSuccess)
+                e = launch_stencil(ctx, nx, ny, nz, A, nullptr, u, sigmaT, sigmaS, dt, dx, dy, dz,
+                                   theta, 1, 0, s);
+        } else if (!pnUseKinetic) {
+            e = launch_flux_moment(ctx, nx, ny, nz, u, flux, dx, dy, dz, s);
+            if (e == hipSuccess)
+                e = launch_euler(nx, ny, nz, nm, u, flux, sigmaT, sigmaS, dt, s);
+        } else {
+            e = launch_ansatz(ctx, nx, ny, nz, u, A, 1, s);
+            if (e == hipSuccess)
+                e = launch_stencil(ctx, nx, ny, nz, A, nullptr, u, sigmaT, sigmaS, dt, dx, dy, dz,
+                                   theta, 1, 0, s);
+        }
+    }
+    if (e == hipSuccess)
+        e = launch_rk2(nx, ny, nz, nm, u, uOld, 1, s);
+    return hip_status(e);
+}
+
+int c3d_make_closure_batch(c3d_ctx ctx, int64_t ncells, uint64_t seed, double smin,
+                           double smax, double *u, double *alpha0, double *alphaStar,
+                           void *stream)
+{
+    if (!ctx || ncells < 0 || (ncells > 0 && (!u || !alpha0)))
+        return C3D_ERR_ARG;
+    return hip_status(launch_make_batch(ctx, ncells, seed, smin, smax, u, alpha0, alphaStar,
+                                        (hipStream_t)stream));
+}
+
+/* ------------------------------------------------------------------------------
+ * Drop-ins for src/cuda/cuda_headers.h.  Like the reference, these keep their device
+ * state in file statics created on first use and block on every call.
+ * ------------------------------------------------------------------------------ */
+static std::mutex g_dropin_mtx;
+static c3d_ctx g_h_ctx = nullptr;  // solveHInit_hip / solveH_hip
+static c3d_ctx g_f_ctx = nullptr;  // solveFluxKinetic_hip
+static int g_f_nm = 0, g_f_nq = 0;
+
+static int order_from_nm(int nm)
+{
+    int N = (int)std::lround(std::sqrt((double)nm)) - 1;
+    return ((N + 1) * (N + 1) == nm) ? N : -1;
+}
+static int quad_from_nq(int nq)
+{
+    int Q = (int)std::lround(std::sqrt(nq / 2.0));
+    return (2 * Q * Q == nq) ? Q : -1;
+}
+
+void solveHInit_hip(int nm, int nm2, int nq, int batchSize, double *w, double *p, double *p2,
+                    double *beta, double *betaVal, int *betaColInd, int *betaRowPtr,
+                    int useGaunt)
+{
+    (void)nm2;
+    (void)batchSize;
+    std::lock_guard<std::mutex> lk(g_dropin_mtx);
+    int N = order_from_nm(nm), Q = quad_from_nq(nq);
+    if (N < 1 || Q < 1) {
+        std::fprintf(stderr, "solveHInit_hip: unsupported nm=%d nq=%d\n", nm, nq);
+        return;
+    }
+    if (g_h_ctx)
+        c3d_destroy(g_h_ctx);
+    g_h_ctx = nullptr;
+    // xi/eta/mu are not needed for H; pass w as a placeholder of the right length
+    int nnz = (useGaunt && betaRowPtr) ? betaRowPtr[nm * nm] : 0;
+    int rc = c3d_create_from_tables(&g_h_ctx, N, Q, w, w, w, w, p, p2, useGaunt ? beta : nullptr,
+                                    nnz, betaVal, betaColInd, betaRowPtr);
+    if (rc != C3D_OK)
+        std::fprintf(stderr, "solveHInit_hip: error %d\n", rc);
+}
+
+void solveH_hip(int nm, int nm2, int nq, int batchSize, double *alphas, double *hs,
+                int useGaunt, int useGauntSparse)
+{
+    (void)nm2;
+    (void)nq;
+    std::lock_guard<std::mutex> lk(g_dropin_mtx);
+    if (!g_h_ctx || g_h_ctx->t.nm != nm || batchSize <= 0)
+        return;
+    double *dA = nullptr, *dH = nullptr;
+    size_t na = (size_t)batchSize * nm, nh = (size_t)batchSize * nm * nm;
+    if (hipMalloc(&dA, na * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dH, nh * sizeof(double)) != hipSuccess) {
+        std::fprintf(stderr, "solveH_hip: allocation failed\n");
+    } else if (hipMemcpy(dA, alphas, na * sizeof(double), hipMemcpyHostToDevice) == hipSuccess) {
+        int rc = c3d_hessian(g_h_ctx, batchSize, dA, dH, useGaunt, useGauntSparse, nullptr);
+        if (rc == C3D_OK && hipMemcpy(hs, dH, nh * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
+        } else {
+            std::fprintf(stderr, "solveH_hip: error %d\n", rc);
+        }
+    }
+    if (dA)
+        (void)hipFree(dA);
+    if (dH)
+        (void)hipFree(dH);
+}
+
+void solveFluxKinetic_hip(int nx, int ny, int nz, int nm, int nq, double dx, double dy,
+                          double dz, double *alpha, double *flux, double *xi, double *eta,
+                          double *mu, double *w, double *p, int isPn)
+{
+    std::lock_guard<std::mutex> lk(g_dropin_mtx);
+    int N = order_from_nm(nm), Q = quad_from_nq(nq);
+    if (N < 1 || Q < 1 || nx < 5 || ny < 5 || nz < 5) {
+        std::fprintf(stderr, "solveFluxKinetic_hip: unsupported sizes\n");
+        return;
+    }
+    if (!g_f_ctx || g_f_nm != nm || g_f_nq != nq) {
+        if (g_f_ctx)
+            c3d_destroy(g_f_ctx);
+        g_f_ctx = nullptr;
+        if (c3d_create_from_tables(&g_f_ctx, N, Q, w, xi, eta, mu, p, nullptr, nullptr, 0,
+                                   nullptr, nullptr, nullptr) != C3D_OK) {
+            std::fprintf(stderr, "solveFluxKinetic_hip: context creation failed\n");
+            return;
+        }
+        g_f_nm = nm;
+        g_f_nq = nq;
+    }
+    const size_t n = (size_t)nx * ny * nz * nm;
+    double *dA = nullptr, *dF = nullptr;
+    if (hipMalloc(&dA, n * sizeof(double)) == hipSuccess &&
+        hipMalloc(&dF, n * sizeof(double)) == hipSuccess &&
+        hipMemcpy(dA, alpha, n * sizeof(double), hipMemcpyHostToDevice) == hipSuccess) {
+        std::lock_guard<std::mutex> lk2(g_f_ctx->mtx);
+        const int ix = nx - 4, iy = ny - 4, iz = nz - 4;
+        const size_t ng = (size_t)nx * ny * nz;
+        hipError_t e = grow(g_f_ctx->ansatz, ng * nq * sizeof(double));
+        double *A = (double *)g_f_ctx->ansatz.p;
+        if (e == hipSuccess)
+            e = launch_ansatz(g_f_ctx, ix, iy, iz, dA, A, isPn, nullptr);
+        if (e == hipSuccess)
+            e = launch_stencil(g_f_ctx, ix, iy, iz, A, dF, nullptr, nullptr, nullptr, 0.0, dx, dy,
+                               dz, 2.0, 0, 1, nullptr);
+        if (e == hipSuccess)
+            e = hipMemcpy(flux, dF, n * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess)
+            std::fprintf(stderr, "solveFluxKinetic_hip: %s\n", hipGetErrorString(e));
+    } else {
+        std::fprintf(stderr, "solveFluxKinetic_hip: allocation/copy failed\n");
+    }
+    if (dA)
+        (void)hipFree(dA);
+    if (dF)
+        (void)hipFree(dF);
+}
+
+}  // extern "C"

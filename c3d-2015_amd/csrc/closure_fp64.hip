@@ -1,0 +1,972 @@
+// closure_fp64.hip -- the M_N dual Newton closure in FP64 on gfx950, as a sequence of
+// bounded passes over a compacted worklist of unconverged cells.
+//
+// Semantics: Optimization::opt with solveTheRest (src/opt/optBatch.cpp:143-282, 312-344,
+// 350-472) or opt()/optScaled (src/opt/opt.cpp:139-355); the Hessian of solveHMn
+// (src/opt/fobj.cpp:34-102); fobjMnUseH (fobj.cpp:171-185); the Armijo line search
+// (src/opt/linesearch.cpp:29-81); LAPACK dpotrf/dpocon/dpotrs and BLAS dnrm2 restated
+// lane-parallel below.
+//
+// One pass = two kernels over the current worklist W (cell ids):
+//   k_sweep  evaluates, for every listed cell at its current alpha, e_q = exp(alpha.m_q),
+//            sum_q w_q e_q and H = M^T diag(w e) M.  A wavefront takes 4 cells; the block's
+//            16 cells share every M chunk staged in LDS, each lane computes one ordinate's
+//            exponent for the 4 cells in the reference's sequential order, and the four
+//            Hessians accumulate on the matrix cores (V_MFMA_F64_16X16X4_F64, 4 ordinates
+//            per instruction, A = (w e m_q)^T, B = m_q; nm = 25 uses the three 16x16 tiles
+//            of the lower block triangle).  Operand lanes: A[i][k], B[k][j] sit in lane
+//            16k + i (16k + j); accumulator register r of lane l holds D[l/16 + 4r][l%16].
+//   k_step   advances each listed cell by exactly one step of its state machine with that
+//            sweep's result: either the Armijo test of a line-search trial, or one Newton
+//            iteration (Cholesky in dpotf2 order, the dlacn2/dtrsv condition estimate, the
+//            restart/ladder decisions, the solve of utils.cpp:71-86, dnrm2, modcfl, line-
+//            search setup).  Row i of H lives in lane i; every scalar is reduced in the
+//            CPU's sequential order without FMA contraction.  Unfinished cells append
+//            themselves to the next worklist.
+// Every loop in both kernels is bounded by sizes fixed at launch; the host caps the number
+// of passes, so the closure always drains.
+#include <cfloat>
+
+#include "c3d_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace c3d {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kCellsPerWave = 4;
+constexpr int kNumReg = 5;
+__constant__ double kReg[kNumReg] = {0.0, 1e-3, 1e-2, 1e-1, 5e-1};  // opt.h:37-38
+
+enum : int { kPhaseH = 0, kPhaseLS = 1 };
+
+// per-cell state, [ncells] of each; vectors are [ncells][kVec][nm]
+enum : int { kVa = 0, kVa0, kVaOld, kVd, kVg, kVu, kVuOrig, kVec };
+enum : int { kSu0 = 0, kSf, kStls, kStprev, kSmaxF, kSgd, kSerr, kSgamma, kSr, kSrtol, kScal };
+enum : int { kIiter = 0, kItot, kIgam, kIreg, kIflags, kIphase, kImaxIt, kInt };
+
+struct State {
+    double *vec;   // [ncells][kVec][nm]
+    double *scal;  // [ncells][kScal]
+    int *ints;     // [ncells][kInt]
+    double *H;     // [ncells][nm][nm]
+    double *fdir;  // [ncells]
+};
+
+struct KParams {
+    int64_t ncells;
+    int nqPad, maxIter, unbatched;
+    double tol, tolGamma, condHMax;
+    double m0, isoAlpha0, minf;
+};
+
+__device__ __forceinline__ double rdl(double v, int src)
+{
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        v = v + __shfl_xor(v, o);
+    return v;
+}
+
+template <int NM>
+struct LaneVec {
+    static __device__ __forceinline__ double dasum(double x)
+    {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i)
+            s = s + fabs(rdl(x, i));
+        return s;
+    }
+    static __device__ __forceinline__ int idamax(double x)
+    {
+        int im = 0;
+        double m = fabs(rdl(x, 0));
+#pragma unroll
+        for (int i = 1; i < NM; ++i) {
+            double v = fabs(rdl(x, i));
+            if (v > m) {
+                m = v;
+                im = i;
+            }
+        }
+        return im;
+    }
+    static __device__ __forceinline__ double dnrm2(double x)  // reference BLAS scaling
+    {
+        double scale = 0.0, ssq = 1.0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            double xi = rdl(x, i);
+            if (xi != 0.0) {
+                double a = fabs(xi);
+                if (scale < a) {
+                    double r = scale / a;
+                    ssq = 1.0 + ssq * (r * r);
+                    scale = a;
+                } else {
+                    double r = a / scale;
+                    ssq = ssq + r * r;
+                }
+            }
+        }
+        return scale * sqrt(ssq);
+    }
+};
+
+// Wave-resident factor: lane i < NM holds row i (L after Cholesky; L(i,i) in row[i]) and
+// col[j] = L(j,i) for j > i, for the transposed solves.
+template <int NM>
+struct Dense {
+    double row[NM];
+    double col[NM];
+
+    __device__ __forceinline__ double diag(int j) const { return rdl(row[j], j); }
+
+    // dpotf2 'L': ajj = a(j,j) - ddot; column by dgemv then dscal(1/ajj)
+    __device__ __forceinline__ int cholesky(int lane)
+    {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < j; ++k)
+                s = s + row[k] * row[k];
+            double ajj = rdl(row[j] - s, j);
+            if (!(ajj > 0.0))
+                return j + 1;
+            ajj = sqrt(ajj);
+            double y = row[j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) {
+                double temp = -rdl(row[k], j);
+                y = y + temp * row[k];
+            }
+            double rr = 1.0 / ajj;
+            if (lane == j)
+                row[j] = ajj;
+            else if (lane > j)
+                row[j] = rr * y;
+        }
+        return 0;
+    }
+
+    __device__ __forceinline__ void build_columns(double *sT, int ld, int lane)
+    {
+        if (lane < NM) {
+#pragma unroll
+            for (int j = 0; j < NM; ++j)
+                if (j <= lane)
+                    sT[lane * ld + j] = row[j];
+        }
+        wave_lds_sync();
+        if (lane < NM) {
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                col[i] = (i > lane) ? sT[i * ld + lane] : 0.0;
+        } else {
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                col[i] = 0.0;
+        }
+        wave_lds_sync();
+    }
+
+    __device__ __forceinline__ void trsv_ln(double &x, int lane)  // dtrsv('L','N','N')
+    {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+            double xj = rdl(x, j);
+            if (xj != 0.0) {
+                xj = xj / diag(j);
+                if (lane == j)
+                    x = xj;
+                else if (lane > j)
+                    x = x - xj * row[j];
+            }
+        }
+    }
+    __device__ __forceinline__ void trsv_lt(double &x, int lane)  // dtrsv('L','T','N')
+    {
+#pragma unroll
+        for (int j = NM - 1; j >= 0; --j) {
+            if (lane == j)
+                x = x / row[j];
+            double xj = rdl(x, j);
+            if (lane < j)
+                x = x - col[j] * xj;
+        }
+    }
+    __device__ __forceinline__ void potrs_custom(double &d, int lane)  // utils.cpp:71-86
+    {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+            if (lane == j)
+                d = d / row[j];
+            double dj = rdl(d, j);
+            if (lane > j)
+                d = d - row[j] * dj;
+        }
+#pragma unroll
+        for (int j = NM - 1; j >= 0; --j) {
+            if (lane == j)
+                d = d / row[j];
+            double dj = rdl(d, j);
+            if (lane < j)
+                d = d - col[j] * dj;
+        }
+    }
+    // LAPACK dpotrs('L'): dtrsm(L) then dtrsm(L^T), reference BLAS order (opt.cpp:285)
+    __device__ __forceinline__ void potrs_lapack(double &d, int lane)
+    {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            double dk = rdl(d, k);
+            if (dk != 0.0) {
+                dk = dk / diag(k);
+                if (lane == k)
+                    d = dk;
+                else if (lane > k)
+                    d = d - dk * row[k];
+            }
+        }
+#pragma unroll
+        for (int i = NM - 1; i >= 0; --i) {
+            double temp = rdl(d, i);
+#pragma unroll
+            for (int k = i + 1; k < NM; ++k)
+                temp = temp - rdl(col[k], i) * rdl(d, k);
+            temp = temp / diag(i);
+            if (lane == i)
+                d = temp;
+        }
+    }
+
+    // dpocon('L'): dlacn2 (Higham 1988, the Hager/Higham 1-norm estimator) driving
+    // dlatrs, which for these factors takes its dtrsv path.  At most 11 applications.
+    __device__ __forceinline__ double pocon(double anorm, int lane)
+    {
+        if (anorm == 0.0)
+            return 0.0;
+        const bool own = lane < NM;
+        double x = own ? 1.0 / (double)NM : 0.0;
+        int isgn = 0;
+        double est = 0.0;
+        int isave1 = 1, isave2 = 0, isave3 = 0;
+        for (int guard = 0; guard < 16; ++guard) {
+            trsv_ln(x, lane);
+            trsv_lt(x, lane);
+            bool l120 = false;
+            if (isave1 == 1) {
+                est = LaneVec<NM>::dasum(x);
+                if (own) {
+                    x = (x >= 0.0) ? 1.0 : -1.0;
+                    isgn = (x >= 0.0) ? 1 : -1;
+                }
+                isave1 = 2;
+                continue;
+            } else if (isave1 == 2) {
+                isave2 = LaneVec<NM>::idamax(x);
+                isave3 = 2;
+                x = (lane == isave2) ? 1.0 : 0.0;
+                isave1 = 3;
+                continue;
+            } else if (isave1 == 3) {
+                double estold = est;
+                est = LaneVec<NM>::dasum(x);
+                bool same = !own || (((x >= 0.0) ? 1 : -1) == isgn);
+                if (__all(same) || est <= estold) {
+                    l120 = true;
+                } else {
+                    if (own) {
+                        x = (x >= 0.0) ? 1.0 : -1.0;
+                        isgn = (x >= 0.0) ? 1 : -1;
+                    }
+                    isave1 = 4;
+                    continue;
+                }
+            } else if (isave1 == 4) {
+                int jlast = isave2;
+                isave2 = LaneVec<NM>::idamax(x);
+                if (rdl(x, jlast) != fabs(rdl(x, isave2)) && isave3 < 5) {
+                    ++isave3;
+                    x = (lane == isave2) ? 1.0 : 0.0;
+                    isave1 = 3;
+                    continue;
+                }
+                l120 = true;
+            } else {
+                double temp = 2.0 * (LaneVec<NM>::dasum(x) / (double)(3 * NM));
+                if (temp > est)
+                    est = temp;
+                break;
+            }
+            if (l120) {
+                x = own ? (((lane & 1) ? -1.0 : 1.0) * (1.0 + (double)lane / (double)(NM - 1)))
+                        : 0.0;
+                isave1 = 5;
+            }
+        }
+        return est != 0.0 ? (1.0 / est) / anorm : 0.0;
+    }
+};
+
+// ---------------------------------------------------------------- init pass
+// preProcess (optBatch.cpp:312-331 / opt.cpp:336-346) and the first worklist.
+template <int NM>
+__global__ __launch_bounds__(kBlock) void k_init(KParams P, const double *__restrict__ u_g,
+                                                  const double *__restrict__ alpha_g, State S,
+                                                  int *W, int *nW, c3d_cell_stats *stats,
+                                                  int *status)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (c >= P.ncells)
+        return;
+    const bool own = lane < NM;
+    double uu = own ? u_g[c * NM + lane] : 0.0;
+    double a = own ? alpha_g[c * NM + lane] : 0.0;
+    const double u0 = rdl(uu, 0);
+    const double alphaScale = log(u0) * 2.0 * sqrt(M_PI);
+    if (lane == 0)
+        a = a - alphaScale;
+    uu = uu / u0;
+    if (u0 <= 0) {
+        if (lane == 0) {
+            atomicOr(status, C3D_FLAG_U0_NONPOS);
+            if (stats) {
+                c3d_cell_stats s0{0.0, 0.0, 0.0, 0, 0, 0, C3D_FLAG_U0_NONPOS};
+                stats[c] = s0;
+            }
+        }
+        return;
+    }
+    double *v = S.vec + c * kVec * NM;
+    if (own) {
+        v[kVa * NM + lane] = a;
+        v[kVa0 * NM + lane] = a;
+        v[kVaOld * NM + lane] = a;
+        v[kVd * NM + lane] = 0.0;
+        v[kVg * NM + lane] = 0.0;
+        v[kVu * NM + lane] = uu;
+        v[kVuOrig * NM + lane] = uu;
+    }
+    const double rtol = P.unbatched ? LaneVec<NM>::dnrm2(own ? uu : 0.0) * P.tol : P.tol;
+    if (lane == 0) {
+        double *s = S.scal + c * kScal;
+        s[kSu0] = u0;
+        s[kSf] = 0.0;
+        s[kStls] = 1.0;
+        s[kStprev] = 1.0;
+        s[kSmaxF] = 0.0;
+        s[kSgd] = 0.0;
+        s[kSerr] = 0.0;
+        s[kSgamma] = 0.0;
+        s[kSr] = 0.0;
+        s[kSrtol] = rtol;
+        int *in = S.ints + c * kInt;
+        in[kIiter] = 0;
+        in[kItot] = 0;
+        in[kIgam] = 0;
+        in[kIreg] = 0;
+        in[kIflags] = 0;
+        in[kIphase] = kPhaseH;
+        in[kImaxIt] = P.maxIter;
+        W[atomicAdd(nW, 1)] = (int)c;
+    }
+}
+
+// ---------------------------------------------------------------- sweep pass
+template <int NM>
+__global__ __launch_bounds__(kBlock) void k_sweep(KParams P, const double *__restrict__ Mg,
+                                                   const double *__restrict__ wg, State S,
+                                                   const int *__restrict__ W,
+                                                   const int *__restrict__ nWp)
+{
+    constexpr int NT = (NM + 15) / 16;
+    constexpr int NB = NT * (NT + 1) / 2;
+    constexpr int NMS = NM | 1;
+    constexpr int CW = kCellsPerWave;
+    __shared__ double sM[kChunk * NMS];
+    __shared__ double sW[kChunk];
+    __shared__ double sA[kWavesPerBlock][CW][NM];
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nW = uniform_int(*nWp);
+    const int groups = (nW + CW - 1) / CW;                  // 4 cells per wave
+    const int nBlocksNeeded = (groups + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int gq = lane >> 4, rq = lane & 15;
+    for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
+        const int grp = blk * kWavesPerBlock + wv;
+        int cells[CW];
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            const int slot = grp * CW + c;
+            cells[c] = (grp < groups && slot < nW) ? W[slot] : -1;
+        }
+        // alphas of the wave's cells to LDS
+#pragma unroll
+        for (int c = 0; c < CW; ++c)
+            if (lane < NM)
+                sA[wv][c][lane] = cells[c] >= 0 ? S.vec[(size_t)cells[c] * kVec * NM + kVa * NM + lane] : 0.0;
+        d4 acc[CW][NB];
+#pragma unroll
+        for (int c = 0; c < CW; ++c)
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+                acc[c][b] = d4{0.0, 0.0, 0.0, 0.0};
+        double fpart[CW];
+#pragma unroll
+        for (int c = 0; c < CW; ++c)
+            fpart[c] = 0.0;
+        const bool any = cells[0] >= 0;
+        for (int ch = 0; ch < P.nqPad / kChunk; ++ch) {
+            __syncthreads();
+            for (int e = tid; e < kChunk * NM; e += kBlock) {
+                const int rr = e / NM, cc = e - rr * NM;
+                sM[rr * NMS + cc] = Mg[(size_t)ch * kChunk * NM + e];
+            }
+            if (tid < kChunk)
+                sW[tid] = wg[(size_t)ch * kChunk + tid];
+            __syncthreads();
+            if (any) {
+                const double *rowp = &sM[lane * NMS];
+                double we[CW];
+#pragma unroll
+                for (int c = 0; c < CW; ++c) {
+                    double x = 0.0;
+#pragma unroll
+                    for (int i = 0; i < NM; ++i)
+                        x = x + sA[wv][c][i] * rowp[i];
+                    we[c] = sW[lane] * exp(x);
+                    fpart[c] = fpart[c] + we[c];
+                }
+#pragma unroll
+                for (int k = 0; k < kChunk / 4; ++k) {
+                    const int ql = 4 * k + gq;
+                    const double *mr = &sM[ql * NMS];
+                    const double b0 = (rq < NM) ? mr[rq] : 0.0;
+                    const double b1 = (NT == 2 && 16 + rq < NM) ? mr[16 + rq] : 0.0;
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) {
+                        const double wq = __shfl(we[c], ql);
+                        acc[c][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(wq * b0, b0, acc[c][0], 0, 0, 0);
+                        if constexpr (NT == 2) {
+                            acc[c][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(wq * b1, b0, acc[c][1], 0, 0, 0);
+                            acc[c][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(wq * b1, b1, acc[c][2], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+        }
+        // results: lower triangle, mirrored (fobj.cpp:96-101)
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            const double fs = wave_sum(fpart[c]);
+            if (cells[c] < 0)
+                continue;
+            double *H = S.H + (size_t)cells[c] * NM * NM;
+            if (lane == 0)
+                S.fdir[cells[c]] = fs;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int R = (b == 0) ? 0 : 1, Cb = (b == 2) ? 1 : 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int i = 16 * R + gq + 4 * k, j = 16 * Cb + rq;
+                    if (i < NM && j < NM && j <= i) {
+                        H[i * NM + j] = acc[c][b][k];
+                        H[j * NM + i] = acc[c][b][k];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- step pass
+template <int NM>
+__global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, const int *__restrict__ W,
+                                                  const int *__restrict__ nWp, int *Wn, int *nWn,
+                                                  double *u_g, double *alpha_g,
+                                                  c3d_cell_stats *stats, int *status)
+{
+    constexpr int LD = NM + 1;
+    __shared__ double sT[kWavesPerBlock][NM * LD];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const bool own = lane < NM;
+    const int nW = uniform_int(*nWp);
+    const int nBlocksNeeded = (nW + kWavesPerBlock - 1) / kWavesPerBlock;
+    for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
+        const int slot = blk * kWavesPerBlock + wv;
+        if (slot >= nW)
+            continue;
+        const int c = uniform_int(W[slot]);
+        double *v = S.vec + (size_t)c * kVec * NM;
+        double *sc = S.scal + (size_t)c * kScal;
+        int *in = S.ints + (size_t)c * kInt;
+        auto ld = [&](int f) { return own ? v[f * NM + lane] : 0.0; };
+        double a = ld(kVa), a0 = ld(kVa0), aOld = ld(kVaOld), d = ld(kVd), g = ld(kVg),
+               uu = ld(kVu), uOrig = ld(kVuOrig);
+        const double u0 = sc[kSu0];
+        double f = sc[kSf], tls = sc[kStls], tprev = sc[kStprev], maxFactor = sc[kSmaxF],
+               gd = sc[kSgd], err = sc[kSerr], gamma = sc[kSgamma], r = sc[kSr],
+               rtol = sc[kSrtol];
+        int iter = in[kIiter], totalIter = in[kItot], iterGamma = in[kIgam],
+            regIndex = in[kIreg], flags = in[kIflags], phase = in[kIphase],
+            maxIterCur = in[kImaxIt];
+        const double tolGamma = P.unbatched ? 1.1 : P.tolGamma;
+        const double fdir = S.fdir[c];
+        int done = 0;  // 0 continue, 1 converged, 2 ladder exhausted
+
+        bool doH = true;
+        if (phase == kPhaseLS) {
+            double fNew = fdir;
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                fNew = fNew - rdl(a, i) * rdl(uu, i);
+            if (fNew <= f + 1e-3 * tls * gd) {
+                tprev = tls;  // accepted; H of this sweep belongs to the new alpha
+            } else {
+                tls = 0.5 * tls;
+                if (tls * maxFactor > DBL_EPSILON / 2) {
+                    a = own ? aOld + tls * d : 0.0;
+                    phase = kPhaseLS;
+                } else {
+                    a = aOld;
+                    tprev = 0.0;
+                    phase = kPhaseH;
+                }
+                doH = false;
+            }
+        }
+        if (doH) {
+            // ============================ one Newton iteration (solveTheRest)
+            Dense<NM> D;
+            const double *H = S.H + (size_t)c * NM * NM;
+#pragma unroll
+            for (int j = 0; j < NM; ++j)
+                D.row[j] = own ? H[lane * NM + j] : 0.0;
+            int maxIt = maxIterCur;
+            if (!P.unbatched && regIndex == kNumReg - 1)
+                maxIt = 10 * maxIterCur;
+            ++iter;
+            ++totalIter;
+            g = own ? D.row[0] / P.m0 - uu : 0.0;  // fobjMnUseH
+            f = rdl(D.row[0], 0) / (P.m0 * P.m0);
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                f = f - rdl(a, i) * rdl(uu, i);
+            double rs = 0.0;  // utils_norm1
+#pragma unroll
+            for (int j = 0; j < NM; ++j)
+                rs = rs + fabs(D.row[j]);
+            double norm1 = 0.0;
+#pragma unroll
+            for (int i = 0; i < NM; ++i) {
+                double vv = rdl(rs, i);
+                if (vv > norm1)
+                    norm1 = vv;
+            }
+            const int cholErr = D.cholesky(lane);
+            double condest = 0.0;
+            if (cholErr == 0) {
+                D.build_columns(sT[wv], LD, lane);
+                condest = 1.0 / D.pocon(norm1, lane);
+            }
+            if (totalIter == 1 && (cholErr != 0 || condest > P.condHMax)) {
+                a0 = (lane == 0) ? P.isoAlpha0 : 0.0;  // isotropic restart, optBatch.cpp:208-218
+                a = a0;
+                if (P.unbatched)
+                    rtol = LaneVec<NM>::dnrm2(uu) * P.tol;
+                iter = 0;
+                iterGamma = 0;
+                tprev = 1.0;
+                flags |= C3D_FLAG_ISO_RESTART;
+                phase = kPhaseH;
+            } else if (iter >= maxIt || cholErr != 0 || tprev == 0.0 || condest > P.condHMax) {
+                if (regIndex < kNumReg - 1) {  // ladder, optBatch.cpp:222-248
+                    ++regIndex;
+                    r = kReg[regIndex];
+                    uu = (lane == 0) ? 1.0 : (own ? uOrig * (1.0 - r) : 0.0);
+                    if (P.unbatched)
+                        rtol = LaneVec<NM>::dnrm2(uu) * P.tol;
+                    a = a0;
+                    iterGamma = 0;
+                    iter = 0;
+                    tprev = 1.0;
+                    if (P.unbatched && regIndex == kNumReg - 1)
+                        maxIterCur = 10 * maxIterCur;
+                    phase = kPhaseH;
+                } else {
+                    done = 2;
+                }
+            } else {
+                d = g;
+                if (P.unbatched)
+                    D.potrs_lapack(d, lane);
+                else
+                    D.potrs_custom(d, lane);
+                d = own ? -d : 0.0;
+                err = LaneVec<NM>::dnrm2(g);
+                double da1 = 0.0;
+#pragma unroll
+                for (int i = 0; i < NM; ++i)
+                    da1 = da1 + fabs(rdl(d, i));
+                gamma = exp(2.0 * da1 * P.minf);
+                if (err <= rtol && gamma <= tolGamma) {
+                    done = 1;
+                } else {
+                    if (err <= rtol && gamma > tolGamma)
+                        ++iterGamma;
+                    aOld = a;  // linesearch.cpp:29-54
+                    maxFactor = 0.0;
+#pragma unroll
+                    for (int i = 0; i < NM; ++i) {
+                        double temp = fabs(rdl(d, i)) / (fabs(rdl(aOld, i)) + 1e-100);
+                        if (temp > maxFactor)
+                            maxFactor = temp;
+                    }
+                    gd = 0.0;
+#pragma unroll
+                    for (int i = 0; i < NM; ++i)
+                        gd = gd + rdl(g, i) * rdl(d, i);
+                    tls = 1.0;
+                    if (tls * maxFactor > DBL_EPSILON / 2) {
+                        a = own ? aOld + tls * d : 0.0;
+                        phase = kPhaseLS;
+                    } else {
+                        a = aOld;
+                        tprev = 0.0;
+                        phase = kPhaseH;
+                    }
+                }
+            }
+        }
+        if (done) {
+            if (done == 2)
+                flags |= C3D_FLAG_LADDER;
+            const double alphaScale = log(u0) * 2.0 * sqrt(M_PI);  // postProcess
+            if (own) {
+                u_g[(size_t)c * NM + lane] = uu * u0;
+                alpha_g[(size_t)c * NM + lane] = (lane == 0) ? a + alphaScale : a;
+            }
+            if (lane == 0) {
+                if (done == 2)
+                    atomicOr(status, C3D_FLAG_LADDER);
+                if (stats) {
+                    c3d_cell_stats s0{r, err, gamma, totalIter, iterGamma, regIndex, flags};
+                    stats[c] = s0;
+                }
+            }
+        } else {
+            if (own) {
+                v[kVa * NM + lane] = a;
+                v[kVa0 * NM + lane] = a0;
+                v[kVaOld * NM + lane] = aOld;
+                v[kVd * NM + lane] = d;
+                v[kVg * NM + lane] = g;
+                v[kVu * NM + lane] = uu;
+            }
+            if (lane == 0) {
+                sc[kSf] = f;
+                sc[kStls] = tls;
+                sc[kStprev] = tprev;
+                sc[kSmaxF] = maxFactor;
+                sc[kSgd] = gd;
+                sc[kSerr] = err;
+                sc[kSgamma] = gamma;
+                sc[kSr] = r;
+                sc[kSrtol] = rtol;
+                in[kIiter] = iter;
+                in[kItot] = totalIter;
+                in[kIgam] = iterGamma;
+                in[kIreg] = regIndex;
+                in[kIflags] = flags;
+                in[kIphase] = phase;
+                in[kImaxIt] = maxIterCur;
+                Wn[atomicAdd(nWn, 1)] = c;
+            }
+        }
+    }
+}
+
+// Cells still listed after the pass budget: report them and leave their input untouched.
+template <int NM>
+__global__ void k_budget(const int *__restrict__ W, const int *__restrict__ nWp,
+                         c3d_cell_stats *stats, int *status)
+{
+    const int nW = *nWp;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nW; s += gridDim.x * blockDim.x) {
+        if (s == 0)
+            atomicOr(status, C3D_FLAG_BUDGET);
+        if (stats) {
+            c3d_cell_stats s0{0.0, 0.0, 0.0, 0, 0, 0, C3D_FLAG_BUDGET};
+            stats[W[s]] = s0;
+        }
+    }
+}
+
+// H only (solveH_cuda / solveHMn semantics): one wave per cell.
+template <int NM>
+__global__ __launch_bounds__(kBlock) void k_hessian_fp64(int64_t ncells, int nqPad,
+                                                        const double *__restrict__ Mg,
+                                                        const double *__restrict__ wg,
+                                                        const double *__restrict__ alpha_g,
+                                                        double *hs)
+{
+    constexpr int NT = (NM + 15) / 16;
+    constexpr int NB = NT * (NT + 1) / 2;
+    constexpr int NMS = NM | 1;
+    __shared__ double sM[kChunk * NMS];
+    __shared__ double sW[kChunk];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t cell = (int64_t)blockIdx.x * kWavesPerBlock + wv;
+    const bool have = cell < ncells;
+    double as[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        as[i] = have ? alpha_g[cell * NM + i] : 0.0;
+    d4 acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+        acc[b] = d4{0.0, 0.0, 0.0, 0.0};
+    const int gq = lane >> 4, rq = lane & 15;
+    for (int c = 0; c < nqPad / kChunk; ++c) {
+        __syncthreads();
+        for (int e = tid; e < kChunk * NM; e += kBlock) {
+            int rr = e / NM, cc = e - rr * NM;
+            sM[rr * NMS + cc] = Mg[(size_t)c * kChunk * NM + e];
+        }
+        if (tid < kChunk)
+            sW[tid] = wg[(size_t)c * kChunk + tid];
+        __syncthreads();
+        if (have) {
+            double x = 0.0;
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                x = x + as[i] * sM[lane * NMS + i];
+            const double we = sW[lane] * exp(x);
+#pragma unroll
+            for (int k = 0; k < kChunk / 4; ++k) {
+                const int ql = 4 * k + gq;
+                const double wq = __shfl(we, ql);
+                const double b0 = (rq < NM) ? sM[ql * NMS + rq] : 0.0;
+                acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(wq * b0, b0, acc[0], 0, 0, 0);
+                if constexpr (NT == 2) {
+                    const double b1 = (16 + rq < NM) ? sM[ql * NMS + 16 + rq] : 0.0;
+                    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(wq * b1, b0, acc[1], 0, 0, 0);
+                    acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(wq * b1, b1, acc[2], 0, 0, 0);
+                }
+            }
+        }
+    }
+    if (!have)
+        return;
+    double *H = hs + cell * NM * NM;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int R = (b == 0) ? 0 : 1, Cb = (b == 2) ? 1 : 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = 16 * R + gq + 4 * k, j = 16 * Cb + rq;
+            if (i < NM && j < NM && j <= i) {
+                H[i * NM + j] = acc[b][k];
+                H[j * NM + i] = acc[b][k];
+            }
+        }
+    }
+}
+
+// The dense part of one Newton iteration for a batch of given Hessians (one wave each):
+// info = dpotrf, rcond = dpocon(norm1(H)), d = -H^{-1} g (custom or LAPACK solve order).
+template <int NM>
+__global__ __launch_bounds__(kBlock) void k_dense_solve(int64_t n, const double *__restrict__ Hs,
+                                                        const double *__restrict__ gs, double *ds,
+                                                        double *rconds, int *infos, double *Ls,
+                                                        int lapackOrder)
+{
+    constexpr int LD = NM + 1;
+    __shared__ double sT[kWavesPerBlock][NM * LD];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t c = (int64_t)blockIdx.x * kWavesPerBlock + wv;
+    if (c >= n)
+        return;
+    const bool own = lane < NM;
+    Dense<NM> D;
+#pragma unroll
+    for (int j = 0; j < NM; ++j)
+        D.row[j] = own ? Hs[c * NM * NM + lane * NM + j] : 0.0;
+    double rs = 0.0;
+#pragma unroll
+    for (int j = 0; j < NM; ++j)
+        rs = rs + fabs(D.row[j]);
+    double norm1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        double vv = rdl(rs, i);
+        if (vv > norm1)
+            norm1 = vv;
+    }
+    const int info = D.cholesky(lane);
+    double rcond = 0.0, d = own ? gs[c * NM + lane] : 0.0;
+    if (info == 0) {
+        D.build_columns(sT[wv], LD, lane);
+        rcond = D.pocon(norm1, lane);
+        if (lapackOrder)
+            D.potrs_lapack(d, lane);
+        else
+            D.potrs_custom(d, lane);
+        d = -d;
+    }
+    if (own) {
+        ds[c * NM + lane] = d;
+#pragma unroll
+        for (int j = 0; j < NM; ++j)
+            Ls[c * NM * NM + lane * NM + j] = (j <= lane) ? D.row[j] : 0.0;
+    }
+    if (lane == 0) {
+        rconds[c] = rcond;
+        infos[c] = info;
+    }
+}
+
+__global__ void k_set_int(int *p, int v) { *p = v; }
+
+unsigned grid_cap(c3d_ctx_s *ctx, int perCU) { return (unsigned)(ctx->numCU * perCU); }
+
+template <int NM>
+hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alpha,
+                       c3d_cell_stats *stats, int maxPasses, hipStream_t s)
+{
+    const size_t n = (size_t)P.ncells;
+    size_t bytes = n * (kVec * NM + kScal + (size_t)NM * NM + 1) * sizeof(double) +
+                   n * (kInt + 2) * sizeof(int) + 4 * sizeof(int);
+    hipError_t e = grow(ctx->closureState, bytes);
+    if (e != hipSuccess)
+        return e;
+    char *base = (char *)ctx->closureState.p;
+    State S;
+    S.vec = (double *)base;
+    S.scal = S.vec + n * kVec * NM;
+    S.H = S.scal + n * kScal;
+    S.fdir = S.H + n * NM * NM;
+    S.ints = (int *)(S.fdir + n);
+    int *W0 = S.ints + n * kInt;
+    int *W1 = W0 + n;
+    int *counts = W1 + n;  // [0],[1] worklist sizes
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, counts + 0, 0);
+    const unsigned gInit = (unsigned)((n * 64 + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_init<NM>, dim3(gInit), dim3(kBlock), 0, s, P, u, alpha, S, W0, counts,
+                       stats, ctx->d_status);
+    int cur = 0;
+    for (int pass = 0; pass < maxPasses; ++pass) {
+        int *Wc = cur ? W1 : W0, *Wn = cur ? W0 : W1;
+        int *nc = counts + cur, *nn = counts + (1 - cur);
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, nn, 0);
+        hipLaunchKernelGGL(k_sweep<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P, ctx->d_M,
+                           ctx->d_wpad, S, Wc, nc);
+        hipLaunchKernelGGL(k_step<NM>, dim3(grid_cap(ctx, 16)), dim3(kBlock), 0, s, P, S, Wc, nc,
+                           Wn, nn, u, alpha, stats, ctx->d_status);
+        cur = 1 - cur;
+        if ((pass & 7) == 7) {
+            int left = 0;
+            e = hipMemcpyAsync(&left, counts + cur, sizeof(int), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(s);
+            if (e != hipSuccess)
+                return e;
+            if (left == 0)
+                return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL(k_budget<NM>, dim3(64), dim3(256), 0, s, cur ? W1 : W0, counts + cur,
+                       stats, ctx->d_status);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_closure_fp64(c3d_ctx_s *ctx, int64_t ncells, double *u, double *alpha,
+                               const c3d_closure_opts &o, c3d_cell_stats *stats,
+                               hipStream_t s)
+{
+    KParams P;
+    P.ncells = ncells;
+    P.nqPad = ctx->nqPad;
+    P.maxIter = o.maxIter;
+    P.unbatched = o.unbatched;
+    P.tol = o.tol;
+    P.tolGamma = o.tolGamma;
+    P.condHMax = o.condHMax;
+    P.m0 = ctx->cc.m0;
+    P.isoAlpha0 = ctx->cc.isoAlpha0;
+    P.minf = ctx->cc.minf;
+    if (ncells <= 0)
+        return hipSuccess;
+    if (ncells > INT32_MAX / 2)
+        return hipErrorInvalidValue;
+    const int maxPasses = ctx->maxPasses;
+    switch (ctx->t.nm) {
+    case 4: return run_closure<4>(ctx, P, u, alpha, stats, maxPasses, s);
+    case 9: return run_closure<9>(ctx, P, u, alpha, stats, maxPasses, s);
+    case 16: return run_closure<16>(ctx, P, u, alpha, stats, maxPasses, s);
+    case 25: return run_closure<25>(ctx, P, u, alpha, stats, maxPasses, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_dense_solve(int nm, int64_t n, const double *H, const double *g, double *d,
+                              double *rcond, int *info, double *L, int lapackOrder, hipStream_t s)
+{
+    if (n <= 0)
+        return hipSuccess;
+    dim3 grid((unsigned)((n + kWavesPerBlock - 1) / kWavesPerBlock));
+#define C3D_DS(NMV) hipLaunchKernelGGL(k_dense_solve<NMV>, grid, dim3(kBlock), 0, s, n, H, g, d, rcond, info, L, lapackOrder)
+    switch (nm) {
+    case 4: C3D_DS(4); break;
+    case 9: C3D_DS(9); break;
+    case 16: C3D_DS(16); break;
+    case 25: C3D_DS(25); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef C3D_DS
+    return hipGetLastError();
+}
+
+hipError_t launch_hessian_fp64(c3d_ctx_s *ctx, int64_t ncells, const double *alpha,
+                               double *hs, hipStream_t s)
+{
+    if (ncells <= 0)
+        return hipSuccess;
+    dim3 grid((unsigned)((ncells + kWavesPerBlock - 1) / kWavesPerBlock));
+    switch (ctx->t.nm) {
+    case 4: hipLaunchKernelGGL(k_hessian_fp64<4>, grid, dim3(kBlock), 0, s, ncells, ctx->nqPad, ctx->d_M, ctx->d_wpad, alpha, hs); break;
+    case 9: hipLaunchKernelGGL(k_hessian_fp64<9>, grid, dim3(kBlock), 0, s, ncells, ctx->nqPad, ctx->d_M, ctx->d_wpad, alpha, hs); break;
+    case 16: hipLaunchKernelGGL(k_hessian_fp64<16>, grid, dim3(kBlock), 0, s, ncells, ctx->nqPad, ctx->d_M, ctx->d_wpad, alpha, hs); break;
+    case 25: hipLaunchKernelGGL(k_hessian_fp64<25>, grid, dim3(kBlock), 0, s, ncells, ctx->nqPad, ctx->d_M, ctx->d_wpad, alpha, hs); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace c3d

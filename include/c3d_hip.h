@@ -1,0 +1,213 @@
+/*
+ * c3d_hip.h -- C-ABI of the MI355X (gfx950) implementation of the C3D-2015 per-timestep
+ * hot path: the M_N dual Newton closure, the kinetic and P_N moment fluxes, the SSP-RK2
+ * stage update and the periodic ghost fill.
+ *
+ * Two layers:
+ *   1. Drop-in replacements for the reference's accelerator seam, src/cuda/cuda_headers.h,
+ *      with identical argument lists and host-pointer, synchronous semantics
+ *      (solveFluxKinetic_hip, solveH_hip, solveHInit_hip).
+ *   2. A device-resident API (c3d_*): an explicit context holding the quadrature/basis
+ *      tables in HBM, device pointers in, device pointers out, one stream argument,
+ *      int status codes instead of abort().
+ *
+ * Layouts (as the reference's, src/global.h:15-19):
+ *   moments/alphas per cell contiguous:  idx(i,j,k,m) = m + nm*(k + (nz+4)*(j + (ny+4)*i))
+ *   for grids (2 ghost layers per side), or cell*nm + m for closure batches;
+ *   basis tables M[q*nm + m], M2[q*nm2 + k]; Hessians row-major nm x nm.
+ * All arithmetic is IEEE FP64 unless a function says otherwise.
+ *
+ * Streams are passed as void* (a hipStream_t; NULL = the null stream).
+ */
+#ifndef C3D_HIP_H
+#define C3D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+#define C3D_OK 0
+#define C3D_ERR_ARG 1          /* bad argument (sizes, NULL pointers, unsupported order) */
+#define C3D_ERR_HIP 2          /* a HIP runtime call failed */
+#define C3D_ERR_U0 3           /* a cell had u0 <= 0 (reference aborts: optBatch.cpp:319-323) */
+#define C3D_ERR_LADDER 4       /* a cell exhausted the regularisation ladder
+                                  (reference dumps and aborts: optBatch.cpp:233-238) */
+#define C3D_ERR_UNSUPPORTED 5  /* option combination not implemented */
+#define C3D_ERR_BUDGET 6       /* cells still unconverged after the pass budget
+                                  (c3d_set_pass_budget); their u, alpha are left as given */
+
+/* per-cell outcome flags (c3d_cell_stats.flags) */
+#define C3D_FLAG_ISO_RESTART 1 /* isotropic restart fired (optBatch.cpp:208-218) */
+#define C3D_FLAG_U0_NONPOS 2
+#define C3D_FLAG_LADDER 4
+#define C3D_FLAG_BUDGET 8      /* cell left unsolved by the pass budget */
+
+typedef struct c3d_ctx_s *c3d_ctx;
+
+/* Closure options: OPTIONS (src/opt/opt.h:14-23) plus the semantics switch.
+ *   unbatched = 0: Optimization::opt semantics (optBatch.cpp:143-282): absolute tol,
+ *                  tolGamma from the deck, the custom dpotrs (utils.cpp:71-86).
+ *   unbatched = 1: opt()/optScaled semantics (opt.cpp:139-355): tol relative to
+ *                  ||u/u0||_2, tolGamma fixed at 1.1 (update.cpp:57), LAPACK dpotrs order. */
+typedef struct {
+    int32_t maxIter;        /* MN_MAX_ITER */
+    int32_t useGaunt;       /* MN_USE_GAUNT_COEF */
+    int32_t useGauntSparse; /* MN_USE_GAUNT_SPARSE */
+    int32_t unbatched;      /* MN_BATCHED false */
+    double tol;             /* MN_TOL */
+    double tolGamma;        /* MN_TOL_GAMMA */
+    double condHMax;        /* MN_COND_H_MAX */
+} c3d_closure_opts;
+
+/* Per-cell outcome: OUTS (src/opt/opt.h:26-33) with the batched counters. 40 bytes. */
+typedef struct {
+    double r;           /* regularisation r of the final rung */
+    double err;         /* final ||grad h||_2 on the u0-scaled problem */
+    double gamma;       /* final modcfl value */
+    int32_t totalIter;  /* Newton iterations including restarts and rungs */
+    int32_t iterGamma;  /* iterations that met tol but not the gamma test */
+    int32_t regIndex;   /* ladder rung 0..4 */
+    int32_t flags;      /* C3D_FLAG_* */
+} c3d_cell_stats;
+
+/* ======================================================================
+ * 1. Drop-in replacements for src/cuda/cuda_headers.h (host pointers, synchronous)
+ * ====================================================================== */
+
+/* Replaces solveFluxKinetic_cuda (src/cuda/cuda_headers.h:10-13; called at
+ * src/flux_kinetic.cpp:84-92).  nx,ny,nz include the ghost layers.  Writes flux on the
+ * interior cells and zero on the ghosts, like the reference kernel; theta = 2 as in
+ * src/cuda/cuda_flux_kinetic.cu:82 (use c3d_flux_kinetic for the deck's THETA). */
+void solveFluxKinetic_hip(int nx, int ny, int nz, int nm, int nq, double dx, double dy,
+                          double dz, double *alpha, double *flux, double *xi, double *eta,
+                          double *mu, double *w, double *p, int isPn);
+
+/* Replaces solveHInit_cuda (src/cuda/cuda_headers.h:19-21; called at
+ * src/opt/optBatch.cpp:364-365).  Uploads w, p = M, p2 = M2 and the Gaunt tables. */
+void solveHInit_hip(int nm, int nm2, int nq, int batchSize, double *w, double *p, double *p2,
+                    double *beta, double *betaVal, int *betaColInd, int *betaRowPtr,
+                    int useGaunt);
+
+/* Replaces solveH_cuda (src/cuda/cuda_headers.h:15-17; called at
+ * src/opt/optBatch.cpp:393): hs[b] = H(alphas[b]) for b < batchSize, row-major nm x nm,
+ * lower triangle mirrored as solveHMn does (src/opt/fobj.cpp:34-102). */
+void solveH_hip(int nm, int nm2, int nq, int batchSize, double *alphas, double *hs,
+                int useGaunt, int useGauntSparse);
+
+/* ======================================================================
+ * 2. Device-resident API
+ * ====================================================================== */
+
+/* Context with tables built by the library from the moment order N and the quadrature
+ * order Q (init.cpp:423-548; Gaunt coefficients init.cpp:287-337; P_N operators
+ * init.cpp:199-234).  N in 1..4.  Binds to the current HIP device. */
+int c3d_create(c3d_ctx *ctx, int moment_order, int quad_order);
+
+/* Same, with caller-supplied tables (as solveHInit_cuda receives them).  M2/beta/CSR may
+ * be NULL when the Gaunt form is not used; nnz = betaRowPtr[nm*nm]. */
+int c3d_create_from_tables(c3d_ctx *ctx, int moment_order, int quad_order, const double *w,
+                           const double *xi, const double *eta, const double *mu,
+                           const double *M, const double *M2, const double *beta, int nnz,
+                           const double *betaVal, const int32_t *betaColInd,
+                           const int32_t *betaRowPtr);
+
+int c3d_destroy(c3d_ctx ctx);
+
+/* The library's own tables on the host, without a device (for inspection and tests):
+ * w, xi, eta, mu [nq]; M [nq][nm]; M2 [nq][nm2]; beta [nm*nm][nm2]; pnOps [6][nm][nm];
+ * any pointer may be NULL.  *nnz receives the Gaunt CSR size; betaVal/betaColInd
+ * (capacity nnzCap) and betaRowPtr [nm*nm+1] are filled when non-NULL and large enough. */
+int c3d_tables_host(int moment_order, int quad_order, double *w, double *xi, double *eta,
+                    double *mu, double *M, double *M2, double *beta, double *pnOps, int *nnz,
+                    int nnzCap, double *betaVal, int32_t *betaColInd, int32_t *betaRowPtr);
+
+/* nm, nm2, nq and the Gaunt nnz of a context. */
+int c3d_sizes(c3d_ctx ctx, int *nm, int *nm2, int *nq, int *nnz);
+
+/* Copy the context's tables to host arrays (any may be NULL).  pnOps: 6 x nm x nm in the
+ * order XiPlus, XiMinus, EtaPlus, EtaMinus, MuPlus, MuMinus. */
+int c3d_get_tables(c3d_ctx ctx, double *w, double *xi, double *eta, double *mu, double *M,
+                   double *M2, double *beta, double *betaVal, int32_t *betaColInd,
+                   int32_t *betaRowPtr, double *pnOps);
+
+/* M_N closure over ncells independent cells, FP64 (replaces the closure of
+ * update.cpp:105-178 / Optimization::opt, optBatch.cpp:350-472, and of
+ * update.cpp:43-99 / opt(), opt.cpp:330-355).  u, alpha: device, AoS [ncells][nm],
+ * updated in place exactly as the reference updates them (alpha solved; u rescaled,
+ * regularised where a rung fired).  stats: device [ncells] or NULL.  Work is queued on
+ * `stream` as passes over a shrinking worklist of unconverged cells; the call waits on the
+ * stream every 8 passes to test for an empty worklist.  Cell errors are reported by
+ * c3d_sync.  Scratch: about (8*nm + nm*nm + 12)*8 bytes per cell, owned by the context. */
+int c3d_closure(c3d_ctx ctx, int64_t ncells, double *u, double *alpha,
+                const c3d_closure_opts *opts, c3d_cell_stats *stats, void *stream);
+
+/* Upper bound on the sweep passes one c3d_closure call may run (default 100000).  A pass
+ * evaluates every unconverged cell once (a Newton iteration or one line-search trial); the
+ * reference has no such bound (its worst case is 1400 iterations x ~1100 backtracking
+ * steps per cell).  No counterpart in the reference. */
+int c3d_set_pass_budget(c3d_ctx ctx, int max_passes);
+
+/* H(alpha) for ncells cells (solveHMn semantics), device pointers, hs [ncells][nm][nm]. */
+int c3d_hessian(c3d_ctx ctx, int64_t ncells, const double *alpha, double *hs, int useGaunt,
+                int useGauntSparse, void *stream);
+
+/* The dense part of one Newton iteration for n given symmetric nm x nm matrices (device,
+ * row-major [n][nm][nm]) and gradients g [n][nm]: Cholesky in LAPACK dpotf2 order (info as
+ * dpotrf, optBatch.cpp:199), rcond as dpocon with the max-row-sum norm (optBatch.cpp:201-204),
+ * and d = -H^{-1} g by utils.cpp:71-86 (lapackOrder = 0) or LAPACK dpotrs (lapackOrder = 1,
+ * opt.cpp:285).  L [n][nm][nm] receives the factor (lower, row-major, zero above).  d is
+ * left equal to g and rcond = 0 where info != 0.  nm in {4, 9, 16, 25}. */
+int c3d_dense_solve(int nm, int64_t n, const double *H, const double *g, double *d,
+                    double *rcond, int32_t *info, double *L, int lapackOrder, void *stream);
+
+/* Wait for `stream`; return C3D_OK, or C3D_ERR_U0 / C3D_ERR_LADDER / C3D_ERR_BUDGET if a
+ * closure since the last c3d_sync hit that condition (the per-cell flags say which cells). */
+int c3d_sync(c3d_ctx ctx, void *stream);
+
+/* Periodic ghost fill of the 2 ghost layers on the 6 faces (serial branch of
+ * communicateBoundaries, comm.cpp:277-323).  u: device grid of the ctx's nm. */
+int c3d_ghost_fill(c3d_ctx ctx, int nx, int ny, int nz, double *u, void *stream);
+
+/* Kinetic flux (flux_kinetic.cpp:66-198): flux on interior cells from alpha on all cells;
+ * isPn selects the linear P_N ansatz (alpha then holds u).  Needs ansatz scratch: the
+ * context allocates (nx+4)(ny+4)(nz+4)*nq doubles on first use and keeps them. */
+int c3d_flux_kinetic(c3d_ctx ctx, int nx, int ny, int nz, const double *alpha, double *flux,
+                     double dx, double dy, double dz, double theta, int isPn, void *stream);
+
+/* P_N moment flux (flux_moment.cpp:30-104) with the context's half-range operators. */
+int c3d_flux_moment(c3d_ctx ctx, int nx, int ny, int nz, const double *u, double *flux,
+                    double dx, double dy, double dz, void *stream);
+
+/* Euler stage with isotropic scattering (update.cpp:21-37) on interior cells. */
+int c3d_euler(c3d_ctx ctx, int nx, int ny, int nz, double *u, const double *flux,
+              const double *sigmaT, const double *sigmaS, double dt, void *stream);
+
+/* uOld <- u (kind 0, update.cpp:206-212) or u <- (u + uOld)/2 (kind 1, update.cpp:268-274)
+ * on interior cells. */
+int c3d_rk2_combine(c3d_ctx ctx, int nx, int ny, int nz, double *u, double *uOld, int kind,
+                    void *stream);
+
+/* One full SSP-RK2 step, update(dt) (update.cpp:185-275): two stages of
+ * [ghost fill, closure over all cells (M_N), flux, Euler], then the average.
+ * solver: 1 = M_N, 0 = P_N.  pnUseKinetic: P_N flux via the kinetic scheme.
+ * Scratch (uOld, flux, ansatz) is owned by the context.  stats may be NULL or a device
+ * array of 2*(nx+4)(ny+4)(nz+4) entries (stage 0 then stage 1). */
+int c3d_update(c3d_ctx ctx, int nx, int ny, int nz, double *u, double *alpha,
+               const double *sigmaT, const double *sigmaS, double dx, double dy, double dz,
+               double theta, double dt, int solver, int pnUseKinetic,
+               const c3d_closure_opts *opts, c3d_cell_stats *stats, void *stream);
+
+/* Synthetic closure batch on the device: per cell s ~ U[smin,smax], alpha*_k ~ N(0,s^2),
+ * u = sum_q w_q m_q exp(alpha*.m_q), alpha0 from init.cpp:265.  SplitMix64 counter RNG
+ * keyed by (seed, cell). alphaStar may be NULL. */
+int c3d_make_closure_batch(c3d_ctx ctx, int64_t ncells, uint64_t seed, double smin,
+                           double smax, double *u, double *alpha0, double *alphaStar,
+                           void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

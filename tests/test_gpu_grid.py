@@ -1,0 +1,136 @@
+"""GPU parity of the time-stepping kernels against the CPU oracle: ghost fill, kinetic flux
+(M_N and P_N ansatz), P_N moment flux, Euler stage, RK2 combine and full SSP-RK2 steps.
+
+Bit-exact where no exponential is involved (ghost fill, moment flux, Euler, RK2, P_N kinetic
+flux); the M_N flux differs only through exp() rounding: moments within 1e-12 relative."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev_arr(x, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+@pytest.fixture(scope="module")
+def T2(O):
+    return O.Tables(2, 8)
+
+
+def _state(O, T, nx, ny, nz, seed=3):
+    u, sT, sS = O.init_grid(nx, ny, nz, T.nm, ic=1, sigma=0.3, floor=1e-3, noise=0.05, seed=seed)
+    rng = np.random.default_rng(seed)
+    u[:, 1:] = u[:, :1] * rng.uniform(-0.15, 0.15, (u.shape[0], T.nm - 1))
+    return u, sT, sS
+
+
+def test_ghost_fill_bit_exact(P, O, dev, T2):
+    nx, ny, nz = 7, 6, 5
+    u, _, _ = _state(O, T2, nx, ny, nz)
+    ref = u.copy()
+    O.lib().orc_ghost_fill(nx, ny, nz, T2.nm, ref)
+    ctx = P.Ctx(tables=T2)
+    ud = _dev_arr(u, dev)
+    ctx.ghost_fill(nx, ny, nz, ud)
+    assert np.array_equal(ud.cpu().numpy(), ref)
+    ctx.close()
+
+
+@pytest.mark.parametrize("isPn,theta", [(0, 2.0), (0, 1.0), (1, 2.0)])
+def test_flux_kinetic(P, O, dev, T2, isPn, theta):
+    import torch
+    nx, ny, nz = 9, 8, 7
+    u, _, _ = _state(O, T2, nx, ny, nz)
+    alpha = u.copy() if isPn else O.init_alpha(u)
+    if not isPn:
+        rng = np.random.default_rng(1)
+        alpha[:, 1:] = rng.normal(0, 0.3, (alpha.shape[0], T2.nm - 1))
+    ng = u.shape[0]
+    ref = np.zeros_like(u)
+    O.lib().orc_flux_kinetic(nx, ny, nz, T2.nm, T2.nq, alpha, ref, O.C.byref(T2.c),
+                             2 / nx, 2 / ny, 2 / nz, theta, isPn, np.zeros(ng), 0)
+    ctx = P.Ctx(tables=T2)
+    fd = torch.zeros_like(_dev_arr(u, dev))
+    ctx.flux_kinetic(nx, ny, nz, _dev_arr(alpha, dev), fd, 2 / nx, 2 / ny, 2 / nz, theta, isPn)
+    torch.cuda.synchronize()
+    f = fd.cpu().numpy().reshape(nx + 4, ny + 4, nz + 4, T2.nm)[2:-2, 2:-2, 2:-2]
+    r = ref.reshape(nx + 4, ny + 4, nz + 4, T2.nm)[2:-2, 2:-2, 2:-2]
+    if isPn:
+        assert np.array_equal(f, r)
+    else:
+        assert np.abs(f - r).max() <= 1e-12 * np.abs(r).max()
+    ctx.close()
+
+
+def test_flux_moment_bit_exact(P, O, dev, T2):
+    import torch
+    nx, ny, nz = 6, 7, 8
+    u, _, _ = _state(O, T2, nx, ny, nz)
+    ref = np.zeros_like(u)
+    O.lib().orc_flux_moment(nx, ny, nz, T2.nm, u, ref, T2.ops, 0.2, 0.25, 0.3, 0)
+    ctx = P.Ctx(tables=T2)
+    fd = torch.zeros_like(_dev_arr(u, dev))
+    ctx.flux_moment(nx, ny, nz, _dev_arr(u, dev), fd, 0.2, 0.25, 0.3)
+    torch.cuda.synchronize()
+    f = fd.cpu().numpy().reshape(nx + 4, ny + 4, nz + 4, T2.nm)[2:-2, 2:-2, 2:-2]
+    r = ref.reshape(nx + 4, ny + 4, nz + 4, T2.nm)[2:-2, 2:-2, 2:-2]
+    assert np.array_equal(f, r)
+    ctx.close()
+
+
+def test_euler_and_rk2_bit_exact(P, O, dev, T2):
+    import torch
+    nx, ny, nz = 5, 6, 7
+    u, sT, sS = _state(O, T2, nx, ny, nz)
+    rng = np.random.default_rng(2)
+    flux = rng.normal(0, 1, u.shape)
+    sT = rng.uniform(0.5, 2, sT.shape)
+    sS = sT * rng.uniform(0, 1, sT.shape)
+    ref = u.copy()
+    O.lib().orc_euler(nx, ny, nz, T2.nm, ref, flux, sT, sS, 0.013)
+    ctx = P.Ctx(tables=T2)
+    ud = _dev_arr(u, dev)
+    ctx.euler(nx, ny, nz, ud, _dev_arr(flux, dev), _dev_arr(sT, dev), _dev_arr(sS, dev), 0.013)
+    assert np.array_equal(ud.cpu().numpy(), ref)
+    old = _dev_arr(np.zeros_like(u), dev)
+    ctx.rk2_combine(nx, ny, nz, ud, old, 0)
+    torch.cuda.synchronize()
+    o = old.cpu().numpy().reshape(nx + 4, ny + 4, nz + 4, -1)
+    assert np.array_equal(o[2:-2, 2:-2, 2:-2], ref.reshape(o.shape)[2:-2, 2:-2, 2:-2])
+    ctx.close()
+
+
+@pytest.mark.parametrize("solver,pnk", [(1, 0), (0, 0), (0, 1)])
+def test_update_steps_match_oracle(P, O, dev, T2, solver, pnk):
+    """Three SSP-RK2 steps of the 20^3-style workload at 8^3 (M_N, P_N moment, P_N kinetic)."""
+    import torch
+    nx = ny = nz = 8
+    d = 2.0 / nx
+    u, sT, sS = O.init_grid(nx, ny, nz, T2.nm, ic=1, sigma=0.25, floor=1e-4)
+    alpha = O.init_alpha(u)
+    theta = 1.0
+    dt = 0.5 * (2 / (2 + theta)) / (3 / d) if solver else 0.5 * 0.5 * d
+    oo = O.opts()
+    ur, ar = u.copy(), alpha.copy()
+    ng = u.shape[0]
+    scratch = [np.zeros_like(u), np.zeros_like(u), np.zeros(ng * T2.nq)]
+    ctx = P.Ctx(tables=T2)
+    ud, ad = _dev_arr(u, dev), _dev_arr(alpha, dev)
+    sTd, sSd = _dev_arr(sT, dev), _dev_arr(sS, dev)
+    for step in range(3):
+        rc = O.lib().orc_update(nx, ny, nz, T2.nm, T2.nm2, T2.nq, solver, pnk, ur, ar,
+                                scratch[0], scratch[1], scratch[2], sT, sS, O.C.byref(T2.c),
+                                T2.ops, O.C.byref(oo), d, d, d, theta, dt, None, 0)
+        assert rc == 0
+        ctx.update(nx, ny, nz, ud, ad, sTd, sSd, d, d, d, theta, dt, solver, pnk,
+                   P.closure_opts())
+        assert ctx.sync() == 0
+    ug = ud.cpu().numpy()
+    if solver == 0:
+        assert np.array_equal(ug, ur)
+    else:
+        rel = np.abs(ug - ur).max() / np.abs(ur).max()
+        assert rel <= 1e-12, rel
+    ctx.close()
