@@ -1,0 +1,164 @@
+"""Benchmark of the M_N closure hot path on one MI355X.
+
+Workload (the metric's configuration): 1,048,576 independent M_3 dual problems (16
+moments), tensor quadrature 24 mu x 48 phi = 1152 ordinates, FP64.  Per cell s ~ U[0.5, 8],
+alpha*_k ~ N(0, s^2), u = <m e^{alpha*.m}>; Newton starts from the isotropic alpha of u
+(init.cpp:265) and stops at ||grad h|| <= 1e-8 on the u0-scaled problem with gamma <= 1.1
+(optBatch.cpp:268), MN_MAX_ITER 100, MN_COND_H_MAX 1e10.  The inputs are generated on the
+device once and stay in HBM; a step restores u and alpha from pristine device copies and
+solves the whole batch.
+
+One JSON line: value = cells solved per second (whole batch / step time); roofline of the
+dominant kernel (the Hessian sweep on the FP64 matrix cores) from HIP events recorded on the
+closure's stream; the CPU oracle timed on a bounded sample of the same inputs.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--cells 1048576] [--cpu-sample 65536]
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = ("M_N cell-updates/s at ‖∇h‖≤1e-8: batched mixed+NN vs plain FP64, "
+          "one GPU")
+FP64_MFMA_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+
+
+def front_end():
+    spec = importlib.util.spec_from_file_location(
+        "c3d_hip", os.path.join(ROOT, "c3d-2015_amd", "c3d_hip.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["c3d_hip"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(u_host, a_host, N, Q, nthreads):
+    """The oracle (plain FP64 C restatement, OpenMP over cells) on a sample of the inputs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    T = O.Tables(N, Q)
+    n = u_host.shape[0]
+    # warm-up on a small slice, then time the sample
+    O.closure(T, u_host[:64].copy(), a_host[:64].copy(), O.opts(), nthreads)
+    t0 = time.perf_counter()
+    O.closure(T, u_host.copy(), a_host.copy(), O.opts(), nthreads)
+    dt = time.perf_counter() - t0
+    return n / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cells", type=int, default=1 << 20)
+    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--seed", type=lambda s: int(s, 0), default=0xC3D2015)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the closure is "
+              "embarrassingly parallel over cells; see DESIGN.md for the sharding design)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    P = front_end()
+    dev = torch.device("cuda:0")
+    N, Q = 3, 24
+    n = args.cells
+    ctx = P.Ctx(N, Q)
+    nm = ctx.nm
+    u0 = torch.empty((n, nm), dtype=torch.float64, device=dev)
+    a0 = torch.empty_like(u0)
+    ctx.make_closure_batch(n, args.seed, 0.5, 8.0, u0, a0)
+    u = torch.empty_like(u0)
+    a = torch.empty_like(a0)
+    stats = P.alloc_stats(n, dev)
+    opts = P.closure_opts(maxIter=100, tol=1e-8, tolGamma=1.1, condHMax=1e10)
+    torch.cuda.synchronize()
+
+    def step():
+        u.copy_(u0)
+        a.copy_(a0)
+        ctx.closure(u, a, opts, stats=stats)
+
+    for _ in range(args.warmup):
+        step()
+    rc = ctx.sync()
+    ctx.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sweep_ms = 0.0
+    step_ms = 0.0
+    cell_sweeps = 0
+    passes = 0
+    for _ in range(args.steps):
+        step()
+        prof = ctx.closure_profile()  # waits for the step's stream
+        sweep_ms += prof["sweep_ms"]
+        step_ms += prof["step_ms"]
+        cell_sweeps += prof["cell_sweeps"]
+        passes += prof["passes"]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    rc = ctx.sync()
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = n / (ms_per_step / 1e3)
+
+    st = P.stats_to_numpy(stats)
+    it = st["totalIter"]
+    hist = np.bincount(st["regIndex"], minlength=5).tolist()
+
+    # roofline of the sweep kernel: H = M^T diag(w e) M, 2 nq nm^2 flops per cell-sweep
+    nq = ctx.nq
+    flops_per_sweep = 2.0 * nq * nm * nm
+    achieved = flops_per_sweep * cell_sweeps / (sweep_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK,
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK, 4),
+                "traffic": None, "kernel": "k_sweep<16> (FP64 MFMA Hessian sweep)",
+                "flops_per_cell_sweep": flops_per_sweep,
+                "cell_sweeps_per_step": cell_sweeps // args.steps,
+                "sweep_kernel_ms_per_step": round(sweep_ms / args.steps, 3),
+                "step_kernel_ms_per_step": round(step_ms / args.steps, 3),
+                "passes_per_step": passes // args.steps}
+
+    cpu = None
+    if args.cpu_sample > 0:
+        ns = min(args.cpu_sample, n)
+        uh = u0[:ns].cpu().numpy().copy()
+        ah = a0[:ns].cpu().numpy().copy()
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        cps, secs = cpu_baseline(uh, ah, N, Q, threads)
+        cpu = {"value": round(cps, 1), "unit": "cell-updates/s", "cores": threads,
+               "kind": "port",
+               "sample": f"first {ns} cells of the same batch, oracle (C, -O3, OpenMP "
+                         f"dynamic over cells), {secs:.1f} s"}
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "closure-only batch: 1048576 independent M_3 dual problems, "
+                               "FP64, 24x48=1152 ordinates, alpha*~N(0,s^2), s~U[0.5,8], "
+                               "tol 1e-8",
+                   "cells": n, "moment_order": N, "quad_order": Q, "ordinates": nq,
+                   "path": "plain FP64 (matrix-core Hessian sums, worklist passes)",
+                   "status": rc,
+                   "iters_per_cell_mean": round(float(it.mean()), 3),
+                   "iters_per_cell_max": int(it.max()),
+                   "regularisation_hist": hist},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -14,11 +14,17 @@ LIB_PATH = os.path.join(HERE, "libc3d_hip.so")
 (C3D_OK, C3D_ERR_ARG, C3D_ERR_HIP, C3D_ERR_U0, C3D_ERR_LADDER, C3D_ERR_UNSUPPORTED,
  C3D_ERR_BUDGET) = range(7)
 FLAG_ISO_RESTART, FLAG_U0_NONPOS, FLAG_LADDER, FLAG_BUDGET = 1, 2, 4, 8
+PREC_FP64, PREC_FP64_REFORDER = 0, 1
 
 _vp = C.c_void_p
 _i32 = C.c_int32
 _i64 = C.c_int64
 _d = C.c_double
+
+
+class ClosureProfile(C.Structure):
+    _fields_ = [("cell_sweeps", _i64), ("passes", _i32), ("reserved", _i32),
+                ("sweep_ms", _d), ("step_ms", _d)]
 
 
 class ClosureOpts(C.Structure):
@@ -45,6 +51,9 @@ SIGNATURES = [
     ("c3d_sizes", C.c_int, [_vp] + [C.POINTER(C.c_int)] * 4),
     ("c3d_get_tables", C.c_int, [_vp] * 12),
     ("c3d_closure", C.c_int, [_vp, _i64, _vp, _vp, C.POINTER(ClosureOpts), _vp, _vp]),
+    ("c3d_set_profiling", C.c_int, [_vp, C.c_int]),
+    ("c3d_get_closure_profile", C.c_int, [_vp, C.POINTER(ClosureProfile)]),
+    ("c3d_set_precision", C.c_int, [_vp, C.c_int]),
     ("c3d_set_pass_budget", C.c_int, [_vp, C.c_int]),
     ("c3d_hessian", C.c_int, [_vp, _i64, _vp, _vp, C.c_int, C.c_int, _vp]),
     ("c3d_dense_solve", C.c_int, [C.c_int, _i64] + [_vp] * 6 + [C.c_int, _vp]),
@@ -149,6 +158,17 @@ class Ctx:
         return check("c3d_closure", self.L.c3d_closure(self.h, n, _p(u), _p(alpha),
                                                        C.byref(opts), _p(stats),
                                                        _stream(stream)))
+
+    def set_profiling(self, on):
+        return check("c3d_set_profiling", self.L.c3d_set_profiling(self.h, int(on)))
+
+    def closure_profile(self):
+        p = ClosureProfile()
+        check("c3d_get_closure_profile", self.L.c3d_get_closure_profile(self.h, C.byref(p)))
+        return {k: getattr(p, k) for k, _ in ClosureProfile._fields_ if k != "reserved"}
+
+    def set_precision(self, mode):
+        return check("c3d_set_precision", self.L.c3d_set_precision(self.h, mode))
 
     def set_pass_budget(self, n):
         return check("c3d_set_pass_budget", self.L.c3d_set_pass_budget(self.h, n))

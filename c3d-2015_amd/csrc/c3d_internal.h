@@ -55,12 +55,19 @@ struct c3d_ctx_s {
     double *d_beta = nullptr, *d_betaVal = nullptr, *d_pnOps = nullptr;
     int32_t *d_betaColInd = nullptr, *d_betaRowPtr = nullptr;
     // control words
-    unsigned long long *d_counter = nullptr;  // work queue head of the persistent closure
     int *d_status = nullptr;                  // OR of per-cell error flags since c3d_sync
     // scratch owned by the context, grown on demand
     c3d::DevBuf ansatz, uOld, flux;
     c3d::DevBuf closureState;  // per-cell Newton state, Hessians, worklists
     int maxPasses = 100000;     // sweep passes per closure call before cells are reported
+    int precision = C3D_PREC_FP64;
+    // profiling of the last closure call
+    bool profiling = false;
+    std::vector<hipEvent_t> evPool;  // 4 per pass: sweep start/end, step start/end
+    int evUsed = 0;
+    int lastPasses = 0;
+    hipStream_t lastStream = nullptr;
+    unsigned long long *d_sweepCount = nullptr;
     std::mutex mtx;
 };
 

@@ -76,7 +76,7 @@ int finish_ctx(c3d_ctx_s *ctx)
     UP(d_pnOps, t.pnOps);
 #undef UP
     if (e == hipSuccess)
-        e = hipMalloc((void **)&ctx->d_counter, sizeof(unsigned long long));
+        e = hipMalloc((void **)&ctx->d_sweepCount, sizeof(unsigned long long));
     if (e == hipSuccess)
         e = hipMalloc((void **)&ctx->d_status, sizeof(int));
     if (e == hipSuccess)
@@ -160,11 +160,13 @@ int c3d_destroy(c3d_ctx ctx)
         return C3D_OK;
     void *ptrs[] = {ctx->d_w, ctx->d_xi, ctx->d_eta, ctx->d_mu, ctx->d_M, ctx->d_wpad,
                     ctx->d_M2, ctx->d_beta, ctx->d_betaVal, ctx->d_pnOps, ctx->d_betaColInd,
-                    ctx->d_betaRowPtr, ctx->d_counter, ctx->d_status, ctx->ansatz.p,
+                    ctx->d_betaRowPtr, ctx->d_sweepCount, ctx->d_status, ctx->ansatz.p,
                     ctx->uOld.p, ctx->flux.p, ctx->closureState.p};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
+    for (hipEvent_t e : ctx->evPool)
+        (void)hipEventDestroy(e);
     delete ctx;
     return C3D_OK;
 }
@@ -251,6 +253,47 @@ int c3d_closure(c3d_ctx ctx, int64_t ncells, double *u, double *alpha,
         return C3D_ERR_ARG;
     return hip_status(launch_closure_fp64(ctx, ncells, u, alpha, *opts, stats,
                                           (hipStream_t)stream));
+}
+
+int c3d_set_profiling(c3d_ctx ctx, int enable)
+{
+    if (!ctx)
+        return C3D_ERR_ARG;
+    ctx->profiling = enable != 0;
+    return C3D_OK;
+}
+
+int c3d_get_closure_profile(c3d_ctx ctx, c3d_closure_profile *out)
+{
+    if (!ctx || !out)
+        return C3D_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    out->passes = ctx->lastPasses;
+    if (!ctx->profiling)
+        return C3D_OK;
+    if (hipStreamSynchronize(ctx->lastStream) != hipSuccess)
+        return C3D_ERR_HIP;
+    unsigned long long n = 0;
+    if (hipMemcpy(&n, ctx->d_sweepCount, sizeof(n), hipMemcpyDeviceToHost) != hipSuccess)
+        return C3D_ERR_HIP;
+    out->cell_sweeps = (int64_t)n;
+    for (int k = 0; k + 3 < ctx->evUsed; k += 4) {
+        float a = 0.f, b = 0.f;
+        if (hipEventElapsedTime(&a, ctx->evPool[k], ctx->evPool[k + 1]) != hipSuccess ||
+            hipEventElapsedTime(&b, ctx->evPool[k + 2], ctx->evPool[k + 3]) != hipSuccess)
+            return C3D_ERR_HIP;
+        out->sweep_ms += a;
+        out->step_ms += b;
+    }
+    return C3D_OK;
+}
+
+int c3d_set_precision(c3d_ctx ctx, int mode)
+{
+    if (!ctx || (mode != C3D_PREC_FP64 && mode != C3D_PREC_FP64_REFORDER))
+        return C3D_ERR_ARG;
+    ctx->precision = mode;
+    return C3D_OK;
 }
 
 int c3d_set_pass_budget(c3d_ctx ctx, int max_passes)

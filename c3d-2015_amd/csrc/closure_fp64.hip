@@ -27,6 +27,7 @@
 // of passes, so the closure always drains.
 #include <cfloat>
 
+#include "c3d_detmath.h"
 #include "c3d_internal.h"
 
 #pragma clang fp contract(off)
@@ -59,7 +60,7 @@ struct State {
 
 struct KParams {
     int64_t ncells;
-    int nqPad, maxIter, unbatched;
+    int nq, nqPad, maxIter, unbatched;
     double tol, tolGamma, condHMax;
     double m0, isoAlpha0, minf;
 };
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_init(KParams P, const double *__rest
     double uu = own ? u_g[c * NM + lane] : 0.0;
     double a = own ? alpha_g[c * NM + lane] : 0.0;
     const double u0 = rdl(uu, 0);
-    const double alphaScale = log(u0) * 2.0 * sqrt(M_PI);
+    const double alphaScale = det_log(u0) * 2.0 * sqrt(M_PI);
     if (lane == 0)
         a = a - alphaScale;
     uu = uu / u0;
@@ -503,18 +504,104 @@ __global__ __launch_bounds__(kBlock) void k_sweep(KParams P, const double *__res
     }
 }
 
+// ---------------------------------------- sweep pass, reference operation order
+// One wavefront per cell.  The exponent of ordinate q is computed as in fobj.cpp:44-47 with
+// the deterministic exp; each lane owns entries (i >= j) of H and adds
+// ((w_q m_qi) m_qj) e_q in ordinate order (fobj.cpp:62-69), and sum_q w_q e_q is
+// accumulated in ordinate order (fobj.cpp:141-164).  Bit-identical to a CPU evaluation of
+// solveHMn / fobjMn with the same exp.
+template <int NM>
+__global__ __launch_bounds__(kBlock) void k_sweep_ref(KParams P, const double *__restrict__ Mg,
+                                                       const double *__restrict__ wg, State S,
+                                                       const int *__restrict__ W,
+                                                       const int *__restrict__ nWp)
+{
+    constexpr int NE = NM * (NM + 1) / 2;
+    constexpr int NS = (NE + kWave - 1) / kWave;
+    constexpr int NMS = NM | 1;
+    __shared__ double sM[kChunk * NMS];
+    __shared__ double sW[kChunk];
+    __shared__ double sE[kWavesPerBlock][kChunk];
+    __shared__ double sA[kWavesPerBlock][NM];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nW = uniform_int(*nWp);
+    const int nBlocksNeeded = (nW + kWavesPerBlock - 1) / kWavesPerBlock;
+    int ei[NS], ej[NS];
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2) {
+        int e = lane + kWave * s2, i = 0;
+        while (i < NM && (i + 1) * (i + 2) / 2 <= e)
+            ++i;
+        ei[s2] = e < NE ? i : 0;
+        ej[s2] = e < NE ? e - i * (i + 1) / 2 : 0;
+    }
+    for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
+        const int slot = blk * kWavesPerBlock + wv;
+        const int cell = slot < nW ? uniform_int(W[slot]) : -1;
+        if (lane < NM)
+            sA[wv][lane] = cell >= 0 ? S.vec[(size_t)cell * kVec * NM + kVa * NM + lane] : 0.0;
+        double h[NS];
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2)
+            h[s2] = 0.0;
+        double f = 0.0;
+        for (int ch = 0; ch < P.nqPad / kChunk; ++ch) {
+            __syncthreads();
+            for (int e = tid; e < kChunk * NM; e += kBlock) {
+                const int rr = e / NM, cc = e - rr * NM;
+                sM[rr * NMS + cc] = Mg[(size_t)ch * kChunk * NM + e];
+            }
+            if (tid < kChunk)
+                sW[tid] = wg[(size_t)ch * kChunk + tid];
+            __syncthreads();
+            if (cell >= 0) {
+                double x = 0.0;
+#pragma unroll
+                for (int i = 0; i < NM; ++i)
+                    x = x + sA[wv][i] * sM[lane * NMS + i];
+                sE[wv][lane] = det_exp(x);
+                wave_lds_sync();
+                const int nq = min(kChunk, P.nq - ch * kChunk);  // padded ordinates skipped
+                for (int q = 0; q < nq; ++q) {
+                    const double wq = sW[q], tq = sE[wv][q];
+#pragma unroll
+                    for (int s2 = 0; s2 < NS; ++s2)
+                        if (lane + kWave * s2 < NE)
+                            h[s2] += wq * sM[q * NMS + ei[s2]] * sM[q * NMS + ej[s2]] * tq;
+                    f = f + wq * tq;
+                }
+                wave_lds_sync();
+            }
+        }
+        if (cell < 0)
+            continue;
+        double *H = S.H + (size_t)cell * NM * NM;
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2)
+            if (lane + kWave * s2 < NE) {
+                H[ei[s2] * NM + ej[s2]] = h[s2];
+                H[ej[s2] * NM + ei[s2]] = h[s2];
+            }
+        if (lane == 0)
+            S.fdir[cell] = f;
+    }
+}
+
 // ---------------------------------------------------------------- step pass
 template <int NM>
 __global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, const int *__restrict__ W,
                                                   const int *__restrict__ nWp, int *Wn, int *nWn,
                                                   double *u_g, double *alpha_g,
-                                                  c3d_cell_stats *stats, int *status)
+                                                  c3d_cell_stats *stats, int *status,
+                                                  unsigned long long *sweepCount)
 {
     constexpr int LD = NM + 1;
     __shared__ double sT[kWavesPerBlock][NM * LD];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const bool own = lane < NM;
     const int nW = uniform_int(*nWp);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && sweepCount)
+        atomicAdd(sweepCount, (unsigned long long)nW);
     const int nBlocksNeeded = (nW + kWavesPerBlock - 1) / kWavesPerBlock;
     for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
         const int slot = blk * kWavesPerBlock + wv;
@@ -632,7 +719,7 @@ __global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, const int *
 #pragma unroll
                 for (int i = 0; i < NM; ++i)
                     da1 = da1 + fabs(rdl(d, i));
-                gamma = exp(2.0 * da1 * P.minf);
+                gamma = det_exp(2.0 * da1 * P.minf);
                 if (err <= rtol && gamma <= tolGamma) {
                     done = 1;
                 } else {
@@ -665,7 +752,7 @@ __global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, const int *
         if (done) {
             if (done == 2)
                 flags |= C3D_FLAG_LADDER;
-            const double alphaScale = log(u0) * 2.0 * sqrt(M_PI);  // postProcess
+            const double alphaScale = det_log(u0) * 2.0 * sqrt(M_PI);  // postProcess
             if (own) {
                 u_g[(size_t)c * NM + lane] = uu * u0;
                 alpha_g[(size_t)c * NM + lane] = (lane == 0) ? a + alphaScale : a;
@@ -851,6 +938,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_solve(int64_t n, const double 
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 unsigned grid_cap(c3d_ctx_s *ctx, int perCU) { return (unsigned)(ctx->numCU * perCU); }
 
@@ -875,6 +963,11 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
     int *W1 = W0 + n;
     int *counts = W1 + n;  // [0],[1] worklist sizes
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, counts + 0, 0);
+    ctx->evUsed = 0;
+    ctx->lastPasses = 0;
+    ctx->lastStream = s;
+    if (ctx->profiling)
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, ctx->d_sweepCount);
     const unsigned gInit = (unsigned)((n * 64 + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_init<NM>, dim3(gInit), dim3(kBlock), 0, s, P, u, alpha, S, W0, counts,
                        stats, ctx->d_status);
@@ -883,10 +976,36 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
         int *Wc = cur ? W1 : W0, *Wn = cur ? W0 : W1;
         int *nc = counts + cur, *nn = counts + (1 - cur);
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, nn, 0);
-        hipLaunchKernelGGL(k_sweep<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P, ctx->d_M,
-                           ctx->d_wpad, S, Wc, nc);
+        hipEvent_t *ev = nullptr;
+        if (ctx->profiling) {
+            if ((int)ctx->evPool.size() < ctx->evUsed + 4) {
+                for (int k = 0; k < 64; ++k) {
+                    hipEvent_t x;
+                    if (hipEventCreate(&x) != hipSuccess)
+                        return hipErrorOutOfMemory;
+                    ctx->evPool.push_back(x);
+                }
+            }
+            ev = &ctx->evPool[ctx->evUsed];
+            ctx->evUsed += 4;
+            hipEventRecord(ev[0], s);
+        }
+        if (ctx->precision == C3D_PREC_FP64_REFORDER)
+            hipLaunchKernelGGL(k_sweep_ref<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P,
+                               ctx->d_M, ctx->d_wpad, S, Wc, nc);
+        else
+            hipLaunchKernelGGL(k_sweep<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P,
+                               ctx->d_M, ctx->d_wpad, S, Wc, nc);
+        if (ev) {
+            hipEventRecord(ev[1], s);
+            hipEventRecord(ev[2], s);
+        }
         hipLaunchKernelGGL(k_step<NM>, dim3(grid_cap(ctx, 16)), dim3(kBlock), 0, s, P, S, Wc, nc,
-                           Wn, nn, u, alpha, stats, ctx->d_status);
+                           Wn, nn, u, alpha, stats, ctx->d_status,
+                           ctx->profiling ? ctx->d_sweepCount : nullptr);
+        if (ev)
+            hipEventRecord(ev[3], s);
+        ctx->lastPasses = pass + 1;
         cur = 1 - cur;
         if ((pass & 7) == 7) {
             int left = 0;
@@ -912,6 +1031,7 @@ hipError_t launch_closure_fp64(c3d_ctx_s *ctx, int64_t ncells, double *u, double
 {
     KParams P;
     P.ncells = ncells;
+    P.nq = ctx->t.nq;
     P.nqPad = ctx->nqPad;
     P.maxIter = o.maxIter;
     P.unbatched = o.unbatched;

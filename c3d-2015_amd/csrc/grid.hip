@@ -18,6 +18,7 @@
 // results match the CPU code bit for bit whenever the exponentials agree.
 #include <cfloat>
 
+#include "c3d_detmath.h"
 #include "c3d_internal.h"
 
 #pragma clang fp contract(off)
@@ -119,7 +120,8 @@ __global__ __launch_bounds__(kTB) void k_ansatz(int64_t ng, int nq, const double
 #pragma unroll
                 for (int m = 0; m < NM; ++m)
                     kin += a[m] * sM[ql * NM + m];
-                A[(size_t)(q0 + ql) * ng + cell] = isPn ? kin : exp(kin);
+                A[(size_t)(q0 + ql) * ng + cell] =
+                    isPn == 1 ? kin : (isPn == 2 ? det_exp(kin) : exp(kin));
             }
         }
     }
@@ -423,6 +425,9 @@ hipError_t launch_ghost_fill(int nx, int ny, int nz, int nm, double *u, hipStrea
 hipError_t launch_ansatz(c3d_ctx_s *ctx, int nx, int ny, int nz, const double *alpha, double *A,
                          int isPn, hipStream_t s)
 {
+    // kind: 1 linear (P_N), 2 deterministic exp (reference-order mode), 0 device exp
+    if (!isPn && ctx->precision == C3D_PREC_FP64_REFORDER)
+        isPn = 2;
     const int64_t ng = (int64_t)(nx + 4) * (ny + 4) * (nz + 4);
     dim3 grid(blocks_for(ng, kTB, INT32_MAX));
     const int nq = ctx->t.nq;

@@ -45,6 +45,11 @@ extern "C" {
 #define C3D_FLAG_LADDER 4
 #define C3D_FLAG_BUDGET 8      /* cell left unsolved by the pass budget */
 
+/* arithmetic modes of a context (c3d_set_precision) */
+#define C3D_PREC_FP64 0          /* FP64; quadrature sums on the matrix cores (default) */
+#define C3D_PREC_FP64_REFORDER 1 /* FP64 in the reference's operation order with a
+                                    deterministic exp/log: bit-reproducible on a CPU */
+
 typedef struct c3d_ctx_s *c3d_ctx;
 
 /* Closure options: OPTIONS (src/opt/opt.h:14-23) plus the semantics switch.
@@ -143,6 +148,28 @@ int c3d_get_tables(c3d_ctx ctx, double *w, double *xi, double *eta, double *mu, 
  * c3d_sync.  Scratch: about (8*nm + nm*nm + 12)*8 bytes per cell, owned by the context. */
 int c3d_closure(c3d_ctx ctx, int64_t ncells, double *u, double *alpha,
                 const c3d_closure_opts *opts, c3d_cell_stats *stats, void *stream);
+
+/* Select the arithmetic of the closure and of the kinetic-flux ansatz (C3D_PREC_*).
+ * C3D_PREC_FP64_REFORDER evaluates H(alpha) entry by entry in ordinate order
+ * (solveHMn's loop, fobj.cpp:42-69) and uses an exp/log built from + - * / (Cody-Waite
+ * reduction, Taylor and atanh series), so that a CPU evaluation of the same method gives
+ * identical bits, decisions and outputs; it is several times slower than C3D_PREC_FP64.
+ * No counterpart in the reference (whose results depend on the host libm). */
+int c3d_set_precision(c3d_ctx ctx, int mode);
+
+/* Kernel-time accounting of c3d_closure (HIP events on the closure's stream around every
+ * sweep and step launch), for throughput and roofline reporting. */
+typedef struct {
+    int64_t cell_sweeps; /* cell evaluations of H(alpha) (Newton iterations + line-search trials) */
+    int32_t passes;      /* worklist passes */
+    int32_t reserved;
+    double sweep_ms;     /* summed duration of the sweep kernels */
+    double step_ms;      /* summed duration of the Newton-step kernels */
+} c3d_closure_profile;
+
+int c3d_set_profiling(c3d_ctx ctx, int enable);
+/* Profile of the most recent c3d_closure call (waits for its stream). */
+int c3d_get_closure_profile(c3d_ctx ctx, c3d_closure_profile *out);
 
 /* Upper bound on the sweep passes one c3d_closure call may run (default 100000).  A pass
  * evaluates every unconverged cell once (a Newton iteration or one line-search trial); the

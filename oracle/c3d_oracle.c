@@ -33,6 +33,67 @@
 
 static const double orc_reg[ORC_NUM_REG] = {0.0, 1e-3, 1e-2, 1e-1, 5e-1}; /* opt.h:37-38 */
 
+/* ---- exponential / logarithm used on the path ----
+ * mode 0: the C library's exp/log, as the reference calls them (fobj.cpp:50,149;
+ *         optBatch.cpp:139,314,336; flux_kinetic.cpp:35).
+ * mode 1: a deterministic evaluation from + - * / only (Cody & Waite 1980 reduction with a
+ *         split ln2 and a degree-13 Taylor polynomial for exp; the atanh series for log),
+ *         which any IEEE binary64 machine evaluates to the same bits.  The device's
+ *         reference-order mode uses the same method, so mode 1 makes the oracle comparable
+ *         to it bit for bit. */
+static int orc_exp_mode = 0;
+void orc_set_exp_mode(int mode) { orc_exp_mode = mode; }
+
+double orc_exp_det(double x)
+{
+    const double kInvLn2 = 0x1.71547652b82fep+0;
+    const double kLn2Hi = 0x1.62e42fefa3800p-1;
+    const double kLn2Lo = 0x1.ef35793c76730p-45;
+    static const double c[14] = {
+        1.0, 1.0, 0.5, 0x1.5555555555555p-3, 0x1.5555555555555p-5, 0x1.1111111111111p-7,
+        0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13, 0x1.a01a01a01a01ap-16,
+        0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22, 0x1.ae64567f544e4p-26,
+        0x1.1eed8eff8d898p-29, 0x1.6124613a86d09p-33};
+    if (x != x)
+        return x + x;
+    if (x > 710.0)
+        return HUGE_VAL;
+    if (x < -746.0)
+        return 0.0;
+    double k = rint(x * kInvLn2);
+    double r = (x - k * kLn2Hi) - k * kLn2Lo;
+    double p = c[13];
+    for (int n = 12; n >= 0; n--)
+        p = p * r + c[n];
+    return ldexp(p, (int)k);
+}
+
+double orc_log_det(double x)
+{
+    const double kLn2Hi = 0x1.62e42fefa3800p-1;
+    const double kLn2Lo = 0x1.ef35793c76730p-45;
+    if (!(x > 0.0))
+        return (x == 0.0) ? -HUGE_VAL : NAN;
+    if (x == HUGE_VAL)
+        return x;
+    int e = 0;
+    double m = frexp(x, &e);
+    if (m < 0x1.6a09e667f3bcdp-1) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+    double p = 1.0 / 23.0;
+    for (int k = 21; k >= 1; k -= 2)
+        p = p * s2 + 1.0 / (double)k;
+    double lm = 2.0 * s * p;
+    double de = (double)e;
+    return de * kLn2Hi + (lm + de * kLn2Lo);
+}
+
+static inline double orc_exp(double x) { return orc_exp_mode ? orc_exp_det(x) : exp(x); }
+static inline double orc_log(double x) { return orc_exp_mode ? orc_log_det(x) : log(x); }
+
 #define ORC_MIN(a, b) (((a) < (b)) ? (a) : (b)) /* utils.h:12 */
 #define ORC_MAX(a, b) (((a) > (b)) ? (a) : (b)) /* utils.h:13 */
 #define ORC_SIGN(a, b) ((b) >= 0.0 ? fabs(a) : -fabs(a)) /* utils.h:14 */
@@ -605,7 +666,7 @@ double orc_modcfl(int n, const double *da)
         da_1 = da_1 + fabs(da[i]);
     int pnOrder = (int)floor(sqrt((double)n)) + 1;
     double m_inf = sqrt((2 * pnOrder + 1) / (2 * M_PI));
-    return exp(2.0 * da_1 * m_inf);
+    return orc_exp(2.0 * da_1 * m_inf);
 }
 
 /* ========================================================== dual objective */
@@ -623,7 +684,7 @@ void orc_solveH(int nm, int nm2, int nq, const double *alpha, const orc_tables *
         double temp = 0;
         for (int i = 0; i < nm; i++)
             temp = temp + alpha[i] * p[q * nm + i];
-        temp = exp(temp);
+        temp = orc_exp(temp);
         if (useGaunt) {
             for (int k = 0; k < nm2; k++)
                 gExt[k] += w[q] * p2[q * nm2 + k] * temp;
@@ -673,7 +734,7 @@ double orc_fobj(int nm, int nq, const double *alpha, const double *u, const doub
         double temp = 0;
         for (int i = 0; i < nm; i++)
             temp = temp + alpha[i] * M[q * nm + i];
-        temp = exp(temp);
+        temp = orc_exp(temp);
         if (g)
             for (int i = 0; i < nm; i++)
                 g[i] += w[q] * M[q * nm + i] * temp;
@@ -750,21 +811,18 @@ int orc_closure_cell(int nm, int nm2, int nq, double *u, double *alpha, const or
     double m0 = t->M[0];
     int flags = 0;
     double u0 = u[0];
-    double alphaScale = log(u0) * 2.0 * sqrt(M_PI);
-    alpha[0] = alpha[0] - alphaScale;
-    for (int k = 0; k < nm; k++)
-        u[k] = u[k] / u0;
     if (u0 <= 0) {
-        /* the reference aborts; restore the caller's data and report */
-        for (int k = 0; k < nm; k++)
-            u[k] = u[k] * u0;
-        alpha[0] = alpha[0] + alphaScale;
+        /* the reference aborts (optBatch.cpp:319-323); leave the cell as given, report */
         if (s) {
             memset(s, 0, sizeof(*s));
             s->flags = ORC_FLAG_U0_NONPOS;
         }
         return ORC_FLAG_U0_NONPOS;
     }
+    double alphaScale = orc_log(u0) * 2.0 * sqrt(M_PI);
+    alpha[0] = alpha[0] - alphaScale;
+    for (int k = 0; k < nm; k++)
+        u[k] = u[k] / u0;
     memcpy(alpha0, alpha, nm * sizeof(double));
     memcpy(uOrig, u, nm * sizeof(double));
 
@@ -843,7 +901,7 @@ int orc_closure_cell(int nm, int nm2, int nq, double *u, double *alpha, const or
         tstep = orc_linesearch(nm, nq, alpha, d, f, g, u, t->w, t->M);
     }
     /* postProcess (optBatch.cpp:334-344) / opt() rescale (opt.cpp:351-354) */
-    double alphaScale2 = log(u0) * 2.0 * sqrt(M_PI);
+    double alphaScale2 = orc_log(u0) * 2.0 * sqrt(M_PI);
     for (int k = 0; k < nm; k++)
         u[k] = u[k] * u0;
     alpha[0] = alpha[0] + alphaScale2;
@@ -957,7 +1015,7 @@ void orc_flux_kinetic(int nx, int ny, int nz, int nm, int nq, const double *alph
                     double kin = 0.0;
                     for (int m = 0; m < nm; m++)
                         kin += a[m] * M[q * nm + m];
-                    ansatz[I3(i, j, k)] = isPn ? kin : exp(kin);
+                    ansatz[I3(i, j, k)] = isPn ? kin : orc_exp(kin);
                 }
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(nthreads)

@@ -67,6 +67,11 @@ typedef struct {
     int32_t *betaRowPtr;             /* [nm*nm+1] */
 } orc_tables;
 
+/* ---- exp/log on the path: 0 = C library (reference), 1 = deterministic method ---- */
+void   orc_set_exp_mode(int mode);
+double orc_exp_det(double x);
+double orc_log_det(double x);
+
 /* ---- tables ---- */
 void   orc_gauss_legendre(int n, double *x, double *w);
 double orc_sph_plm(int l, int m, double x);

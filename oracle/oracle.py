@@ -50,6 +50,11 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("oracle library missing: run `make -C oracle` (or build())")
         L = C.CDLL(LIB_PATH)
+        L.orc_set_exp_mode.argtypes = [C.c_int]
+        L.orc_exp_det.argtypes = [C.c_double]
+        L.orc_exp_det.restype = C.c_double
+        L.orc_log_det.argtypes = [C.c_double]
+        L.orc_log_det.restype = C.c_double
         L.orc_gauss_legendre.argtypes = [C.c_int, _dp, _dp]
         L.orc_sph_plm.argtypes = [C.c_int, C.c_int, C.c_double]
         L.orc_sph_plm.restype = C.c_double
@@ -113,6 +118,20 @@ def lib():
                                               C.c_double, C.c_double, _dp, _dp, _dp]
         _lib = L
     return _lib
+
+
+class exp_mode:
+    """Context manager selecting the oracle's exp/log: 0 = C library, 1 = deterministic."""
+
+    def __init__(self, mode):
+        self.mode = mode
+
+    def __enter__(self):
+        lib().orc_set_exp_mode(self.mode)
+        return self
+
+    def __exit__(self, *a):
+        lib().orc_set_exp_mode(0)
 
 
 def _ptr(a):
