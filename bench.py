@@ -94,17 +94,19 @@ def main():
     ctx.set_profiling(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sweep_ms = 0.0
-    step_ms = 0.0
-    cell_sweeps = 0
-    passes = 0
+    sweep_ms = step_ms = tail_ms = 0.0
+    cell_sweeps = tail_sweeps = 0
+    passes = tail_launches = 0
     for _ in range(args.steps):
         step()
         prof = ctx.closure_profile()  # waits for the step's stream
         sweep_ms += prof["sweep_ms"]
         step_ms += prof["step_ms"]
+        tail_ms += prof["tail_ms"]
         cell_sweeps += prof["cell_sweeps"]
+        tail_sweeps += prof["tail_cell_sweeps"]
         passes += prof["passes"]
+        tail_launches += prof["tail_launches"]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     rc = ctx.sync()
@@ -126,6 +128,9 @@ def main():
                 "cell_sweeps_per_step": cell_sweeps // args.steps,
                 "sweep_kernel_ms_per_step": round(sweep_ms / args.steps, 3),
                 "step_kernel_ms_per_step": round(step_ms / args.steps, 3),
+                "tail_kernel_ms_per_step": round(tail_ms / args.steps, 3),
+                "tail_cell_sweeps_per_step": tail_sweeps // args.steps,
+                "tail_launches_per_step": tail_launches // args.steps,
                 "passes_per_step": passes // args.steps}
 
     cpu = None

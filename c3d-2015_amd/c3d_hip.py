@@ -23,8 +23,8 @@ _d = C.c_double
 
 
 class ClosureProfile(C.Structure):
-    _fields_ = [("cell_sweeps", _i64), ("passes", _i32), ("reserved", _i32),
-                ("sweep_ms", _d), ("step_ms", _d)]
+    _fields_ = [("cell_sweeps", _i64), ("tail_cell_sweeps", _i64), ("passes", _i32),
+                ("tail_launches", _i32), ("sweep_ms", _d), ("step_ms", _d), ("tail_ms", _d)]
 
 
 class ClosureOpts(C.Structure):
@@ -53,6 +53,7 @@ SIGNATURES = [
     ("c3d_closure", C.c_int, [_vp, _i64, _vp, _vp, C.POINTER(ClosureOpts), _vp, _vp]),
     ("c3d_set_profiling", C.c_int, [_vp, C.c_int]),
     ("c3d_get_closure_profile", C.c_int, [_vp, C.POINTER(ClosureProfile)]),
+    ("c3d_set_tail", C.c_int, [_vp, _i32, _i32]),
     ("c3d_set_precision", C.c_int, [_vp, C.c_int]),
     ("c3d_set_pass_budget", C.c_int, [_vp, C.c_int]),
     ("c3d_hessian", C.c_int, [_vp, _i64, _vp, _vp, C.c_int, C.c_int, _vp]),
@@ -165,13 +166,16 @@ class Ctx:
     def closure_profile(self):
         p = ClosureProfile()
         check("c3d_get_closure_profile", self.L.c3d_get_closure_profile(self.h, C.byref(p)))
-        return {k: getattr(p, k) for k, _ in ClosureProfile._fields_ if k != "reserved"}
+        return {k: getattr(p, k) for k, _ in ClosureProfile._fields_}
 
     def set_precision(self, mode):
         return check("c3d_set_precision", self.L.c3d_set_precision(self.h, mode))
 
     def set_pass_budget(self, n):
         return check("c3d_set_pass_budget", self.L.c3d_set_pass_budget(self.h, n))
+
+    def set_tail(self, max_cells, passes_per_launch=32):
+        return check("c3d_set_tail", self.L.c3d_set_tail(self.h, max_cells, passes_per_launch))
 
     def hessian(self, alpha, hs, stream=None):
         n = alpha.numel() // self.nm

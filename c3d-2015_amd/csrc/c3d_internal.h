@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "../../include/c3d_hip.h"
@@ -63,11 +64,16 @@ struct c3d_ctx_s {
     int precision = C3D_PREC_FP64;
     // profiling of the last closure call
     bool profiling = false;
-    std::vector<hipEvent_t> evPool;  // 4 per pass: sweep start/end, step start/end
+    std::vector<hipEvent_t> evPool;
+    std::vector<std::pair<int, int>> evSpans;  // (0: sweep+step pass, 4 events | 1: tail, 2)
     int evUsed = 0;
     int lastPasses = 0;
+    int lastTailLaunches = 0;
     hipStream_t lastStream = nullptr;
-    unsigned long long *d_sweepCount = nullptr;
+    unsigned long long *d_sweepCount = nullptr;  // [0] bulk sweep kernel, [1] tail kernel
+    // fused tail: taken once at most tailCells cells remain listed, tailPasses per launch
+    int tailCells = 8192;
+    int tailPasses = 32;
     std::mutex mtx;
 };
 

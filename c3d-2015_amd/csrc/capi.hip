@@ -76,7 +76,7 @@ int finish_ctx(c3d_ctx_s *ctx)
     UP(d_pnOps, t.pnOps);
 #undef UP
     if (e == hipSuccess)
-        e = hipMalloc((void **)&ctx->d_sweepCount, sizeof(unsigned long long));
+        e = hipMalloc((void **)&ctx->d_sweepCount, 2 * sizeof(unsigned long long));
     if (e == hipSuccess)
         e = hipMalloc((void **)&ctx->d_status, sizeof(int));
     if (e == hipSuccess)
@@ -269,22 +269,40 @@ int c3d_get_closure_profile(c3d_ctx ctx, c3d_closure_profile *out)
         return C3D_ERR_ARG;
     std::memset(out, 0, sizeof(*out));
     out->passes = ctx->lastPasses;
+    out->tail_launches = ctx->lastTailLaunches;
     if (!ctx->profiling)
         return C3D_OK;
     if (hipStreamSynchronize(ctx->lastStream) != hipSuccess)
         return C3D_ERR_HIP;
-    unsigned long long n = 0;
-    if (hipMemcpy(&n, ctx->d_sweepCount, sizeof(n), hipMemcpyDeviceToHost) != hipSuccess)
+    unsigned long long n[2] = {0, 0};
+    if (hipMemcpy(n, ctx->d_sweepCount, sizeof(n), hipMemcpyDeviceToHost) != hipSuccess)
         return C3D_ERR_HIP;
-    out->cell_sweeps = (int64_t)n;
-    for (int k = 0; k + 3 < ctx->evUsed; k += 4) {
+    out->cell_sweeps = (int64_t)n[0];
+    out->tail_cell_sweeps = (int64_t)n[1];
+    for (const auto &sp : ctx->evSpans) {
+        const int k = sp.second;
         float a = 0.f, b = 0.f;
-        if (hipEventElapsedTime(&a, ctx->evPool[k], ctx->evPool[k + 1]) != hipSuccess ||
-            hipEventElapsedTime(&b, ctx->evPool[k + 2], ctx->evPool[k + 3]) != hipSuccess)
-            return C3D_ERR_HIP;
-        out->sweep_ms += a;
-        out->step_ms += b;
+        if (sp.first == 0) {
+            if (hipEventElapsedTime(&a, ctx->evPool[k], ctx->evPool[k + 1]) != hipSuccess ||
+                hipEventElapsedTime(&b, ctx->evPool[k + 2], ctx->evPool[k + 3]) != hipSuccess)
+                return C3D_ERR_HIP;
+            out->sweep_ms += a;
+            out->step_ms += b;
+        } else {
+            if (hipEventElapsedTime(&a, ctx->evPool[k], ctx->evPool[k + 1]) != hipSuccess)
+                return C3D_ERR_HIP;
+            out->tail_ms += a;
+        }
     }
+    return C3D_OK;
+}
+
+int c3d_set_tail(c3d_ctx ctx, int max_cells, int passes_per_launch)
+{
+    if (!ctx || max_cells < 0 || passes_per_launch < 1 || passes_per_launch > 4096)
+        return C3D_ERR_ARG;
+    ctx->tailCells = max_cells;
+    ctx->tailPasses = passes_per_launch;
     return C3D_OK;
 }
 

@@ -25,6 +25,7 @@
 //            themselves to the next worklist.
 // Every loop in both kernels is bounded by sizes fixed at launch; the host caps the number
 // of passes, so the closure always drains.
+#include <algorithm>
 #include <cfloat>
 
 #include "c3d_detmath.h"
@@ -396,7 +397,131 @@ __global__ __launch_bounds__(kBlock) void k_init(KParams P, const double *__rest
     }
 }
 
+// ---------------------------------------------------------------- sweep building blocks
+// Matrix-core accumulation of one LDS chunk of kChunk ordinates for CW cells: lane l owns
+// ordinate l of the chunk for the exponent, then the MFMA loop consumes 4 ordinates per
+// instruction.  The operation sequence per cell does not depend on CW or on the other cells
+// of the wave, so a cell's H is the same bits whichever kernel evaluates it.
+template <int NM, int CW>
+__device__ __forceinline__ void chunk_mfma(const double *sMc, double wl, const double *sAc,
+                                           d4 (&acc)[CW][(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
+                                           double (&fpart)[CW], int lane)
+{
+    constexpr int NT = (NM + 15) / 16;
+    constexpr int NMS = NM | 1;
+    const int gq = lane >> 4, rq = lane & 15;
+    const double *rowp = &sMc[lane * NMS];
+    double we[CW];
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+        double x = 0.0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i)
+            x = x + sAc[c * NM + i] * rowp[i];
+        we[c] = wl * exp(x);
+        fpart[c] = fpart[c] + we[c];
+    }
+#pragma unroll
+    for (int k = 0; k < kChunk / 4; ++k) {
+        const int ql = 4 * k + gq;
+        const double *mr = &sMc[ql * NMS];
+        const double b0 = (rq < NM) ? mr[rq] : 0.0;
+        const double b1 = (NT == 2 && 16 + rq < NM) ? mr[16 + rq] : 0.0;
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            const double wq = __shfl(we[c], ql);
+            acc[c][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(wq * b0, b0, acc[c][0], 0, 0, 0);
+            if constexpr (NT == 2) {
+                acc[c][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(wq * b1, b0, acc[c][1], 0, 0, 0);
+                acc[c][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(wq * b1, b1, acc[c][2], 0, 0, 0);
+            }
+        }
+    }
+}
+
+// H from one cell's accumulators: lower triangle, mirrored (fobj.cpp:96-101).
+template <int NM>
+__device__ __forceinline__ void store_H(const d4 *acc, double *H, int lane)
+{
+    constexpr int NT = (NM + 15) / 16;
+    constexpr int NB = NT * (NT + 1) / 2;
+    const int gq = lane >> 4, rq = lane & 15;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int R = (b == 0) ? 0 : 1, Cb = (b == 2) ? 1 : 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = 16 * R + gq + 4 * k, j = 16 * Cb + rq;
+            if (i < NM && j < NM && j <= i) {
+                H[i * NM + j] = acc[b][k];
+                H[j * NM + i] = acc[b][k];
+            }
+        }
+    }
+}
+
+// Entry-per-lane assignment of the lower triangle for the reference-order sweep.
+template <int NM>
+struct RefEntries {
+    static constexpr int NE = NM * (NM + 1) / 2;
+    static constexpr int NS = (NE + kWave - 1) / kWave;
+    int ei[NS], ej[NS];
+    __device__ __forceinline__ explicit RefEntries(int lane)
+    {
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2) {
+            int e = lane + kWave * s2, i = 0;
+            while (i < NM && (i + 1) * (i + 2) / 2 <= e)
+                ++i;
+            ei[s2] = e < NE ? i : 0;
+            ej[s2] = e < NE ? e - i * (i + 1) / 2 : 0;
+        }
+    }
+};
+
+// One chunk in the reference's order for one cell (one wave): the exponent of the lane's
+// ordinate as in fobj.cpp:44-47 with the deterministic exp, then each lane adds
+// ((w_q m_qi) m_qj) e_q for its entries in ordinate order (fobj.cpp:62-69) and
+// sum_q w_q e_q in ordinate order (fobj.cpp:141-164).
+template <int NM>
+__device__ __forceinline__ void chunk_ref(const double *sMc, const double *sWc, double *sEc,
+                                          const double *sAc, const RefEntries<NM> &E,
+                                          double (&h)[RefEntries<NM>::NS], double &f, int nqc,
+                                          int lane)
+{
+    constexpr int NMS = NM | 1;
+    double x = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        x = x + sAc[i] * sMc[lane * NMS + i];
+    sEc[lane] = det_exp(x);
+    wave_lds_sync();
+    for (int q = 0; q < nqc; ++q) {  // padded ordinates skipped
+        const double wq = sWc[q], tq = sEc[q];
+#pragma unroll
+        for (int s2 = 0; s2 < RefEntries<NM>::NS; ++s2)
+            if (lane + kWave * s2 < RefEntries<NM>::NE)
+                h[s2] += wq * sMc[q * NMS + E.ei[s2]] * sMc[q * NMS + E.ej[s2]] * tq;
+        f = f + wq * tq;
+    }
+    wave_lds_sync();
+}
+
+template <int NM>
+__device__ __forceinline__ void store_H_ref(const RefEntries<NM> &E,
+                                            const double (&h)[RefEntries<NM>::NS], double *H,
+                                            int lane)
+{
+#pragma unroll
+    for (int s2 = 0; s2 < RefEntries<NM>::NS; ++s2)
+        if (lane + kWave * s2 < RefEntries<NM>::NE) {
+            H[E.ei[s2] * NM + E.ej[s2]] = h[s2];
+            H[E.ej[s2] * NM + E.ei[s2]] = h[s2];
+        }
+}
+
 // ---------------------------------------------------------------- sweep pass
+// A wavefront takes 4 cells; the block's 16 cells share every M chunk staged in LDS.
 template <int NM>
 __global__ __launch_bounds__(kBlock) void k_sweep(KParams P, const double *__restrict__ Mg,
                                                    const double *__restrict__ wg, State S,
@@ -409,13 +534,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep(KParams P, const double *__res
     constexpr int CW = kCellsPerWave;
     __shared__ double sM[kChunk * NMS];
     __shared__ double sW[kChunk];
-    __shared__ double sA[kWavesPerBlock][CW][NM];
+    __shared__ double sA[kWavesPerBlock][CW * NM];
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nW = uniform_int(*nWp);
-    const int groups = (nW + CW - 1) / CW;                  // 4 cells per wave
+    const int groups = (nW + CW - 1) / CW;
     const int nBlocksNeeded = (groups + kWavesPerBlock - 1) / kWavesPerBlock;
-    const int gq = lane >> 4, rq = lane & 15;
     for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
         const int grp = blk * kWavesPerBlock + wv;
         int cells[CW];
@@ -424,11 +548,11 @@ __global__ __launch_bounds__(kBlock) void k_sweep(KParams P, const double *__res
             const int slot = grp * CW + c;
             cells[c] = (grp < groups && slot < nW) ? W[slot] : -1;
         }
-        // alphas of the wave's cells to LDS
 #pragma unroll
         for (int c = 0; c < CW; ++c)
             if (lane < NM)
-                sA[wv][c][lane] = cells[c] >= 0 ? S.vec[(size_t)cells[c] * kVec * NM + kVa * NM + lane] : 0.0;
+                sA[wv][c * NM + lane] =
+                    cells[c] >= 0 ? S.vec[(size_t)cells[c] * kVec * NM + kVa * NM + lane] : 0.0;
         d4 acc[CW][NB];
 #pragma unroll
         for (int c = 0; c < CW; ++c)
@@ -449,75 +573,30 @@ __global__ __launch_bounds__(kBlock) void k_sweep(KParams P, const double *__res
             if (tid < kChunk)
                 sW[tid] = wg[(size_t)ch * kChunk + tid];
             __syncthreads();
-            if (any) {
-                const double *rowp = &sM[lane * NMS];
-                double we[CW];
-#pragma unroll
-                for (int c = 0; c < CW; ++c) {
-                    double x = 0.0;
-#pragma unroll
-                    for (int i = 0; i < NM; ++i)
-                        x = x + sA[wv][c][i] * rowp[i];
-                    we[c] = sW[lane] * exp(x);
-                    fpart[c] = fpart[c] + we[c];
-                }
-#pragma unroll
-                for (int k = 0; k < kChunk / 4; ++k) {
-                    const int ql = 4 * k + gq;
-                    const double *mr = &sM[ql * NMS];
-                    const double b0 = (rq < NM) ? mr[rq] : 0.0;
-                    const double b1 = (NT == 2 && 16 + rq < NM) ? mr[16 + rq] : 0.0;
-#pragma unroll
-                    for (int c = 0; c < CW; ++c) {
-                        const double wq = __shfl(we[c], ql);
-                        acc[c][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(wq * b0, b0, acc[c][0], 0, 0, 0);
-                        if constexpr (NT == 2) {
-                            acc[c][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(wq * b1, b0, acc[c][1], 0, 0, 0);
-                            acc[c][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(wq * b1, b1, acc[c][2], 0, 0, 0);
-                        }
-                    }
-                }
-            }
+            if (any)
+                chunk_mfma<NM, CW>(sM, sW[lane], sA[wv], acc, fpart, lane);
         }
-        // results: lower triangle, mirrored (fobj.cpp:96-101)
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
             const double fs = wave_sum(fpart[c]);
             if (cells[c] < 0)
                 continue;
-            double *H = S.H + (size_t)cells[c] * NM * NM;
             if (lane == 0)
                 S.fdir[cells[c]] = fs;
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const int R = (b == 0) ? 0 : 1, Cb = (b == 2) ? 1 : 0;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int i = 16 * R + gq + 4 * k, j = 16 * Cb + rq;
-                    if (i < NM && j < NM && j <= i) {
-                        H[i * NM + j] = acc[c][b][k];
-                        H[j * NM + i] = acc[c][b][k];
-                    }
-                }
-            }
+            store_H<NM>(acc[c], S.H + (size_t)cells[c] * NM * NM, lane);
         }
     }
 }
 
 // ---------------------------------------- sweep pass, reference operation order
-// One wavefront per cell.  The exponent of ordinate q is computed as in fobj.cpp:44-47 with
-// the deterministic exp; each lane owns entries (i >= j) of H and adds
-// ((w_q m_qi) m_qj) e_q in ordinate order (fobj.cpp:62-69), and sum_q w_q e_q is
-// accumulated in ordinate order (fobj.cpp:141-164).  Bit-identical to a CPU evaluation of
-// solveHMn / fobjMn with the same exp.
+// One wavefront per cell; bit-identical to a CPU evaluation of solveHMn / fobjMn with the
+// same exp.
 template <int NM>
 __global__ __launch_bounds__(kBlock) void k_sweep_ref(KParams P, const double *__restrict__ Mg,
                                                        const double *__restrict__ wg, State S,
                                                        const int *__restrict__ W,
                                                        const int *__restrict__ nWp)
 {
-    constexpr int NE = NM * (NM + 1) / 2;
-    constexpr int NS = (NE + kWave - 1) / kWave;
     constexpr int NMS = NM | 1;
     __shared__ double sM[kChunk * NMS];
     __shared__ double sW[kChunk];
@@ -526,23 +605,15 @@ __global__ __launch_bounds__(kBlock) void k_sweep_ref(KParams P, const double *_
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nW = uniform_int(*nWp);
     const int nBlocksNeeded = (nW + kWavesPerBlock - 1) / kWavesPerBlock;
-    int ei[NS], ej[NS];
-#pragma unroll
-    for (int s2 = 0; s2 < NS; ++s2) {
-        int e = lane + kWave * s2, i = 0;
-        while (i < NM && (i + 1) * (i + 2) / 2 <= e)
-            ++i;
-        ei[s2] = e < NE ? i : 0;
-        ej[s2] = e < NE ? e - i * (i + 1) / 2 : 0;
-    }
+    const RefEntries<NM> E(lane);
     for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
         const int slot = blk * kWavesPerBlock + wv;
         const int cell = slot < nW ? uniform_int(W[slot]) : -1;
         if (lane < NM)
             sA[wv][lane] = cell >= 0 ? S.vec[(size_t)cell * kVec * NM + kVa * NM + lane] : 0.0;
-        double h[NS];
+        double h[RefEntries<NM>::NS];
 #pragma unroll
-        for (int s2 = 0; s2 < NS; ++s2)
+        for (int s2 = 0; s2 < RefEntries<NM>::NS; ++s2)
             h[s2] = 0.0;
         double f = 0.0;
         for (int ch = 0; ch < P.nqPad / kChunk; ++ch) {
@@ -554,38 +625,245 @@ __global__ __launch_bounds__(kBlock) void k_sweep_ref(KParams P, const double *_
             if (tid < kChunk)
                 sW[tid] = wg[(size_t)ch * kChunk + tid];
             __syncthreads();
-            if (cell >= 0) {
-                double x = 0.0;
-#pragma unroll
-                for (int i = 0; i < NM; ++i)
-                    x = x + sA[wv][i] * sM[lane * NMS + i];
-                sE[wv][lane] = det_exp(x);
-                wave_lds_sync();
-                const int nq = min(kChunk, P.nq - ch * kChunk);  // padded ordinates skipped
-                for (int q = 0; q < nq; ++q) {
-                    const double wq = sW[q], tq = sE[wv][q];
-#pragma unroll
-                    for (int s2 = 0; s2 < NS; ++s2)
-                        if (lane + kWave * s2 < NE)
-                            h[s2] += wq * sM[q * NMS + ei[s2]] * sM[q * NMS + ej[s2]] * tq;
-                    f = f + wq * tq;
-                }
-                wave_lds_sync();
-            }
+            if (cell >= 0)
+                chunk_ref<NM>(sM, sW, sE[wv], sA[wv], E, h, f, min(kChunk, P.nq - ch * kChunk),
+                              lane);
         }
         if (cell < 0)
             continue;
-        double *H = S.H + (size_t)cell * NM * NM;
-#pragma unroll
-        for (int s2 = 0; s2 < NS; ++s2)
-            if (lane + kWave * s2 < NE) {
-                H[ei[s2] * NM + ej[s2]] = h[s2];
-                H[ej[s2] * NM + ei[s2]] = h[s2];
-            }
+        store_H_ref<NM>(E, h, S.H + (size_t)cell * NM * NM, lane);
         if (lane == 0)
             S.fdir[cell] = f;
     }
 }
+
+// ---------------------------------------------------------------- one cell's state machine
+// Register image of a cell's Newton state; vectors are distributed (lane i < NM holds
+// component i), scalars are wave-uniform.
+template <int NM>
+struct Cell {
+    double a, a0, aOld, d, g, uu, uOrig;
+    double u0, f, tls, tprev, maxF, gd, err, gamma, r, rtol;
+    int iter, tot, igam, reg, flags, phase, maxIt;
+
+    __device__ __forceinline__ void load(const State &S, int c, int lane)
+    {
+        const bool own = lane < NM;
+        const double *v = S.vec + (size_t)c * kVec * NM;
+        a = own ? v[kVa * NM + lane] : 0.0;
+        a0 = own ? v[kVa0 * NM + lane] : 0.0;
+        aOld = own ? v[kVaOld * NM + lane] : 0.0;
+        d = own ? v[kVd * NM + lane] : 0.0;
+        g = own ? v[kVg * NM + lane] : 0.0;
+        uu = own ? v[kVu * NM + lane] : 0.0;
+        uOrig = own ? v[kVuOrig * NM + lane] : 0.0;
+        const double *sc = S.scal + (size_t)c * kScal;
+        u0 = sc[kSu0];
+        f = sc[kSf];
+        tls = sc[kStls];
+        tprev = sc[kStprev];
+        maxF = sc[kSmaxF];
+        gd = sc[kSgd];
+        err = sc[kSerr];
+        gamma = sc[kSgamma];
+        r = sc[kSr];
+        rtol = sc[kSrtol];
+        const int *in = S.ints + (size_t)c * kInt;
+        iter = in[kIiter];
+        tot = in[kItot];
+        igam = in[kIgam];
+        reg = in[kIreg];
+        flags = in[kIflags];
+        phase = in[kIphase];
+        maxIt = in[kImaxIt];
+    }
+
+    __device__ __forceinline__ void store(const State &S, int c, int lane) const
+    {
+        double *v = S.vec + (size_t)c * kVec * NM;
+        if (lane < NM) {
+            v[kVa * NM + lane] = a;
+            v[kVa0 * NM + lane] = a0;
+            v[kVaOld * NM + lane] = aOld;
+            v[kVd * NM + lane] = d;
+            v[kVg * NM + lane] = g;
+            v[kVu * NM + lane] = uu;
+        }
+        if (lane == 0) {
+            double *sc = S.scal + (size_t)c * kScal;
+            sc[kSf] = f;
+            sc[kStls] = tls;
+            sc[kStprev] = tprev;
+            sc[kSmaxF] = maxF;
+            sc[kSgd] = gd;
+            sc[kSerr] = err;
+            sc[kSgamma] = gamma;
+            sc[kSr] = r;
+            sc[kSrtol] = rtol;
+            int *in = S.ints + (size_t)c * kInt;
+            in[kIiter] = iter;
+            in[kItot] = tot;
+            in[kIgam] = igam;
+            in[kIreg] = reg;
+            in[kIflags] = flags;
+            in[kIphase] = phase;
+            in[kImaxIt] = maxIt;
+        }
+    }
+
+    // postProcess (optBatch.cpp:333-344 / opt.cpp:347-353) and the per-cell statistics
+    __device__ __forceinline__ void finish(int done, int c, double *u_g, double *alpha_g,
+                                           c3d_cell_stats *stats, int *status, int lane)
+    {
+        if (done == 2)
+            flags |= C3D_FLAG_LADDER;
+        const double alphaScale = det_log(u0) * 2.0 * sqrt(M_PI);
+        if (lane < NM) {
+            u_g[(size_t)c * NM + lane] = uu * u0;
+            alpha_g[(size_t)c * NM + lane] = (lane == 0) ? a + alphaScale : a;
+        }
+        if (lane == 0) {
+            if (done == 2)
+                atomicOr(status, C3D_FLAG_LADDER);
+            if (stats) {
+                c3d_cell_stats s0{r, err, gamma, tot, igam, reg, flags};
+                stats[c] = s0;
+            }
+        }
+    }
+
+    // One step of the state machine with the sweep result at the current alpha (H row-major,
+    // in LDS or global memory; fdir = sum_q w_q e_q): either the Armijo test of a line-search
+    // trial, or one Newton iteration of solveTheRest / optScaled.  sT: NM x (NM+1) doubles of
+    // LDS for the transposed factor.  Returns 0 to continue, 1 converged, 2 ladder exhausted.
+    __device__ __forceinline__ int advance(const KParams &P, const double *H, double fdir,
+                                           double *sT, int lane)
+    {
+        constexpr int LD = NM + 1;
+        const bool own = lane < NM;
+        const double tolGamma = P.unbatched ? 1.1 : P.tolGamma;
+        if (phase == kPhaseLS) {
+            double fNew = fdir;
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                fNew = fNew - rdl(a, i) * rdl(uu, i);
+            if (fNew <= f + 1e-3 * tls * gd) {
+                tprev = tls;  // accepted; H of this sweep belongs to the new alpha
+            } else {
+                tls = 0.5 * tls;
+                if (tls * maxF > DBL_EPSILON / 2) {
+                    a = own ? aOld + tls * d : 0.0;
+                    phase = kPhaseLS;
+                } else {
+                    a = aOld;
+                    tprev = 0.0;
+                    phase = kPhaseH;
+                }
+                return 0;
+            }
+        }
+        // ================================ one Newton iteration (solveTheRest)
+        Dense<NM> D;
+#pragma unroll
+        for (int j = 0; j < NM; ++j)
+            D.row[j] = own ? H[lane * NM + j] : 0.0;
+        int maxItNow = maxIt;
+        if (!P.unbatched && reg == kNumReg - 1)
+            maxItNow = 10 * maxIt;
+        ++iter;
+        ++tot;
+        g = own ? D.row[0] / P.m0 - uu : 0.0;  // fobjMnUseH
+        f = rdl(D.row[0], 0) / (P.m0 * P.m0);
+#pragma unroll
+        for (int i = 0; i < NM; ++i)
+            f = f - rdl(a, i) * rdl(uu, i);
+        double rs = 0.0;  // utils_norm1
+#pragma unroll
+        for (int j = 0; j < NM; ++j)
+            rs = rs + fabs(D.row[j]);
+        double norm1 = 0.0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            double vv = rdl(rs, i);
+            if (vv > norm1)
+                norm1 = vv;
+        }
+        const int cholErr = D.cholesky(lane);
+        double condest = 0.0;
+        if (cholErr == 0) {
+            D.build_columns(sT, LD, lane);
+            condest = 1.0 / D.pocon(norm1, lane);
+        }
+        if (tot == 1 && (cholErr != 0 || condest > P.condHMax)) {
+            a0 = (lane == 0) ? P.isoAlpha0 : 0.0;  // isotropic restart, optBatch.cpp:208-218
+            a = a0;
+            if (P.unbatched)
+                rtol = LaneVec<NM>::dnrm2(uu) * P.tol;
+            iter = 0;
+            igam = 0;
+            tprev = 1.0;
+            flags |= C3D_FLAG_ISO_RESTART;
+            phase = kPhaseH;
+            return 0;
+        }
+        if (iter >= maxItNow || cholErr != 0 || tprev == 0.0 || condest > P.condHMax) {
+            if (reg < kNumReg - 1) {  // ladder, optBatch.cpp:222-248
+                ++reg;
+                r = kReg[reg];
+                uu = (lane == 0) ? 1.0 : (own ? uOrig * (1.0 - r) : 0.0);
+                if (P.unbatched)
+                    rtol = LaneVec<NM>::dnrm2(uu) * P.tol;
+                a = a0;
+                igam = 0;
+                iter = 0;
+                tprev = 1.0;
+                if (P.unbatched && reg == kNumReg - 1)
+                    maxIt = 10 * maxIt;
+                phase = kPhaseH;
+                return 0;
+            }
+            return 2;
+        }
+        d = g;
+        if (P.unbatched)
+            D.potrs_lapack(d, lane);
+        else
+            D.potrs_custom(d, lane);
+        d = own ? -d : 0.0;
+        err = LaneVec<NM>::dnrm2(g);
+        double da1 = 0.0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i)
+            da1 = da1 + fabs(rdl(d, i));
+        gamma = det_exp(2.0 * da1 * P.minf);
+        if (err <= rtol && gamma <= tolGamma)
+            return 1;
+        if (err <= rtol && gamma > tolGamma)
+            ++igam;
+        aOld = a;  // linesearch.cpp:29-54
+        maxF = 0.0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            double temp = fabs(rdl(d, i)) / (fabs(rdl(aOld, i)) + 1e-100);
+            if (temp > maxF)
+                maxF = temp;
+        }
+        gd = 0.0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i)
+            gd = gd + rdl(g, i) * rdl(d, i);
+        tls = 1.0;
+        if (tls * maxF > DBL_EPSILON / 2) {
+            a = own ? aOld + tls * d : 0.0;
+            phase = kPhaseLS;
+        } else {
+            a = aOld;
+            tprev = 0.0;
+            phase = kPhaseH;
+        }
+        return 0;
+    }
+};
 
 // ---------------------------------------------------------------- step pass
 template <int NM>
@@ -598,7 +876,6 @@ __global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, const int *
     constexpr int LD = NM + 1;
     __shared__ double sT[kWavesPerBlock][NM * LD];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const bool own = lane < NM;
     const int nW = uniform_int(*nWp);
     if (blockIdx.x == 0 && threadIdx.x == 0 && sweepCount)
         atomicAdd(sweepCount, (unsigned long long)nW);
@@ -608,192 +885,128 @@ __global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, const int *
         if (slot >= nW)
             continue;
         const int c = uniform_int(W[slot]);
-        double *v = S.vec + (size_t)c * kVec * NM;
-        double *sc = S.scal + (size_t)c * kScal;
-        int *in = S.ints + (size_t)c * kInt;
-        auto ld = [&](int f) { return own ? v[f * NM + lane] : 0.0; };
-        double a = ld(kVa), a0 = ld(kVa0), aOld = ld(kVaOld), d = ld(kVd), g = ld(kVg),
-               uu = ld(kVu), uOrig = ld(kVuOrig);
-        const double u0 = sc[kSu0];
-        double f = sc[kSf], tls = sc[kStls], tprev = sc[kStprev], maxFactor = sc[kSmaxF],
-               gd = sc[kSgd], err = sc[kSerr], gamma = sc[kSgamma], r = sc[kSr],
-               rtol = sc[kSrtol];
-        int iter = in[kIiter], totalIter = in[kItot], iterGamma = in[kIgam],
-            regIndex = in[kIreg], flags = in[kIflags], phase = in[kIphase],
-            maxIterCur = in[kImaxIt];
-        const double tolGamma = P.unbatched ? 1.1 : P.tolGamma;
-        const double fdir = S.fdir[c];
-        int done = 0;  // 0 continue, 1 converged, 2 ladder exhausted
-
-        bool doH = true;
-        if (phase == kPhaseLS) {
-            double fNew = fdir;
-#pragma unroll
-            for (int i = 0; i < NM; ++i)
-                fNew = fNew - rdl(a, i) * rdl(uu, i);
-            if (fNew <= f + 1e-3 * tls * gd) {
-                tprev = tls;  // accepted; H of this sweep belongs to the new alpha
-            } else {
-                tls = 0.5 * tls;
-                if (tls * maxFactor > DBL_EPSILON / 2) {
-                    a = own ? aOld + tls * d : 0.0;
-                    phase = kPhaseLS;
-                } else {
-                    a = aOld;
-                    tprev = 0.0;
-                    phase = kPhaseH;
-                }
-                doH = false;
-            }
-        }
-        if (doH) {
-            // ============================ one Newton iteration (solveTheRest)
-            Dense<NM> D;
-            const double *H = S.H + (size_t)c * NM * NM;
-#pragma unroll
-            for (int j = 0; j < NM; ++j)
-                D.row[j] = own ? H[lane * NM + j] : 0.0;
-            int maxIt = maxIterCur;
-            if (!P.unbatched && regIndex == kNumReg - 1)
-                maxIt = 10 * maxIterCur;
-            ++iter;
-            ++totalIter;
-            g = own ? D.row[0] / P.m0 - uu : 0.0;  // fobjMnUseH
-            f = rdl(D.row[0], 0) / (P.m0 * P.m0);
-#pragma unroll
-            for (int i = 0; i < NM; ++i)
-                f = f - rdl(a, i) * rdl(uu, i);
-            double rs = 0.0;  // utils_norm1
-#pragma unroll
-            for (int j = 0; j < NM; ++j)
-                rs = rs + fabs(D.row[j]);
-            double norm1 = 0.0;
-#pragma unroll
-            for (int i = 0; i < NM; ++i) {
-                double vv = rdl(rs, i);
-                if (vv > norm1)
-                    norm1 = vv;
-            }
-            const int cholErr = D.cholesky(lane);
-            double condest = 0.0;
-            if (cholErr == 0) {
-                D.build_columns(sT[wv], LD, lane);
-                condest = 1.0 / D.pocon(norm1, lane);
-            }
-            if (totalIter == 1 && (cholErr != 0 || condest > P.condHMax)) {
-                a0 = (lane == 0) ? P.isoAlpha0 : 0.0;  // isotropic restart, optBatch.cpp:208-218
-                a = a0;
-                if (P.unbatched)
-                    rtol = LaneVec<NM>::dnrm2(uu) * P.tol;
-                iter = 0;
-                iterGamma = 0;
-                tprev = 1.0;
-                flags |= C3D_FLAG_ISO_RESTART;
-                phase = kPhaseH;
-            } else if (iter >= maxIt || cholErr != 0 || tprev == 0.0 || condest > P.condHMax) {
-                if (regIndex < kNumReg - 1) {  // ladder, optBatch.cpp:222-248
-                    ++regIndex;
-                    r = kReg[regIndex];
-                    uu = (lane == 0) ? 1.0 : (own ? uOrig * (1.0 - r) : 0.0);
-                    if (P.unbatched)
-                        rtol = LaneVec<NM>::dnrm2(uu) * P.tol;
-                    a = a0;
-                    iterGamma = 0;
-                    iter = 0;
-                    tprev = 1.0;
-                    if (P.unbatched && regIndex == kNumReg - 1)
-                        maxIterCur = 10 * maxIterCur;
-                    phase = kPhaseH;
-                } else {
-                    done = 2;
-                }
-            } else {
-                d = g;
-                if (P.unbatched)
-                    D.potrs_lapack(d, lane);
-                else
-                    D.potrs_custom(d, lane);
-                d = own ? -d : 0.0;
-                err = LaneVec<NM>::dnrm2(g);
-                double da1 = 0.0;
-#pragma unroll
-                for (int i = 0; i < NM; ++i)
-                    da1 = da1 + fabs(rdl(d, i));
-                gamma = det_exp(2.0 * da1 * P.minf);
-                if (err <= rtol && gamma <= tolGamma) {
-                    done = 1;
-                } else {
-                    if (err <= rtol && gamma > tolGamma)
-                        ++iterGamma;
-                    aOld = a;  // linesearch.cpp:29-54
-                    maxFactor = 0.0;
-#pragma unroll
-                    for (int i = 0; i < NM; ++i) {
-                        double temp = fabs(rdl(d, i)) / (fabs(rdl(aOld, i)) + 1e-100);
-                        if (temp > maxFactor)
-                            maxFactor = temp;
-                    }
-                    gd = 0.0;
-#pragma unroll
-                    for (int i = 0; i < NM; ++i)
-                        gd = gd + rdl(g, i) * rdl(d, i);
-                    tls = 1.0;
-                    if (tls * maxFactor > DBL_EPSILON / 2) {
-                        a = own ? aOld + tls * d : 0.0;
-                        phase = kPhaseLS;
-                    } else {
-                        a = aOld;
-                        tprev = 0.0;
-                        phase = kPhaseH;
-                    }
-                }
-            }
-        }
+        Cell<NM> C;
+        C.load(S, c, lane);
+        const int done = C.advance(P, S.H + (size_t)c * NM * NM, S.fdir[c], sT[wv], lane);
         if (done) {
-            if (done == 2)
-                flags |= C3D_FLAG_LADDER;
-            const double alphaScale = det_log(u0) * 2.0 * sqrt(M_PI);  // postProcess
-            if (own) {
-                u_g[(size_t)c * NM + lane] = uu * u0;
-                alpha_g[(size_t)c * NM + lane] = (lane == 0) ? a + alphaScale : a;
-            }
-            if (lane == 0) {
-                if (done == 2)
-                    atomicOr(status, C3D_FLAG_LADDER);
-                if (stats) {
-                    c3d_cell_stats s0{r, err, gamma, totalIter, iterGamma, regIndex, flags};
-                    stats[c] = s0;
-                }
-            }
+            C.finish(done, c, u_g, alpha_g, stats, status, lane);
         } else {
-            if (own) {
-                v[kVa * NM + lane] = a;
-                v[kVa0 * NM + lane] = a0;
-                v[kVaOld * NM + lane] = aOld;
-                v[kVd * NM + lane] = d;
-                v[kVg * NM + lane] = g;
-                v[kVu * NM + lane] = uu;
-            }
-            if (lane == 0) {
-                sc[kSf] = f;
-                sc[kStls] = tls;
-                sc[kStprev] = tprev;
-                sc[kSmaxF] = maxFactor;
-                sc[kSgd] = gd;
-                sc[kSerr] = err;
-                sc[kSgamma] = gamma;
-                sc[kSr] = r;
-                sc[kSrtol] = rtol;
-                in[kIiter] = iter;
-                in[kItot] = totalIter;
-                in[kIgam] = iterGamma;
-                in[kIreg] = regIndex;
-                in[kIflags] = flags;
-                in[kIphase] = phase;
-                in[kImaxIt] = maxIterCur;
+            C.store(S, c, lane);
+            if (lane == 0)
                 Wn[atomicAdd(nWn, 1)] = c;
-            }
         }
+    }
+}
+
+// ---------------------------------------------------------------- fused tail pass
+// For the last few cells: one wave per cell runs up to `passes` sweep+step rounds in one
+// launch, with the cell's state in registers, its alpha, M chunks, H and the transposed
+// factor in the wave's own LDS.  The sweep performs the same operations in the same order as
+// k_sweep / k_sweep_ref do for that cell, so a cell's result does not depend on which kernel
+// finished it.  Launch: one 64-thread block per listed cell (grid >= *nWp).
+template <int NM, bool REF>
+__global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restrict__ Mg,
+                                                 const double *__restrict__ wg, State S,
+                                                 const int *__restrict__ W,
+                                                 const int *__restrict__ nWp, int *Wn, int *nWn,
+                                                 double *u_g, double *alpha_g,
+                                                 c3d_cell_stats *stats, int *status, int passes,
+                                                 unsigned long long *sweepCount)
+{
+    constexpr int NT = (NM + 15) / 16;
+    constexpr int NB = NT * (NT + 1) / 2;
+    constexpr int NMS = NM | 1;
+    static_assert(NM * NM + NM * (NM + 1) <= kChunk * NMS, "H and factor alias the M chunk");
+    __shared__ double sM[kChunk * NMS];
+    __shared__ double sW[kChunk];
+    __shared__ double sE[kChunk];
+    __shared__ double sA[NM];
+    const int lane = threadIdx.x;
+    const int nW = uniform_int(*nWp);
+    const int slot = blockIdx.x;
+    if (slot >= nW)
+        return;
+    const int c = uniform_int(W[slot]);
+    Cell<NM> C;
+    C.load(S, c, lane);
+    double *sH = sM;               // valid between a sweep and the next one
+    double *sT = sM + NM * NM;
+    const int nch = P.nqPad / kChunk;
+    int done = 0;
+    int sweeps = 0;
+    for (int p = 0; p < passes; ++p) {
+        if (lane < NM)
+            sA[lane] = C.a;
+        double fdir;
+        if constexpr (REF) {
+            const RefEntries<NM> E(lane);
+            double h[RefEntries<NM>::NS];
+#pragma unroll
+            for (int s2 = 0; s2 < RefEntries<NM>::NS; ++s2)
+                h[s2] = 0.0;
+            double f = 0.0;
+            for (int ch = 0; ch < nch; ++ch) {
+                wave_lds_sync();
+                for (int e = lane; e < kChunk * NM; e += kWave) {
+                    const int rr = e / NM, cc = e - rr * NM;
+                    sM[rr * NMS + cc] = Mg[(size_t)ch * kChunk * NM + e];
+                }
+                sW[lane] = wg[(size_t)ch * kChunk + lane];
+                wave_lds_sync();
+                chunk_ref<NM>(sM, sW, sE, sA, E, h, f, min(kChunk, P.nq - ch * kChunk), lane);
+            }
+            wave_lds_sync();
+            store_H_ref<NM>(E, h, sH, lane);
+            fdir = f;
+        } else {
+            d4 acc[1][NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+                acc[0][b] = d4{0.0, 0.0, 0.0, 0.0};
+            double fpart[1] = {0.0};
+            double pre[NM];
+            double prew = wg[lane];
+#pragma unroll
+            for (int k = 0; k < NM; ++k)
+                pre[k] = Mg[lane + kWave * k];
+            for (int ch = 0; ch < nch; ++ch) {
+                wave_lds_sync();
+#pragma unroll
+                for (int k = 0; k < NM; ++k) {
+                    const int e = lane + kWave * k;
+                    const int rr = e / NM, cc = e - rr * NM;
+                    sM[rr * NMS + cc] = pre[k];
+                }
+                const double wl = prew;
+                wave_lds_sync();
+                if (ch + 1 < nch) {
+                    const size_t base = (size_t)(ch + 1) * kChunk;
+                    prew = wg[base + lane];
+#pragma unroll
+                    for (int k = 0; k < NM; ++k)
+                        pre[k] = Mg[base * NM + lane + kWave * k];
+                }
+                chunk_mfma<NM, 1>(sM, wl, sA, acc, fpart, lane);
+            }
+            fdir = wave_sum(fpart[0]);
+            wave_lds_sync();
+            store_H<NM>(acc[0], sH, lane);
+        }
+        wave_lds_sync();
+        ++sweeps;
+        done = C.advance(P, sH, fdir, sT, lane);
+        wave_lds_sync();
+        if (done)
+            break;
+    }
+    if (lane == 0 && sweepCount)
+        atomicAdd(sweepCount, (unsigned long long)sweeps);
+    if (done) {
+        C.finish(done, c, u_g, alpha_g, stats, status, lane);
+    } else {
+        C.store(S, c, lane);
+        if (lane == 0)
+            Wn[atomicAdd(nWn, 1)] = c;
     }
 }
 
@@ -938,9 +1151,25 @@ __global__ __launch_bounds__(kBlock) void k_dense_solve(int64_t n, const double 
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
-__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+__global__ void k_zero_u64(unsigned long long *p) { p[threadIdx.x] = 0ull; }
 
 unsigned grid_cap(c3d_ctx_s *ctx, int perCU) { return (unsigned)(ctx->numCU * perCU); }
+
+hipEvent_t *take_events(c3d_ctx_s *ctx, int n, int kind)
+{
+    if ((int)ctx->evPool.size() < ctx->evUsed + n) {
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t x;
+            if (hipEventCreate(&x) != hipSuccess)
+                return nullptr;
+            ctx->evPool.push_back(x);
+        }
+    }
+    ctx->evSpans.push_back({kind, ctx->evUsed});
+    hipEvent_t *ev = &ctx->evPool[ctx->evUsed];
+    ctx->evUsed += n;
+    return ev;
+}
 
 template <int NM>
 hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alpha,
@@ -962,60 +1191,81 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
     int *W0 = S.ints + n * kInt;
     int *W1 = W0 + n;
     int *counts = W1 + n;  // [0],[1] worklist sizes
+    const bool ref = ctx->precision == C3D_PREC_FP64_REFORDER;
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, counts + 0, 0);
     ctx->evUsed = 0;
+    ctx->evSpans.clear();
     ctx->lastPasses = 0;
+    ctx->lastTailLaunches = 0;
     ctx->lastStream = s;
+    unsigned long long *cntBulk = ctx->profiling ? ctx->d_sweepCount : nullptr;
+    unsigned long long *cntTail = ctx->profiling ? ctx->d_sweepCount + 1 : nullptr;
     if (ctx->profiling)
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, ctx->d_sweepCount);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(2), 0, s, ctx->d_sweepCount);
     const unsigned gInit = (unsigned)((n * 64 + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_init<NM>, dim3(gInit), dim3(kBlock), 0, s, P, u, alpha, S, W0, counts,
                        stats, ctx->d_status);
     int cur = 0;
-    for (int pass = 0; pass < maxPasses; ++pass) {
+    int64_t left = (int64_t)n;  // upper bound on the listed cells until the first read-back
+    int pass = 0;
+    while (pass < maxPasses) {
         int *Wc = cur ? W1 : W0, *Wn = cur ? W0 : W1;
         int *nc = counts + cur, *nn = counts + (1 - cur);
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, nn, 0);
-        hipEvent_t *ev = nullptr;
-        if (ctx->profiling) {
-            if ((int)ctx->evPool.size() < ctx->evUsed + 4) {
-                for (int k = 0; k < 64; ++k) {
-                    hipEvent_t x;
-                    if (hipEventCreate(&x) != hipSuccess)
-                        return hipErrorOutOfMemory;
-                    ctx->evPool.push_back(x);
-                }
+        const bool tail = ctx->tailCells > 0 && left <= ctx->tailCells;
+        if (tail) {
+            const int K = std::min(ctx->tailPasses, maxPasses - pass);
+            hipEvent_t *ev = ctx->profiling ? take_events(ctx, 2, 1) : nullptr;
+            if (ctx->profiling && !ev)
+                return hipErrorOutOfMemory;
+            if (ev)
+                hipEventRecord(ev[0], s);
+            if (ref)
+                hipLaunchKernelGGL((k_tail<NM, true>), dim3((unsigned)left), dim3(kWave), 0, s,
+                                   P, ctx->d_M, ctx->d_wpad, S, Wc, nc, Wn, nn, u, alpha, stats,
+                                   ctx->d_status, K, cntTail);
+            else
+                hipLaunchKernelGGL((k_tail<NM, false>), dim3((unsigned)left), dim3(kWave), 0, s,
+                                   P, ctx->d_M, ctx->d_wpad, S, Wc, nc, Wn, nn, u, alpha, stats,
+                                   ctx->d_status, K, cntTail);
+            if (ev)
+                hipEventRecord(ev[1], s);
+            pass += K;
+            ++ctx->lastTailLaunches;
+        } else {
+            hipEvent_t *ev = ctx->profiling ? take_events(ctx, 4, 0) : nullptr;
+            if (ctx->profiling && !ev)
+                return hipErrorOutOfMemory;
+            if (ev)
+                hipEventRecord(ev[0], s);
+            if (ref)
+                hipLaunchKernelGGL(k_sweep_ref<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s,
+                                   P, ctx->d_M, ctx->d_wpad, S, Wc, nc);
+            else
+                hipLaunchKernelGGL(k_sweep<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P,
+                                   ctx->d_M, ctx->d_wpad, S, Wc, nc);
+            if (ev) {
+                hipEventRecord(ev[1], s);
+                hipEventRecord(ev[2], s);
             }
-            ev = &ctx->evPool[ctx->evUsed];
-            ctx->evUsed += 4;
-            hipEventRecord(ev[0], s);
+            hipLaunchKernelGGL(k_step<NM>, dim3(grid_cap(ctx, 16)), dim3(kBlock), 0, s, P, S, Wc,
+                               nc, Wn, nn, u, alpha, stats, ctx->d_status, cntBulk);
+            if (ev)
+                hipEventRecord(ev[3], s);
+            ++pass;
         }
-        if (ctx->precision == C3D_PREC_FP64_REFORDER)
-            hipLaunchKernelGGL(k_sweep_ref<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P,
-                               ctx->d_M, ctx->d_wpad, S, Wc, nc);
-        else
-            hipLaunchKernelGGL(k_sweep<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P,
-                               ctx->d_M, ctx->d_wpad, S, Wc, nc);
-        if (ev) {
-            hipEventRecord(ev[1], s);
-            hipEventRecord(ev[2], s);
-        }
-        hipLaunchKernelGGL(k_step<NM>, dim3(grid_cap(ctx, 16)), dim3(kBlock), 0, s, P, S, Wc, nc,
-                           Wn, nn, u, alpha, stats, ctx->d_status,
-                           ctx->profiling ? ctx->d_sweepCount : nullptr);
-        if (ev)
-            hipEventRecord(ev[3], s);
-        ctx->lastPasses = pass + 1;
+        ctx->lastPasses = pass;
         cur = 1 - cur;
-        if ((pass & 7) == 7) {
-            int left = 0;
-            e = hipMemcpyAsync(&left, counts + cur, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (tail || (pass & 7) == 0) {
+            int lft = 0;
+            e = hipMemcpyAsync(&lft, counts + cur, sizeof(int), hipMemcpyDeviceToHost, s);
             if (e == hipSuccess)
                 e = hipStreamSynchronize(s);
             if (e != hipSuccess)
                 return e;
-            if (left == 0)
+            if (lft == 0)
                 return hipGetLastError();
+            left = lft;
         }
     }
     hipLaunchKernelGGL(k_budget<NM>, dim3(64), dim3(256), 0, s, cur ? W1 : W0, counts + cur,

@@ -158,13 +158,15 @@ int c3d_closure(c3d_ctx ctx, int64_t ncells, double *u, double *alpha,
 int c3d_set_precision(c3d_ctx ctx, int mode);
 
 /* Kernel-time accounting of c3d_closure (HIP events on the closure's stream around every
- * sweep and step launch), for throughput and roofline reporting. */
+ * launch), for throughput and roofline reporting. */
 typedef struct {
-    int64_t cell_sweeps; /* cell evaluations of H(alpha) (Newton iterations + line-search trials) */
-    int32_t passes;      /* worklist passes */
-    int32_t reserved;
-    double sweep_ms;     /* summed duration of the sweep kernels */
-    double step_ms;      /* summed duration of the Newton-step kernels */
+    int64_t cell_sweeps;      /* cell evaluations of H(alpha) by the bulk sweep kernel */
+    int64_t tail_cell_sweeps; /* cell evaluations of H(alpha) inside the fused tail kernel */
+    int32_t passes;           /* worklist passes (a tail launch counts its pass limit) */
+    int32_t tail_launches;
+    double sweep_ms;          /* summed duration of the bulk sweep kernels */
+    double step_ms;           /* summed duration of the bulk Newton-step kernels */
+    double tail_ms;           /* summed duration of the fused tail kernels */
 } c3d_closure_profile;
 
 int c3d_set_profiling(c3d_ctx ctx, int enable);
@@ -176,6 +178,13 @@ int c3d_get_closure_profile(c3d_ctx ctx, c3d_closure_profile *out);
  * reference has no such bound (its worst case is 1400 iterations x ~1100 backtracking
  * steps per cell).  No counterpart in the reference. */
 int c3d_set_pass_budget(c3d_ctx ctx, int max_passes);
+
+/* Once at most max_cells cells remain unconverged, c3d_closure switches from one launch pair
+ * per pass (sweep kernel over all listed cells, then step kernel) to a fused kernel in which
+ * one wavefront runs up to passes_per_launch sweep+step rounds for its cell.  Results are
+ * identical either way (same operations in the same order per cell).  max_cells = 0
+ * disables the fused kernel.  Defaults 8192 and 32.  No counterpart in the reference. */
+int c3d_set_tail(c3d_ctx ctx, int max_cells, int passes_per_launch);
 
 /* H(alpha) for ncells cells (solveHMn semantics), device pointers, hs [ncells][nm][nm]. */
 int c3d_hessian(c3d_ctx ctx, int64_t ncells, const double *alpha, double *hs, int useGaunt,

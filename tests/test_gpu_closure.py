@@ -26,10 +26,25 @@ def _tables(O, N, Q):
     return O.Tables(N, Q)
 
 
-def _run_gpu_closure(P, dev, T, u, a, opts, precision=0):
+# worklist strategies of c3d_closure: launch pairs only, the fused tail kernel only (few
+# passes per launch, so cells resume across launches), and a switch part way through
+TAILS = ("bulk", "fused", "switch")
+
+
+def _set_tail(ctx, tail, n):
+    if tail == "bulk":
+        ctx.set_tail(0)
+    elif tail == "fused":
+        ctx.set_tail(1 << 30, 7)
+    elif tail == "switch":
+        ctx.set_tail(max(1, n // 5), 3)
+
+
+def _run_gpu_closure(P, dev, T, u, a, opts, precision=0, tail=None):
     import torch
     ctx = P.Ctx(tables=T)
     ctx.set_precision(precision)
+    _set_tail(ctx, tail, u.shape[0])
     ud = torch.from_numpy(u.copy()).to(dev)
     ad = torch.from_numpy(a.copy()).to(dev)
     st = P.alloc_stats(u.shape[0], dev)
@@ -55,10 +70,10 @@ def hess_lmin(T, a):
     return np.linalg.eigvalsh(H)[:, 0]
 
 
-def _compare(O, T, u_in, a_in, o_opts, p_opts, P, dev, min_match=0.98):
+def _compare(O, T, u_in, a_in, o_opts, p_opts, P, dev, min_match=0.98, tail=None):
     uo, ao = u_in.copy(), a_in.copy()
     rc_o, so = O.closure(T, uo, ao, o_opts)
-    ug, ag, sg, rc_g = _run_gpu_closure(P, dev, T, u_in, a_in, p_opts)
+    ug, ag, sg, rc_g = _run_gpu_closure(P, dev, T, u_in, a_in, p_opts, tail=tail)
     same = np.ones(len(so), bool)
     for k in KEYS:
         same &= so[k] == sg[k]
@@ -81,12 +96,12 @@ def _compare(O, T, u_in, a_in, o_opts, p_opts, P, dev, min_match=0.98):
     return so, sg, same, rc_o, rc_g
 
 
-def _bitwise(O, T, u_in, a_in, o_opts, p_opts, P, dev):
+def _bitwise(O, T, u_in, a_in, o_opts, p_opts, P, dev, tail=None):
     uo, ao = u_in.copy(), a_in.copy()
     with O.exp_mode(1):
         rc_o, so = O.closure(T, uo, ao, o_opts)
     ug, ag, sg, rc_g = _run_gpu_closure(P, dev, T, u_in, a_in, p_opts,
-                                        precision=P.PREC_FP64_REFORDER)
+                                        precision=P.PREC_FP64_REFORDER, tail=tail)
     for k in KEYS + ("r", "err", "gamma"):
         diff = np.where(so[k] != sg[k])[0]
         assert len(diff) == 0, (k, diff[:5], so[diff[:2]], sg[diff[:2]])
@@ -118,34 +133,57 @@ def test_hessian_matches_oracle(P, O, dev, N, Q):
     assert rel < 1e-14, rel
 
 
+@pytest.mark.parametrize("tail", ["bulk", "switch"])
 @pytest.mark.parametrize("N,Q,n", [(3, 24, 3000), (2, 8, 2000), (1, 6, 1000), (4, 12, 600)])
-def test_closure_batched_matches_oracle(P, O, dev, N, Q, n):
+def test_closure_batched_matches_oracle(P, O, dev, N, Q, n, tail):
     T = _tables(O, N, Q)
     a_star, u, a0 = O.closure_batch_input(T, n, 0xC3D2015 + N)
-    so, sg, same, rc_o, rc_g = _compare(O, T, u, a0, O.opts(), P.closure_opts(), P, dev)
+    so, sg, same, rc_o, rc_g = _compare(O, T, u, a0, O.opts(), P.closure_opts(), P, dev,
+                                        tail=tail)
     assert rc_o == 0 and rc_g in (0, P.C3D_ERR_LADDER)
     assert np.all(sg["err"][sg["flags"] & 4 == 0] <= 1e-8)
 
 
+@pytest.mark.parametrize("tail", TAILS)
 @pytest.mark.parametrize("N,Q,n,smax", [(3, 24, 1500, 8.0), (2, 8, 1500, 8.0), (1, 6, 800, 8.0),
                                         (4, 12, 300, 6.0)])
-def test_closure_reference_order_bit_exact(P, O, dev, N, Q, n, smax):
+def test_closure_reference_order_bit_exact(P, O, dev, N, Q, n, smax, tail):
     T = _tables(O, N, Q)
     a_star, u, a0 = O.closure_batch_input(T, n, 0xC3D2015 + 10 * N, 0.5, smax)
-    so, rc_o, rc_g = _bitwise(O, T, u, a0, O.opts(), P.closure_opts(), P, dev)
+    so, rc_o, rc_g = _bitwise(O, T, u, a0, O.opts(), P.closure_opts(), P, dev, tail=tail)
     assert rc_o == 0 and rc_g == 0
 
 
-def test_closure_reference_order_ladder_restart_unbatched(P, O, dev):
+@pytest.mark.parametrize("N,Q,n", [(3, 24, 2500), (4, 12, 500), (1, 6, 700)])
+def test_closure_result_independent_of_worklist_strategy(P, O, dev, N, Q, n):
+    """The matrix-core mode gives the same bits per cell whether a cell finishes in the
+    launch-pair passes, in the fused tail kernel, or moves between them."""
+    T = _tables(O, N, Q)
+    a_star, u, a0 = O.closure_batch_input(T, n, 0xC3D2015 + 3 * N, 0.5, 8.0)
+    outs = [_run_gpu_closure(P, dev, T, u, a0, P.closure_opts(), tail=t) for t in TAILS]
+    for ug, ag, sg, rc in outs[1:]:
+        assert rc == outs[0][3]
+        assert np.array_equal(ag, outs[0][1]) and np.array_equal(ug, outs[0][0])
+        for k in KEYS + ("r", "err", "gamma"):
+            assert np.array_equal(sg[k], outs[0][2][k]), k
+    # the bulk run of a subset gives the same bits for those cells (batch invariance)
+    sub = np.arange(0, n, 3)
+    ug, ag, sg, rc = _run_gpu_closure(P, dev, T, u[sub], a0[sub], P.closure_opts(), tail="bulk")
+    assert np.array_equal(ag, outs[0][1][sub]) and np.array_equal(ug, outs[0][0][sub])
+
+
+@pytest.mark.parametrize("tail", TAILS)
+def test_closure_reference_order_ladder_restart_unbatched(P, O, dev, tail):
     T = _tables(O, 2, 8)
     a_star, u, a0 = O.closure_batch_input(T, 400, 5, 2.0, 6.0)
-    so, _, _ = _bitwise(O, T, u, a0, O.opts(maxIter=2), P.closure_opts(maxIter=2), P, dev)
+    so, _, _ = _bitwise(O, T, u, a0, O.opts(maxIter=2), P.closure_opts(maxIter=2), P, dev,
+                        tail=tail)
     assert (so["regIndex"] > 0).any()
     so, _, _ = _bitwise(O, T, u, 2.0 * a_star, O.opts(condHMax=1e4),
-                        P.closure_opts(condHMax=1e4), P, dev)
+                        P.closure_opts(condHMax=1e4), P, dev, tail=tail)
     assert (so["flags"] & 1).any()
     _bitwise(O, T, u, a0, O.opts(unbatched=True, tol=1e-9),
-             P.closure_opts(unbatched=True, tol=1e-9), P, dev)
+             P.closure_opts(unbatched=True, tol=1e-9), P, dev, tail=tail)
 
 
 def test_closure_unbatched_semantics(P, O, dev):
@@ -236,3 +274,26 @@ def test_dense_solve_bit_exact(P, O, dev, nm, lapack_order):
         d = gs[c].copy()
         (L.orc_dpotrs_lapack if lapack_order else L.orc_dpotrs_custom)(nm, Af, d)
         assert np.array_equal(-d, d_g[c]), c
+
+
+@pytest.mark.parametrize("tail", ["bulk", "fused"])
+def test_closure_pass_budget_reports_and_leaves_cells_untouched(P, O, dev, tail):
+    import torch
+    T = _tables(O, 2, 8)
+    n = 300
+    a_star, u, a0 = O.closure_batch_input(T, n, 31, 0.5, 8.0)
+    ufull, afull, sfull, _ = _run_gpu_closure(P, dev, T, u, a0, P.closure_opts(), tail=tail)
+    ctx = P.Ctx(tables=T)
+    _set_tail(ctx, tail, n)
+    ctx.set_pass_budget(5)
+    ud, ad = torch.from_numpy(u.copy()).to(dev), torch.from_numpy(a0.copy()).to(dev)
+    st = P.alloc_stats(n, dev)
+    ctx.closure(ud, ad, P.closure_opts(), stats=st)
+    rc = ctx.sync()
+    ug, ag, sg = ud.cpu().numpy(), ad.cpu().numpy(), P.stats_to_numpy(st)
+    ctx.close()
+    left = (sg["flags"] & P.FLAG_BUDGET) != 0
+    assert rc == P.C3D_ERR_BUDGET and left.any() and not left.all()
+    assert np.array_equal(ug[left], u[left]) and np.array_equal(ag[left], a0[left])
+    # cells that finished inside the budget are the unbudgeted run's cells, bit for bit
+    assert np.array_equal(ag[~left], afull[~left]) and np.array_equal(ug[~left], ufull[~left])
