@@ -1,18 +1,25 @@
 """Benchmark of the M_N closure hot path on one MI355X.
 
 Workload (the metric's configuration): 1,048,576 independent M_3 dual problems (16
-moments), tensor quadrature 24 mu x 48 phi = 1152 ordinates, FP64.  Per cell s ~ U[0.5, 8],
+moments), tensor quadrature 24 mu x 48 phi = 1152 ordinates.  Per cell s ~ U[0.5, 8],
 alpha*_k ~ N(0, s^2), u = <m e^{alpha*.m}>; Newton starts from the isotropic alpha of u
-(init.cpp:265) and stops at ||grad h|| <= 1e-8 on the u0-scaled problem with gamma <= 1.1
-(optBatch.cpp:268), MN_MAX_ITER 100, MN_COND_H_MAX 1e10.  The inputs are generated on the
-device once and stay in HBM; a step restores u and alpha from pristine device copies and
-solves the whole batch.
+(init.cpp:265) and stops at ||grad h|| <= 1e-8 (FP64) on the u0-scaled problem with
+gamma <= 1.1 (optBatch.cpp:268), MN_MAX_ITER 100, MN_COND_H_MAX 1e10.  The inputs are
+generated on the device once and stay in HBM; a step restores u and alpha from pristine
+device copies and solves the whole batch.
 
-One JSON line: value = cells solved per second (whole batch / step time); roofline of the
-dominant kernel (the Hessian sweep on the FP64 matrix cores) from HIP events recorded on the
-closure's stream; the CPU oracle timed on a bounded sample of the same inputs.
+Paths (--path): "fp64" plain FP64 (matrix-core Hessian sums); "mixed" FP32 matrix-core
+Hessian under a log-sum-exp shift with FP64 gradient/f/Cholesky/stopping test and FP64
+hand-over of ill-conditioned cells; "+nn" adds the learned warm start, trained first (not
+timed) on --train-pairs (u, alpha) pairs that the FP64 closure solves from a different seed.
+The plain FP64 path is always timed beside the chosen one (same inputs, same process), and
+the chosen path's outputs are compared with its outputs cell by cell.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--cells 1048576] [--cpu-sample 65536]
+One JSON line: value = cells solved per second by the chosen path; roofline of the FP64
+sweep kernel (HIP events on the closure's stream, cell-sweeps counted on the device);
+the CPU oracle timed on a bounded sample of the same inputs.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--path mixed+nn] [--cpu-sample 65536]
 """
 import argparse
 import importlib.util
@@ -44,12 +51,15 @@ def cpu_baseline(u_host, a_host, N, Q, nthreads):
     import oracle as O
     T = O.Tables(N, Q)
     n = u_host.shape[0]
-    # warm-up on a small slice, then time the sample
     O.closure(T, u_host[:64].copy(), a_host[:64].copy(), O.opts(), nthreads)
     t0 = time.perf_counter()
     O.closure(T, u_host.copy(), a_host.copy(), O.opts(), nthreads)
     dt = time.perf_counter() - t0
     return n / dt, dt
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def main():
@@ -60,6 +70,15 @@ def main():
     ap.add_argument("--cells", type=int, default=1 << 20)
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--seed", type=lambda s: int(s, 0), default=0xC3D2015)
+    ap.add_argument("--path", choices=("fp64", "mixed", "fp64+nn", "mixed+nn"),
+                    default="mixed+nn")
+    ap.add_argument("--mixed-cond", type=float, default=1e8)
+    ap.add_argument("--mixed-iter", type=int, default=40)
+    ap.add_argument("--train-pairs", type=int, default=1 << 22)
+    ap.add_argument("--train-steps", type=int, default=2000, help="MSE steps")
+    ap.add_argument("--dual-steps", type=int, default=3000, help="dual-gap steps")
+    ap.add_argument("--train-batch", type=int, default=8192)
+    ap.add_argument("--train-lr", type=float, default=3e-3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the closure is "
@@ -73,7 +92,8 @@ def main():
     N, Q = 3, 24
     n = args.cells
     ctx = P.Ctx(N, Q)
-    nm = ctx.nm
+    ctx.set_mixed(args.mixed_cond, args.mixed_iter)
+    nm, nq = ctx.nm, ctx.nq
     u0 = torch.empty((n, nm), dtype=torch.float64, device=dev)
     a0 = torch.empty_like(u0)
     ctx.make_closure_batch(n, args.seed, 0.5, 8.0, u0, a0)
@@ -83,55 +103,91 @@ def main():
     opts = P.closure_opts(maxIter=100, tol=1e-8, tolGamma=1.1, condHMax=1e10)
     torch.cuda.synchronize()
 
-    def step():
-        u.copy_(u0)
-        a.copy_(a0)
-        ctx.closure(u, a, opts, stats=stats)
+    train = None
+    if args.path.endswith("+nn"):
+        log(f"training the warm start on {args.train_pairs} solver pairs")
+        t0 = time.perf_counter()
+        m = args.train_pairs
+        ut = torch.empty((m, nm), dtype=torch.float64, device=dev)
+        at = torch.empty_like(ut)
+        ctx.set_precision(P.PREC_FP64)
+        ctx.make_closure_batch(m, args.seed + 1, 0.5, 8.0, ut, at)
+        ctx.closure(ut, at, opts)
+        ctx.sync()
+        t1 = time.perf_counter()
+        l0, l1 = ctx.mlp_train(ut, at, mse_steps=args.train_steps, dual_steps=args.dual_steps,
+                               batch=args.train_batch, lr=args.train_lr, seed=args.seed)
+        t2 = time.perf_counter()
+        train = {"pairs": m, "pair_seed": hex(args.seed + 1), "mse_steps": args.train_steps,
+                 "dual_steps": args.dual_steps, "batch": args.train_batch, "lr": args.train_lr,
+                 "mse_loss_first": round(l0, 5), "dual_gap_last": round(l1, 5), "solve_s": round(t1 - t0, 2),
+                 "train_s": round(t2 - t1, 2)}
+        log(f"trained: {train}")
+        del ut, at
+        torch.cuda.empty_cache()
 
-    for _ in range(args.warmup):
-        step()
-    rc = ctx.sync()
-    ctx.set_profiling(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sweep_ms = step_ms = tail_ms = 0.0
-    cell_sweeps = tail_sweeps = 0
-    passes = tail_launches = 0
-    for _ in range(args.steps):
-        step()
-        prof = ctx.closure_profile()  # waits for the step's stream
-        sweep_ms += prof["sweep_ms"]
-        step_ms += prof["step_ms"]
-        tail_ms += prof["tail_ms"]
-        cell_sweeps += prof["cell_sweeps"]
-        tail_sweeps += prof["tail_cell_sweeps"]
-        passes += prof["passes"]
-        tail_launches += prof["tail_launches"]
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    rc = ctx.sync()
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = n / (ms_per_step / 1e3)
+    def measure(path, steps, warmup):
+        ctx.set_precision(P.PREC_MIXED if path.startswith("mixed") else P.PREC_FP64)
+        ctx.set_warmstart(path.endswith("+nn"))
 
-    st = P.stats_to_numpy(stats)
-    it = st["totalIter"]
-    hist = np.bincount(st["regIndex"], minlength=5).tolist()
+        def step():
+            u.copy_(u0)
+            a.copy_(a0)
+            ctx.closure(u, a, opts, stats=stats)
 
-    # roofline of the sweep kernel: H = M^T diag(w e) M, 2 nq nm^2 flops per cell-sweep
-    nq = ctx.nq
+        for _ in range(warmup):
+            step()
+        ctx.sync()
+        ctx.set_profiling(True)
+        acc = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+            prof = ctx.closure_profile()  # waits for the step's stream
+            for k, v in prof.items():
+                acc[k] = acc.get(k, 0) + v
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        rc = ctx.sync()
+        ctx.set_profiling(False)
+        st = P.stats_to_numpy(stats)
+        out = {"ms_per_step": 1e3 * elapsed / steps, "status": rc,
+               "prof": {k: v / steps for k, v in acc.items()},
+               "iters_mean": float(st["totalIter"].mean()), "iters_max": int(st["totalIter"].max()),
+               "reg_hist": np.bincount(st["regIndex"], minlength=5).tolist(),
+               "fallback": float(((st["flags"] & P.FLAG_FP64_FALLBACK) != 0).mean()),
+               "warm_taken": float(((st["flags"] & P.FLAG_WARM) != 0).mean()),
+               "u": u.cpu().numpy(), "a": a.cpu().numpy(), "reg": st["regIndex"].copy()}
+        log(f"{path}: {out['ms_per_step']:.1f} ms/step iters {out['iters_mean']:.2f}")
+        return out
+
+    main_r = measure(args.path, args.steps, args.warmup)
+    base_r = main_r if args.path == "fp64" else measure("fp64", args.steps, args.warmup)
+    value = n / (main_r["ms_per_step"] / 1e3)
+    fp64_value = n / (base_r["ms_per_step"] / 1e3)
+
+    # agreement of the chosen path with the FP64 path on every cell
+    du = np.abs(main_r["u"] - base_r["u"]).max(axis=1) / np.abs(base_r["u"][:, 0])
+    agree = {"moments_within_1e-6": round(float((du <= 1e-6).mean()), 6),
+             "rung_agreement": round(float((main_r["reg"] == base_r["reg"]).mean()), 6)}
+
+    # roofline of the FP64 sweep kernel: H = M^T diag(w e) M, 2 nq nm^2 flops per cell-sweep
+    bp = base_r["prof"]
     flops_per_sweep = 2.0 * nq * nm * nm
-    achieved = flops_per_sweep * cell_sweeps / (sweep_ms * 1e-3) / 1e12
+    achieved = flops_per_sweep * bp["cell_sweeps"] / (bp["sweep_ms"] * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK, 4),
-                "traffic": None, "kernel": "k_sweep<16> (FP64 MFMA Hessian sweep)",
+                "traffic": None, "kernel": "k_sweep<16> (FP64 MFMA Hessian sweep, fp64 path)",
                 "flops_per_cell_sweep": flops_per_sweep,
-                "cell_sweeps_per_step": cell_sweeps // args.steps,
-                "sweep_kernel_ms_per_step": round(sweep_ms / args.steps, 3),
-                "step_kernel_ms_per_step": round(step_ms / args.steps, 3),
-                "tail_kernel_ms_per_step": round(tail_ms / args.steps, 3),
-                "tail_cell_sweeps_per_step": tail_sweeps // args.steps,
-                "tail_launches_per_step": tail_launches // args.steps,
-                "passes_per_step": passes // args.steps}
+                "cell_sweeps_per_step": int(bp["cell_sweeps"]),
+                "sweep_kernel_ms_per_step": round(bp["sweep_ms"], 3)}
+
+    def kernel_ms(r):
+        p = r["prof"]
+        out = {k: round(p[k], 3) for k in ("sweep_ms", "step_ms", "tail_ms", "warm_ms")}
+        out.update(passes=int(p["passes"]), tail_launches=int(p["tail_launches"]))
+        return out
 
     cpu = None
     if args.cpu_sample > 0:
@@ -143,22 +199,34 @@ def main():
         cpu = {"value": round(cps, 1), "unit": "cell-updates/s", "cores": threads,
                "kind": "port",
                "sample": f"first {ns} cells of the same batch, oracle (C, -O3, OpenMP "
-                         f"dynamic over cells), {secs:.1f} s"}
+                         f"dynamic over cells, plain FP64), {secs:.1f} s"}
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(main_r["ms_per_step"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "closure-only batch: 1048576 independent M_3 dual problems, "
-                               "FP64, 24x48=1152 ordinates, alpha*~N(0,s^2), s~U[0.5,8], "
-                               "tol 1e-8",
+                               "24x48=1152 ordinates, alpha*~N(0,s^2), s~U[0.5,8], "
+                               "||grad h||<=1e-8 in FP64",
                    "cells": n, "moment_order": N, "quad_order": Q, "ordinates": nq,
-                   "path": "plain FP64 (matrix-core Hessian sums, worklist passes)",
-                   "status": rc,
-                   "iters_per_cell_mean": round(float(it.mean()), 3),
-                   "iters_per_cell_max": int(it.max()),
-                   "regularisation_hist": hist},
+                   "path": args.path, "status": main_r["status"],
+                   "iters_per_cell_mean": round(main_r["iters_mean"], 3),
+                   "iters_per_cell_max": main_r["iters_max"],
+                   "regularisation_hist": main_r["reg_hist"],
+                   "fp64_fallback_frac": round(main_r["fallback"], 5),
+                   "warm_start_taken_frac": round(main_r["warm_taken"], 5),
+                   "kernel_ms_per_step": kernel_ms(main_r),
+                   "mixed_cond_max": args.mixed_cond,
+                   "fp64_path": {"value": round(fp64_value, 1),
+                                 "ms_per_step": round(base_r["ms_per_step"], 3),
+                                 "iters_per_cell_mean": round(base_r["iters_mean"], 3),
+                                 "regularisation_hist": base_r["reg_hist"],
+                                 "kernel_ms_per_step": kernel_ms(base_r)},
+                   "speedup_vs_fp64": round(value / fp64_value, 3),
+                   "agreement_with_fp64": agree,
+                   "warm_start_training": train},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -13,8 +13,8 @@ LIB_PATH = os.path.join(HERE, "libc3d_hip.so")
 
 (C3D_OK, C3D_ERR_ARG, C3D_ERR_HIP, C3D_ERR_U0, C3D_ERR_LADDER, C3D_ERR_UNSUPPORTED,
  C3D_ERR_BUDGET) = range(7)
-FLAG_ISO_RESTART, FLAG_U0_NONPOS, FLAG_LADDER, FLAG_BUDGET = 1, 2, 4, 8
-PREC_FP64, PREC_FP64_REFORDER = 0, 1
+FLAG_ISO_RESTART, FLAG_U0_NONPOS, FLAG_LADDER, FLAG_BUDGET, FLAG_FP64_FALLBACK = 1, 2, 4, 8, 16
+PREC_FP64, PREC_FP64_REFORDER, PREC_MIXED = 0, 1, 2
 
 _vp = C.c_void_p
 _i32 = C.c_int32
@@ -24,7 +24,28 @@ _d = C.c_double
 
 class ClosureProfile(C.Structure):
     _fields_ = [("cell_sweeps", _i64), ("tail_cell_sweeps", _i64), ("passes", _i32),
-                ("tail_launches", _i32), ("sweep_ms", _d), ("step_ms", _d), ("tail_ms", _d)]
+                ("tail_launches", _i32), ("sweep_ms", _d), ("step_ms", _d), ("tail_ms", _d),
+                ("warm_ms", _d)]
+
+
+MLP_IN, MLP_HID, MLP_NPARAM = 25, 128, 23065
+FLAG_WARM = 32
+
+
+class MlpNorm(C.Structure):
+    _fields_ = [("x_mean", C.c_float * MLP_IN), ("x_std", C.c_float * MLP_IN),
+                ("alpha_mean", C.c_float * MLP_IN), ("alpha_std", C.c_float * MLP_IN)]
+
+    def to_numpy(self):
+        import numpy as np
+        return {k: np.array(getattr(self, k)[:], dtype=np.float32) for k, _ in self._fields_}
+
+    @classmethod
+    def from_numpy(cls, d):
+        n = cls()
+        for k, _ in cls._fields_:
+            getattr(n, k)[:] = [float(x) for x in d[k]]
+        return n
 
 
 class ClosureOpts(C.Structure):
@@ -55,6 +76,7 @@ SIGNATURES = [
     ("c3d_get_closure_profile", C.c_int, [_vp, C.POINTER(ClosureProfile)]),
     ("c3d_set_tail", C.c_int, [_vp, _i32, _i32]),
     ("c3d_set_precision", C.c_int, [_vp, C.c_int]),
+    ("c3d_set_mixed", C.c_int, [_vp, _d, C.c_int]),
     ("c3d_set_pass_budget", C.c_int, [_vp, C.c_int]),
     ("c3d_hessian", C.c_int, [_vp, _i64, _vp, _vp, C.c_int, C.c_int, _vp]),
     ("c3d_dense_solve", C.c_int, [C.c_int, _i64] + [_vp] * 6 + [C.c_int, _vp]),
@@ -68,6 +90,17 @@ SIGNATURES = [
     ("c3d_update", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _d, _d, _d,
                              _d, _d, C.c_int, C.c_int, C.POINTER(ClosureOpts), _vp, _vp]),
     ("c3d_make_closure_batch", C.c_int, [_vp, _i64, C.c_uint64, _d, _d, _vp, _vp, _vp, _vp]),
+    ("c3d_mlp_train", C.c_int, [_vp, _i64, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float,
+                                C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_float), _vp]),
+    ("c3d_mlp_dual_loss_grad", C.c_int, [_vp, _i64, _vp, _vp, _vp, C.POINTER(MlpNorm), _vp, _vp,
+                                         _vp]),
+    ("c3d_mlp_get", C.c_int, [_vp, _vp, C.POINTER(MlpNorm)]),
+    ("c3d_mlp_set", C.c_int, [_vp, _vp, C.POINTER(MlpNorm)]),
+    ("c3d_set_warmstart", C.c_int, [_vp, C.c_int]),
+    ("c3d_mlp_features", C.c_int, [_vp, _i64, _vp, _vp, C.POINTER(MlpNorm), _vp, _vp, _vp]),
+    ("c3d_mlp_forward", C.c_int, [_vp, _i64, _vp, _vp, _vp, _vp]),
+    ("c3d_mlp_infer", C.c_int, [_vp, _i64, _vp, _vp, _vp]),
+    ("c3d_mlp_loss_grad", C.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
 ]
 
 _lib = None
@@ -171,6 +204,9 @@ class Ctx:
     def set_precision(self, mode):
         return check("c3d_set_precision", self.L.c3d_set_precision(self.h, mode))
 
+    def set_mixed(self, cond_max=1e5, max_iter=40):
+        return check("c3d_set_mixed", self.L.c3d_set_mixed(self.h, cond_max, max_iter))
+
     def set_pass_budget(self, n):
         return check("c3d_set_pass_budget", self.L.c3d_set_pass_budget(self.h, n))
 
@@ -218,6 +254,65 @@ class Ctx:
     def make_closure_batch(self, n, seed, smin, smax, u, alpha0, alphaStar=None, stream=None):
         return check("c3d_make_closure_batch", self.L.c3d_make_closure_batch(
             self.h, n, seed, smin, smax, _p(u), _p(alpha0), _p(alphaStar), _stream(stream)))
+
+    # ---- learned warm start
+    def mlp_train(self, u, alpha, mse_steps, dual_steps=0, batch=4096, lr=2e-3, seed=1,
+                  stream=None):
+        """Train the context's network on (u, alpha) device pairs; returns (first, last) loss."""
+        n = u.numel() // self.nm
+        l0, l1 = C.c_float(), C.c_float()
+        check("c3d_mlp_train", self.L.c3d_mlp_train(self.h, n, _p(u), _p(alpha), mse_steps,
+                                                    dual_steps, batch, lr, seed, C.byref(l0),
+                                                    C.byref(l1), _stream(stream)))
+        return l0.value, l1.value
+
+    def mlp_dual_loss_grad(self, params, X, u, norm, grad, loss, stream=None):
+        n = X.numel() // MLP_IN
+        if isinstance(norm, dict):
+            norm = MlpNorm.from_numpy(norm)
+        return check("c3d_mlp_dual_loss_grad", self.L.c3d_mlp_dual_loss_grad(
+            self.h, n, _p(params), _p(X), _p(u), C.byref(norm), _p(grad), _p(loss),
+            _stream(stream)))
+
+    def mlp_get(self):
+        import numpy as np
+        p = np.zeros(MLP_NPARAM, dtype=np.float32)
+        nrm = MlpNorm()
+        check("c3d_mlp_get", self.L.c3d_mlp_get(self.h, p.ctypes.data, C.byref(nrm)))
+        return p, nrm
+
+    def mlp_set(self, params, norm):
+        import numpy as np
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == MLP_NPARAM
+        if isinstance(norm, dict):
+            norm = MlpNorm.from_numpy(norm)
+        return check("c3d_mlp_set", self.L.c3d_mlp_set(self.h, p.ctypes.data, C.byref(norm)))
+
+    def set_warmstart(self, on):
+        return check("c3d_set_warmstart", self.L.c3d_set_warmstart(self.h, int(on)))
+
+    def mlp_features(self, u, norm, X, alpha=None, T=None, stream=None):
+        n = u.numel() // self.nm
+        if isinstance(norm, dict):
+            norm = MlpNorm.from_numpy(norm)
+        return check("c3d_mlp_features", self.L.c3d_mlp_features(
+            self.h, n, _p(u), _p(alpha), C.byref(norm), _p(X), _p(T), _stream(stream)))
+
+    def mlp_forward(self, params, X, Y, stream=None):
+        n = X.numel() // MLP_IN
+        return check("c3d_mlp_forward", self.L.c3d_mlp_forward(self.h, n, _p(params), _p(X),
+                                                               _p(Y), _stream(stream)))
+
+    def mlp_infer(self, X, Yd, stream=None):
+        n = X.numel() // MLP_IN
+        return check("c3d_mlp_infer", self.L.c3d_mlp_infer(self.h, n, _p(X), _p(Yd),
+                                                           _stream(stream)))
+
+    def mlp_loss_grad(self, params, X, T, grad, loss, stream=None):
+        n = X.numel() // MLP_IN
+        return check("c3d_mlp_loss_grad", self.L.c3d_mlp_loss_grad(
+            self.h, n, _p(params), _p(X), _p(T), _p(grad), _p(loss), _stream(stream)))
 
 
 STATS_FIELDS = [("r", "f8"), ("err", "f8"), ("gamma", "f8"), ("totalIter", "i4"),

@@ -40,6 +40,17 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// The warm-start network of a context (warmstart.hip)
+struct Mlp {
+    bool have = false;
+    float normHost[4 * C3D_MLP_IN] = {};  // x mean, x std, alpha mean, alpha std
+    DevBuf params;                        // FP32 master parameters
+    DevBuf opt;                           // gradient, Adam moments
+    DevBuf data;                          // training features/targets and minibatch
+    DevBuf work;                          // activations
+    DevBuf w16, bias, normDev;            // inference images
+};
+
 }  // namespace c3d
 
 struct c3d_ctx_s {
@@ -60,8 +71,13 @@ struct c3d_ctx_s {
     // scratch owned by the context, grown on demand
     c3d::DevBuf ansatz, uOld, flux;
     c3d::DevBuf closureState;  // per-cell Newton state, Hessians, worklists
+    c3d::Mlp mlp;
+    bool warm = false;         // closure starts from the network's proposal where it is better
+    c3d::DevBuf warmBuf;       // features and proposals of the closure's cells
     int maxPasses = 100000;     // sweep passes per closure call before cells are reported
     int precision = C3D_PREC_FP64;
+    double mixedCondMax = 1e5;  // C3D_PREC_MIXED hand-over to the FP64 path
+    int mixedMaxIter = 40;
     // profiling of the last closure call
     bool profiling = false;
     std::vector<hipEvent_t> evPool;
@@ -89,6 +105,24 @@ hipError_t launch_dense_solve(int nm, int64_t n, const double *H, const double *
                               double *rcond, int *info, double *L, int lapackOrder, hipStream_t s);
 hipError_t launch_hessian_fp64(c3d_ctx_s *ctx, int64_t ncells, const double *alpha,
                                double *hs, hipStream_t s);
+
+// warmstart.hip
+hipError_t mlp_pack(c3d_ctx_s *ctx, hipStream_t s);
+hipError_t mlp_infer(c3d_ctx_s *ctx, int64_t n, const float *X, double *out, hipStream_t s);
+hipError_t mlp_features(c3d_ctx_s *ctx, int64_t n, const double *u, const double *alpha,
+                        const float *normDev, float *X, float *T, hipStream_t s);
+hipError_t mlp_forward(c3d_ctx_s *ctx, int64_t n, const float *p, const float *X, float *Y,
+                       hipStream_t s);
+hipError_t mlp_loss_grad(c3d_ctx_s *ctx, int64_t n, const float *p, const float *X,
+                         const float *T, float *grad, float *loss, hipStream_t s);
+int mlp_train(c3d_ctx_s *ctx, int64_t n, const double *u, const double *alpha, int mseSteps,
+              int dualSteps, int batch, float lr, uint64_t seed, float *lossFirst,
+              float *lossLast, hipStream_t s);
+hipError_t mlp_dual_loss_grad(c3d_ctx_s *ctx, int64_t n, const float *p, const float *X,
+                              const double *u, const float *normDev, float *grad, float *loss,
+                              hipStream_t s);
+int mlp_set(c3d_ctx_s *ctx, const float *params, const float *norm);
+int mlp_get(c3d_ctx_s *ctx, float *params, float *norm);
 
 // grid.hip
 hipError_t launch_ghost_fill(int nx, int ny, int nz, int nm, double *u, hipStream_t s);

@@ -291,7 +291,7 @@ int c3d_get_closure_profile(c3d_ctx ctx, c3d_closure_profile *out)
         } else {
             if (hipEventElapsedTime(&a, ctx->evPool[k], ctx->evPool[k + 1]) != hipSuccess)
                 return C3D_ERR_HIP;
-            out->tail_ms += a;
+            (sp.first == 1 ? out->tail_ms : out->warm_ms) += a;
         }
     }
     return C3D_OK;
@@ -308,10 +308,130 @@ int c3d_set_tail(c3d_ctx ctx, int max_cells, int passes_per_launch)
 
 int c3d_set_precision(c3d_ctx ctx, int mode)
 {
-    if (!ctx || (mode != C3D_PREC_FP64 && mode != C3D_PREC_FP64_REFORDER))
+    if (!ctx || (mode != C3D_PREC_FP64 && mode != C3D_PREC_FP64_REFORDER &&
+                 mode != C3D_PREC_MIXED))
         return C3D_ERR_ARG;
     ctx->precision = mode;
     return C3D_OK;
+}
+
+int c3d_set_mixed(c3d_ctx ctx, double cond_max, int max_iter)
+{
+    if (!ctx || !(cond_max >= 1.0) || max_iter < 1)
+        return C3D_ERR_ARG;
+    ctx->mixedCondMax = cond_max;
+    ctx->mixedMaxIter = max_iter;
+    return C3D_OK;
+}
+
+int c3d_mlp_train(c3d_ctx ctx, int64_t n, const double *u, const double *alpha, int mse_steps,
+                  int dual_steps, int batch, float lr, uint64_t seed, float *loss_first,
+                  float *loss_last, void *stream)
+{
+    if (!ctx || n < 1 || n > INT32_MAX || !u || !alpha || mse_steps < 0 || dual_steps < 0 ||
+        mse_steps + dual_steps < 1 || batch < 1 || batch > (1 << 22) || !(lr > 0.0f))
+        return C3D_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    return mlp_train(ctx, n, u, alpha, mse_steps, dual_steps, batch, lr, seed, loss_first,
+                     loss_last, (hipStream_t)stream);
+}
+
+int c3d_mlp_get(c3d_ctx ctx, float *params, c3d_mlp_norm *norm)
+{
+    if (!ctx)
+        return C3D_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    return mlp_get(ctx, params, norm ? &norm->x_mean[0] : nullptr);
+}
+
+int c3d_mlp_set(c3d_ctx ctx, const float *params, const c3d_mlp_norm *norm)
+{
+    if (!ctx || !params || !norm)
+        return C3D_ERR_ARG;
+    for (int k = 0; k < C3D_MLP_IN; ++k)
+        if (!(norm->x_std[k] > 0.0f) || !(norm->alpha_std[k] > 0.0f))
+            return C3D_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    return mlp_set(ctx, params, &norm->x_mean[0]);
+}
+
+int c3d_set_warmstart(c3d_ctx ctx, int enable)
+{
+    if (!ctx)
+        return C3D_ERR_ARG;
+    if (enable && !ctx->mlp.have)
+        return C3D_ERR_ARG;
+    ctx->warm = enable != 0;
+    return C3D_OK;
+}
+
+int c3d_mlp_features(c3d_ctx ctx, int64_t n, const double *u, const double *alpha,
+                     const c3d_mlp_norm *norm, float *X, float *T, void *stream)
+{
+    if (!ctx || n < 0 || !u || !norm || !X || (T && !alpha))
+        return C3D_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    hipStream_t s = (hipStream_t)stream;
+    c3d::DevBuf tmp;  // the caller's normalisation (the network's own copy stays intact)
+    hipError_t e = grow(tmp, sizeof(c3d_mlp_norm));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(tmp.p, norm, sizeof(c3d_mlp_norm), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = mlp_features(ctx, n, u, T ? alpha : nullptr, (const float *)tmp.p, X, T, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (tmp.p)
+        (void)hipFree(tmp.p);
+    return hip_status(e);
+}
+
+int c3d_mlp_forward(c3d_ctx ctx, int64_t n, const float *params, const float *X, float *Y,
+                    void *stream)
+{
+    if (!ctx || n < 0 || !params || !X || !Y)
+        return C3D_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    return hip_status(mlp_forward(ctx, n, params, X, Y, (hipStream_t)stream));
+}
+
+int c3d_mlp_infer(c3d_ctx ctx, int64_t n, const float *X, double *Yd, void *stream)
+{
+    if (!ctx || n < 0 || !X || !Yd)
+        return C3D_ERR_ARG;
+    if (!ctx->mlp.have)
+        return C3D_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    return hip_status(mlp_infer(ctx, n, X, Yd, (hipStream_t)stream));
+}
+
+int c3d_mlp_loss_grad(c3d_ctx ctx, int64_t n, const float *params, const float *X,
+                      const float *T, float *grad, float *loss, void *stream)
+{
+    if (!ctx || n < 1 || !params || !X || !T || !grad || !loss)
+        return C3D_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    return hip_status(mlp_loss_grad(ctx, n, params, X, T, grad, loss, (hipStream_t)stream));
+}
+
+int c3d_mlp_dual_loss_grad(c3d_ctx ctx, int64_t n, const float *params, const float *X,
+                           const double *u, const c3d_mlp_norm *norm, float *grad, float *loss,
+                           void *stream)
+{
+    if (!ctx || n < 1 || n > INT32_MAX || !params || !X || !u || !norm || !grad || !loss)
+        return C3D_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    hipStream_t s = (hipStream_t)stream;
+    c3d::DevBuf tmp;
+    hipError_t e = grow(tmp, sizeof(c3d_mlp_norm));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(tmp.p, norm, sizeof(c3d_mlp_norm), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = mlp_dual_loss_grad(ctx, n, params, X, u, (const float *)tmp.p, grad, loss, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (tmp.p)
+        (void)hipFree(tmp.p);
+    return hip_status(e);
 }
 
 int c3d_set_pass_budget(c3d_ctx ctx, int max_passes)

@@ -47,8 +47,9 @@ __constant__ double kReg[kNumReg] = {0.0, 1e-3, 1e-2, 1e-1, 5e-1};  // opt.h:37-
 enum : int { kPhaseH = 0, kPhaseLS = 1 };
 
 // per-cell state, [ncells] of each; vectors are [ncells][kVec][nm]
-enum : int { kVa = 0, kVa0, kVaOld, kVd, kVg, kVu, kVuOrig, kVec };
-enum : int { kSu0 = 0, kSf, kStls, kStprev, kSmaxF, kSgd, kSerr, kSgamma, kSr, kSrtol, kScal };
+enum : int { kVa = 0, kVa0, kVaOld, kVd, kVg, kVu, kVuOrig, kVaIn, kVec };
+enum : int { kSu0 = 0, kSf, kStls, kStprev, kSmaxF, kSgd, kSerr, kSgamma, kSr, kSrtol, kSshift,
+             kScal };
 enum : int { kIiter = 0, kItot, kIgam, kIreg, kIflags, kIphase, kImaxIt, kInt };
 
 struct State {
@@ -56,7 +57,8 @@ struct State {
     double *scal;  // [ncells][kScal]
     int *ints;     // [ncells][kInt]
     double *H;     // [ncells][nm][nm]
-    double *fdir;  // [ncells]
+    double *fdir;  // [ncells]  sum_q w_q e_q
+    double *gs;    // [ncells][nm]  sum_q w_q e_q m_q (mixed mode only)
 };
 
 struct KParams {
@@ -64,6 +66,11 @@ struct KParams {
     int nq, nqPad, maxIter, unbatched;
     double tol, tolGamma, condHMax;
     double m0, isoAlpha0, minf;
+    // mixed mode: a cell leaves the FP32-Hessian iteration for the FP64 path (restarting
+    // from its input) on condest > mixedCondMax, a failed factorisation or line search, or
+    // more than mixedMaxIter iterations
+    int mixed, mixedMaxIter;
+    double mixedCondMax;
 };
 
 __device__ __forceinline__ double rdl(double v, int src)
@@ -136,11 +143,14 @@ struct LaneVec {
 };
 
 // Wave-resident factor: lane i < NM holds row i (L after Cholesky; L(i,i) in row[i]) and
-// col[j] = L(j,i) for j > i, for the transposed solves.
-template <int NM>
+// col[j] = L(j,i) for j > i, for the transposed solves.  EXACT divides by the diagonal as
+// LAPACK/BLAS do; otherwise the triangular solves multiply by reciprocals of the diagonal
+// computed once per factor (dinv, lane i: 1/L(i,i)), which shortens their critical path.
+template <int NM, bool EXACT = true>
 struct Dense {
     double row[NM];
     double col[NM];
+    double dinv;
 
     __device__ __forceinline__ double diag(int j) const { return rdl(row[j], j); }
 
@@ -174,6 +184,13 @@ struct Dense {
 
     __device__ __forceinline__ void build_columns(double *sT, int ld, int lane)
     {
+        if constexpr (!EXACT) {
+            double djj = 1.0;
+#pragma unroll
+            for (int j = 0; j < NM; ++j)
+                djj = (lane == j) ? row[j] : djj;
+            dinv = 1.0 / djj;
+        }
         if (lane < NM) {
 #pragma unroll
             for (int j = 0; j < NM; ++j)
@@ -199,7 +216,10 @@ struct Dense {
         for (int j = 0; j < NM; ++j) {
             double xj = rdl(x, j);
             if (xj != 0.0) {
-                xj = xj / diag(j);
+                if constexpr (EXACT)
+                    xj = xj / diag(j);
+                else
+                    xj = xj * rdl(dinv, j);
                 if (lane == j)
                     x = xj;
                 else if (lane > j)
@@ -212,7 +232,7 @@ struct Dense {
 #pragma unroll
         for (int j = NM - 1; j >= 0; --j) {
             if (lane == j)
-                x = x / row[j];
+                x = EXACT ? x / row[j] : x * dinv;
             double xj = rdl(x, j);
             if (lane < j)
                 x = x - col[j] * xj;
@@ -223,7 +243,7 @@ struct Dense {
 #pragma unroll
         for (int j = 0; j < NM; ++j) {
             if (lane == j)
-                d = d / row[j];
+                d = EXACT ? d / row[j] : d * dinv;
             double dj = rdl(d, j);
             if (lane > j)
                 d = d - row[j] * dj;
@@ -231,7 +251,7 @@ struct Dense {
 #pragma unroll
         for (int j = NM - 1; j >= 0; --j) {
             if (lane == j)
-                d = d / row[j];
+                d = EXACT ? d / row[j] : d * dinv;
             double dj = rdl(d, j);
             if (lane < j)
                 d = d - col[j] * dj;
@@ -244,7 +264,7 @@ struct Dense {
         for (int k = 0; k < NM; ++k) {
             double dk = rdl(d, k);
             if (dk != 0.0) {
-                dk = dk / diag(k);
+                dk = EXACT ? dk / diag(k) : dk * rdl(dinv, k);
                 if (lane == k)
                     d = dk;
                 else if (lane > k)
@@ -257,7 +277,7 @@ struct Dense {
 #pragma unroll
             for (int k = i + 1; k < NM; ++k)
                 temp = temp - rdl(col[k], i) * rdl(d, k);
-            temp = temp / diag(i);
+            temp = EXACT ? temp / diag(i) : temp * rdl(dinv, i);
             if (lane == i)
                 d = temp;
         }
@@ -371,6 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_init(KParams P, const double *__rest
         v[kVg * NM + lane] = 0.0;
         v[kVu * NM + lane] = uu;
         v[kVuOrig * NM + lane] = uu;
+        v[kVaIn * NM + lane] = a;
     }
     const double rtol = P.unbatched ? LaneVec<NM>::dnrm2(own ? uu : 0.0) * P.tol : P.tol;
     if (lane == 0) {
@@ -385,6 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_init(KParams P, const double *__rest
         s[kSgamma] = 0.0;
         s[kSr] = 0.0;
         s[kSrtol] = rtol;
+        s[kSshift] = 0.0;
         int *in = S.ints + c * kInt;
         in[kIiter] = 0;
         in[kItot] = 0;
@@ -637,14 +659,193 @@ __global__ __launch_bounds__(kBlock) void k_sweep_ref(KParams P, const double *_
     }
 }
 
+// ---------------------------------------------------------------- mixed-precision sweep
+// One cell per wave.  FP64 on the vector units: the exponent alpha.m_q (FMA chain), the
+// shifted weight w_q e^{alpha.m_q - shift}, sum_q w e and sum_q w e m_q (so f and the
+// gradient, hence the stopping test, keep FP64 accuracy).  The Hessian on the FP32 matrix
+// cores (V_MFMA_F32_16X16X4_F32, 4 ordinates per instruction) from the shifted weights;
+// the shift is log sum_q w e at the cell's last Newton iterate (a log-sum-exp shift), so the
+// FP32 operands stay near unit scale.  FP32 accumulator register r of lane l holds
+// D[4(l/16) + r][l%16].
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <int NM>
+__device__ __forceinline__ void chunk_mixed(const double *sMc, double wl, const double *sAc,
+                                            double shift,
+                                            f4 (&acc)[(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
+                                            double &fpart, double (&gpart)[NM], int lane)
+{
+    constexpr int NT = (NM + 15) / 16;
+    constexpr int NMS = NM | 1;
+    const int gq = lane >> 4, rq = lane & 15;
+    const double *rowp = &sMc[lane * NMS];
+    double x = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        x = fma(sAc[i], rowp[i], x);
+    const double we = wl * exp(x - shift);
+    fpart = fpart + we;
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        gpart[i] = fma(we, rowp[i], gpart[i]);
+    const float wef = (float)we;
+#pragma unroll
+    for (int k = 0; k < kChunk / 4; ++k) {
+        const int ql = 4 * k + gq;
+        const double *mr = &sMc[ql * NMS];
+        const float b0 = (rq < NM) ? (float)mr[rq] : 0.0f;
+        const float b1 = (NT == 2 && 16 + rq < NM) ? (float)mr[16 + rq] : 0.0f;
+        const float wq = __shfl(wef, ql);
+        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b0, b0, acc[0], 0, 0, 0);
+        if constexpr (NT == 2) {
+            acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b1, b0, acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b1, b1, acc[2], 0, 0, 0);
+        }
+    }
+}
+
+// f alone with chunk_mixed's operations (for line-search trials in the tail kernel)
+template <int NM>
+__device__ __forceinline__ double f_only_mixed(const KParams &P, const double *__restrict__ Mg,
+                                               const double *__restrict__ wg, const double *sA,
+                                               double shift, int lane)
+{
+    double fpart = 0.0;
+    for (int ch = 0; ch < P.nqPad / kChunk; ++ch) {
+        const size_t q = (size_t)ch * kChunk + lane;
+        double x = 0.0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i)
+            x = fma(sA[i], Mg[q * NM + i], x);
+        fpart = fpart + wg[q] * exp(x - shift);
+    }
+    return wave_sum(fpart) * exp(shift);
+}
+
+// Results of a mixed sweep: fdir and the gradient sums (lane i < NM gets component i) are
+// returned; H = e^shift * H32 is written row-major to H.
+template <int NM>
+__device__ __forceinline__ double finish_mixed(const f4 *acc, double fpart, const double *gpart,
+                                               double shift, double *H, double &gsLane, int lane)
+{
+    constexpr int NT = (NM + 15) / 16;
+    constexpr int NB = NT * (NT + 1) / 2;
+    const double es = exp(shift);
+    const double fdir = wave_sum(fpart) * es;
+    gsLane = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        const double t = wave_sum(gpart[i]) * es;
+        if (lane == i)
+            gsLane = t;
+    }
+    const int gq = lane >> 4, rq = lane & 15;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int R = (b == 0) ? 0 : 1, Cb = (b == 2) ? 1 : 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = 16 * R + 4 * gq + k, j = 16 * Cb + rq;
+            if (i < NM && j < NM && j <= i) {
+                const double h = (double)acc[b][k] * es;
+                H[i * NM + j] = h;
+                H[j * NM + i] = h;
+            }
+        }
+    }
+    return fdir;
+}
+
+// Sweep pass of the mixed mode: one cell per wave, 4 cells per block sharing each LDS
+// chunk.  Cells the mixed iteration handed to the FP64 path are swept exactly as k_sweep
+// sweeps them (same bits).
+template <int NM>
+__global__ __launch_bounds__(kBlock) void k_sweep_mixed(KParams P, const double *__restrict__ Mg,
+                                                         const double *__restrict__ wg, State S,
+                                                         const int *__restrict__ W,
+                                                         const int *__restrict__ nWp)
+{
+    constexpr int NT = (NM + 15) / 16;
+    constexpr int NB = NT * (NT + 1) / 2;
+    constexpr int NMS = NM | 1;
+    __shared__ double sM[kChunk * NMS];
+    __shared__ double sW[kChunk];
+    __shared__ double sA[kWavesPerBlock][NM];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nW = uniform_int(*nWp);
+    const int nBlocksNeeded = (nW + kWavesPerBlock - 1) / kWavesPerBlock;
+    for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
+        const int slot = blk * kWavesPerBlock + wv;
+        const int cell = slot < nW ? uniform_int(W[slot]) : -1;
+        bool mixedCell = false;
+        double shift = 0.0;
+        if (cell >= 0) {
+            mixedCell = !(S.ints[(size_t)cell * kInt + kIflags] & C3D_FLAG_FP64_FALLBACK);
+            shift = S.scal[(size_t)cell * kScal + kSshift];
+            if (lane < NM)
+                sA[wv][lane] = S.vec[(size_t)cell * kVec * NM + kVa * NM + lane];
+        }
+        d4 acc64[1][NB];
+        f4 acc32[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            acc64[0][b] = d4{0.0, 0.0, 0.0, 0.0};
+            acc32[b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+        double fpart[1] = {0.0};
+        double gpart[NM];
+#pragma unroll
+        for (int i = 0; i < NM; ++i)
+            gpart[i] = 0.0;
+        for (int ch = 0; ch < P.nqPad / kChunk; ++ch) {
+            __syncthreads();
+            for (int e = tid; e < kChunk * NM; e += kBlock) {
+                const int rr = e / NM, cc = e - rr * NM;
+                sM[rr * NMS + cc] = Mg[(size_t)ch * kChunk * NM + e];
+            }
+            if (tid < kChunk)
+                sW[tid] = wg[(size_t)ch * kChunk + tid];
+            __syncthreads();
+            if (cell >= 0) {
+                if (mixedCell)
+                    chunk_mixed<NM>(sM, sW[lane], sA[wv], shift, acc32, fpart[0], gpart, lane);
+                else
+                    chunk_mfma<NM, 1>(sM, sW[lane], sA[wv], acc64, fpart, lane);
+            }
+        }
+        if (cell < 0)
+            continue;
+        double *H = S.H + (size_t)cell * NM * NM;
+        if (mixedCell) {
+            double gsl;
+            const double fd = finish_mixed<NM>(acc32, fpart[0], gpart, shift, H, gsl, lane);
+            if (lane < NM)
+                S.gs[(size_t)cell * NM + lane] = gsl;
+            if (lane == 0)
+                S.fdir[cell] = fd;
+        } else {
+            const double fs = wave_sum(fpart[0]);
+            if (lane == 0)
+                S.fdir[cell] = fs;
+            store_H<NM>(acc64[0], H, lane);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- one cell's state machine
 // Register image of a cell's Newton state; vectors are distributed (lane i < NM holds
 // component i), scalars are wave-uniform.
 template <int NM>
 struct Cell {
-    double a, a0, aOld, d, g, uu, uOrig;
-    double u0, f, tls, tprev, maxF, gd, err, gamma, r, rtol;
+    double a, a0, aOld, d, g, uu, uOrig, aIn;
+    double u0, f, tls, tprev, maxF, gd, err, gamma, r, rtol, shift;
     int iter, tot, igam, reg, flags, phase, maxIt;
+
+    // mixed mode and not yet handed to the FP64 path
+    __device__ __forceinline__ bool mixed(const KParams &P) const
+    {
+        return P.mixed && !(flags & C3D_FLAG_FP64_FALLBACK);
+    }
 
     __device__ __forceinline__ void load(const State &S, int c, int lane)
     {
@@ -657,6 +858,7 @@ struct Cell {
         g = own ? v[kVg * NM + lane] : 0.0;
         uu = own ? v[kVu * NM + lane] : 0.0;
         uOrig = own ? v[kVuOrig * NM + lane] : 0.0;
+        aIn = own ? v[kVaIn * NM + lane] : 0.0;
         const double *sc = S.scal + (size_t)c * kScal;
         u0 = sc[kSu0];
         f = sc[kSf];
@@ -668,6 +870,7 @@ struct Cell {
         gamma = sc[kSgamma];
         r = sc[kSr];
         rtol = sc[kSrtol];
+        shift = sc[kSshift];
         const int *in = S.ints + (size_t)c * kInt;
         iter = in[kIiter];
         tot = in[kItot];
@@ -700,6 +903,7 @@ struct Cell {
             sc[kSgamma] = gamma;
             sc[kSr] = r;
             sc[kSrtol] = rtol;
+            sc[kSshift] = shift;
             int *in = S.ints + (size_t)c * kInt;
             in[kIiter] = iter;
             in[kItot] = tot;
@@ -709,6 +913,34 @@ struct Cell {
             in[kIphase] = phase;
             in[kImaxIt] = maxIt;
         }
+    }
+
+    // Mixed mode hands the cell to the FP64 path: the state k_init set up, from the input.
+    __device__ __forceinline__ void restart_fp64(const KParams &P, int lane)
+    {
+        a = aIn;
+        a0 = aIn;
+        aOld = aIn;
+        d = 0.0;
+        g = 0.0;
+        uu = uOrig;
+        f = 0.0;
+        tls = 1.0;
+        tprev = 1.0;
+        maxF = 0.0;
+        gd = 0.0;
+        err = 0.0;
+        gamma = 0.0;
+        r = 0.0;
+        rtol = P.unbatched ? LaneVec<NM>::dnrm2(lane < NM ? uOrig : 0.0) * P.tol : P.tol;
+        shift = 0.0;
+        iter = 0;
+        tot = 0;
+        igam = 0;
+        reg = 0;
+        flags = C3D_FLAG_FP64_FALLBACK;
+        phase = kPhaseH;
+        maxIt = P.maxIter;
     }
 
     // postProcess (optBatch.cpp:333-344 / opt.cpp:347-353) and the per-cell statistics
@@ -732,22 +964,32 @@ struct Cell {
         }
     }
 
+    // Armijo test of the current line-search trial, f(alpha) = fdir - alpha.u
+    // (linesearch.cpp:60-66)
+    __device__ __forceinline__ bool ls_accepts(double fdir) const
+    {
+        double fNew = fdir;
+#pragma unroll
+        for (int i = 0; i < NM; ++i)
+            fNew = fNew - rdl(a, i) * rdl(uu, i);
+        return fNew <= f + 1e-3 * tls * gd;
+    }
+
     // One step of the state machine with the sweep result at the current alpha (H row-major,
-    // in LDS or global memory; fdir = sum_q w_q e_q): either the Armijo test of a line-search
+    // in LDS or global memory; fdir = sum_q w_q e_q; gs = sum_q w_q e_q m_q, read in mixed
+    // mode only): either the Armijo test of a line-search
     // trial, or one Newton iteration of solveTheRest / optScaled.  sT: NM x (NM+1) doubles of
     // LDS for the transposed factor.  Returns 0 to continue, 1 converged, 2 ladder exhausted.
+    // H is not read when a line-search trial is rejected (it may then be null).
+    template <bool EXACT>
     __device__ __forceinline__ int advance(const KParams &P, const double *H, double fdir,
-                                           double *sT, int lane)
+                                           const double *gs, double *sT, int lane)
     {
         constexpr int LD = NM + 1;
         const bool own = lane < NM;
         const double tolGamma = P.unbatched ? 1.1 : P.tolGamma;
         if (phase == kPhaseLS) {
-            double fNew = fdir;
-#pragma unroll
-            for (int i = 0; i < NM; ++i)
-                fNew = fNew - rdl(a, i) * rdl(uu, i);
-            if (fNew <= f + 1e-3 * tls * gd) {
+            if (ls_accepts(fdir)) {
                 tprev = tls;  // accepted; H of this sweep belongs to the new alpha
             } else {
                 tls = 0.5 * tls;
@@ -763,7 +1005,7 @@ struct Cell {
             }
         }
         // ================================ one Newton iteration (solveTheRest)
-        Dense<NM> D;
+        Dense<NM, EXACT> D;
 #pragma unroll
         for (int j = 0; j < NM; ++j)
             D.row[j] = own ? H[lane * NM + j] : 0.0;
@@ -772,8 +1014,14 @@ struct Cell {
             maxItNow = 10 * maxIt;
         ++iter;
         ++tot;
-        g = own ? D.row[0] / P.m0 - uu : 0.0;  // fobjMnUseH
-        f = rdl(D.row[0], 0) / (P.m0 * P.m0);
+        const bool mx = mixed(P);
+        if (mx) {  // fobjMn from the FP64 sums (fobj.cpp:135-164)
+            g = own ? gs[lane] - uu : 0.0;
+            f = fdir;
+        } else {  // fobjMnUseH
+            g = own ? D.row[0] / P.m0 - uu : 0.0;
+            f = rdl(D.row[0], 0) / (P.m0 * P.m0);
+        }
 #pragma unroll
         for (int i = 0; i < NM; ++i)
             f = f - rdl(a, i) * rdl(uu, i);
@@ -794,7 +1042,15 @@ struct Cell {
             D.build_columns(sT, LD, lane);
             condest = 1.0 / D.pocon(norm1, lane);
         }
-        if (tot == 1 && (cholErr != 0 || condest > P.condHMax)) {
+        if (mx) {
+            if (cholErr != 0 || !(condest <= P.mixedCondMax) || tprev == 0.0 ||
+                iter >= min(maxItNow, P.mixedMaxIter)) {
+                restart_fp64(P, lane);
+                return 0;
+            }
+            const double lf = log(fdir);  // shift of the sweeps from this iterate on
+            shift = (lf > -700.0 && lf < 700.0) ? lf : 0.0;
+        } else if (tot == 1 && (cholErr != 0 || condest > P.condHMax)) {
             a0 = (lane == 0) ? P.isoAlpha0 : 0.0;  // isotropic restart, optBatch.cpp:208-218
             a = a0;
             if (P.unbatched)
@@ -806,7 +1062,7 @@ struct Cell {
             phase = kPhaseH;
             return 0;
         }
-        if (iter >= maxItNow || cholErr != 0 || tprev == 0.0 || condest > P.condHMax) {
+        if (!mx && (iter >= maxItNow || cholErr != 0 || tprev == 0.0 || condest > P.condHMax)) {
             if (reg < kNumReg - 1) {  // ladder, optBatch.cpp:222-248
                 ++reg;
                 r = kReg[reg];
@@ -866,7 +1122,7 @@ struct Cell {
 };
 
 // ---------------------------------------------------------------- step pass
-template <int NM>
+template <int NM, bool EXACT>
 __global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, const int *__restrict__ W,
                                                   const int *__restrict__ nWp, int *Wn, int *nWn,
                                                   double *u_g, double *alpha_g,
@@ -887,7 +1143,8 @@ __global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, const int *
         const int c = uniform_int(W[slot]);
         Cell<NM> C;
         C.load(S, c, lane);
-        const int done = C.advance(P, S.H + (size_t)c * NM * NM, S.fdir[c], sT[wv], lane);
+        const int done = C.template advance<EXACT>(P, S.H + (size_t)c * NM * NM, S.fdir[c],
+                                   S.gs + (size_t)c * NM, sT[wv], lane);
         if (done) {
             C.finish(done, c, u_g, alpha_g, stats, status, lane);
         } else {
@@ -898,13 +1155,48 @@ __global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, const int *
     }
 }
 
+// sum_q w_q exp(alpha.m_q) alone for one cell (one wave; alpha in LDS), with exactly the
+// operations the sweeps use for it: chunk_mfma's per-lane partials and final wave_sum, or
+// chunk_ref's ordinate-order sum.  A rejected line-search trial needs nothing else.
+template <int NM, bool REF>
+__device__ __forceinline__ double f_only(const KParams &P, const double *__restrict__ Mg,
+                                         const double *__restrict__ wg, const double *sA,
+                                         double *sW, double *sE, int lane)
+{
+    const int nch = P.nqPad / kChunk;
+    double fpart = 0.0;
+    for (int ch = 0; ch < nch; ++ch) {
+        const size_t q = (size_t)ch * kChunk + lane;
+        double x = 0.0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i)
+            x = x + sA[i] * Mg[q * NM + i];
+        if constexpr (REF) {
+            wave_lds_sync();
+            sE[lane] = det_exp(x);
+            sW[lane] = wg[q];
+            wave_lds_sync();
+            const int nqc = min(kChunk, P.nq - ch * kChunk);
+            for (int k = 0; k < nqc; ++k)
+                fpart = fpart + sW[k] * sE[k];
+        } else {
+            fpart = fpart + wg[q] * exp(x);
+        }
+    }
+    if constexpr (REF)
+        return fpart;
+    else
+        return wave_sum(fpart);
+}
+
 // ---------------------------------------------------------------- fused tail pass
 // For the last few cells: one wave per cell runs up to `passes` sweep+step rounds in one
 // launch, with the cell's state in registers, its alpha, M chunks, H and the transposed
 // factor in the wave's own LDS.  The sweep performs the same operations in the same order as
-// k_sweep / k_sweep_ref do for that cell, so a cell's result does not depend on which kernel
-// finished it.  Launch: one 64-thread block per listed cell (grid >= *nWp).
-template <int NM, bool REF>
+// k_sweep / k_sweep_ref / k_sweep_mixed do for that cell, so a cell's result does not depend
+// on which kernel finished it.  A line-search trial evaluates f alone first and sweeps H only
+// when the trial is accepted.  Launch: one 64-thread block per listed cell (grid >= *nWp).
+template <int NM, bool REF, bool MIXED>
 __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restrict__ Mg,
                                                  const double *__restrict__ wg, State S,
                                                  const int *__restrict__ W,
@@ -916,7 +1208,9 @@ __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restr
     constexpr int NT = (NM + 15) / 16;
     constexpr int NB = NT * (NT + 1) / 2;
     constexpr int NMS = NM | 1;
-    static_assert(NM * NM + NM * (NM + 1) <= kChunk * NMS, "H and factor alias the M chunk");
+    static_assert(!(REF && MIXED), "one arithmetic mode");
+    static_assert(NM * NM + NM * (NM + 1) + NM <= kChunk * NMS,
+                  "H, factor and gradient alias the M chunk");
     __shared__ double sM[kChunk * NMS];
     __shared__ double sW[kChunk];
     __shared__ double sE[kChunk];
@@ -929,14 +1223,28 @@ __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restr
     const int c = uniform_int(W[slot]);
     Cell<NM> C;
     C.load(S, c, lane);
-    double *sH = sM;               // valid between a sweep and the next one
+    double *sH = sM;  // valid between a sweep and the next one
     double *sT = sM + NM * NM;
+    double *sG = sT + NM * (NM + 1);
     const int nch = P.nqPad / kChunk;
     int done = 0;
     int sweeps = 0;
     for (int p = 0; p < passes; ++p) {
+        const bool mx = MIXED && C.mixed(P);
         if (lane < NM)
             sA[lane] = C.a;
+        if (C.phase == kPhaseLS) {  // a trial first needs f only; H only if it is accepted
+            wave_lds_sync();
+            const double ft = mx ? f_only_mixed<NM>(P, Mg, wg, sA, C.shift, lane)
+                                 : f_only<NM, REF>(P, Mg, wg, sA, sW, sE, lane);
+            if (!C.ls_accepts(ft)) {
+                done = C.template advance<REF>(P, nullptr, ft, nullptr, sT, lane);
+                wave_lds_sync();
+                if (done)
+                    break;
+                continue;
+            }
+        }
         double fdir;
         if constexpr (REF) {
             const RefEntries<NM> E(lane);
@@ -960,10 +1268,17 @@ __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restr
             fdir = f;
         } else {
             d4 acc[1][NB];
+            f4 acc32[NB];
 #pragma unroll
-            for (int b = 0; b < NB; ++b)
+            for (int b = 0; b < NB; ++b) {
                 acc[0][b] = d4{0.0, 0.0, 0.0, 0.0};
+                acc32[b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            }
             double fpart[1] = {0.0};
+            double gpart[NM];
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                gpart[i] = 0.0;
             double pre[NM];
             double prew = wg[lane];
 #pragma unroll
@@ -986,15 +1301,25 @@ __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restr
                     for (int k = 0; k < NM; ++k)
                         pre[k] = Mg[base * NM + lane + kWave * k];
                 }
-                chunk_mfma<NM, 1>(sM, wl, sA, acc, fpart, lane);
+                if (mx)
+                    chunk_mixed<NM>(sM, wl, sA, C.shift, acc32, fpart[0], gpart, lane);
+                else
+                    chunk_mfma<NM, 1>(sM, wl, sA, acc, fpart, lane);
             }
-            fdir = wave_sum(fpart[0]);
             wave_lds_sync();
-            store_H<NM>(acc[0], sH, lane);
+            if (mx) {
+                double gsl;
+                fdir = finish_mixed<NM>(acc32, fpart[0], gpart, C.shift, sH, gsl, lane);
+                if (lane < NM)
+                    sG[lane] = gsl;
+            } else {
+                fdir = wave_sum(fpart[0]);
+                store_H<NM>(acc[0], sH, lane);
+            }
         }
         wave_lds_sync();
         ++sweeps;
-        done = C.advance(P, sH, fdir, sT, lane);
+        done = C.template advance<REF>(P, sH, fdir, sG, sT, lane);
         wave_lds_sync();
         if (done)
             break;
@@ -1007,6 +1332,116 @@ __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restr
         C.store(S, c, lane);
         if (lane == 0)
             Wn[atomicAdd(nWn, 1)] = c;
+    }
+}
+
+// ---------------------------------------------------------------- learned warm start
+// Features of the listed cells for the network, row = worklist slot: x_k = (u_k/u0 - xm_k)/
+// xs_k from the state's u0-scaled moments (k < NM; zero beyond and for slots >= *nWp).
+template <int NM>
+__global__ void k_warm_features(int64_t rows, State S, const int *__restrict__ W,
+                                const int *__restrict__ nWp, const float *__restrict__ norm,
+                                float *X)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * C3D_MLP_IN)
+        return;
+    const int64_t slot = e / C3D_MLP_IN;
+    const int k = (int)(e % C3D_MLP_IN);
+    float x = 0.0f;
+    if (slot < *nWp && k < NM) {
+        const double uk = S.vec[(size_t)W[slot] * kVec * NM + kVu * NM + k];
+        x = ((float)uk - norm[k]) / norm[C3D_MLP_IN + k];
+    }
+    X[e] = x;
+}
+
+// The gradient-decrease test: for each listed cell, ||grad h||_2 at the caller's alpha and at
+// the network's proposal anet[slot] (FP64 sums, one wave per cell); the proposal becomes the
+// starting point (alpha, alpha0 and the line-search anchor) only if its gradient is smaller.
+// Either way the mixed mode's first shift is log sum_q w e at the chosen point.
+template <int NM>
+__global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double *__restrict__ Mg,
+                                                         const double *__restrict__ wg, State S,
+                                                         const int *__restrict__ W,
+                                                         const int *__restrict__ nWp,
+                                                         const double *__restrict__ anet)
+{
+    constexpr int NMS = NM | 1;
+    __shared__ double sM[kChunk * NMS];
+    __shared__ double sW[kChunk];
+    __shared__ double sA[kWavesPerBlock][2][NM];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nW = uniform_int(*nWp);
+    const int nBlocksNeeded = (nW + kWavesPerBlock - 1) / kWavesPerBlock;
+    for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
+        const int slot = blk * kWavesPerBlock + wv;
+        const int cell = slot < nW ? uniform_int(W[slot]) : -1;
+        if (cell >= 0 && lane < NM) {
+            sA[wv][0][lane] = S.vec[(size_t)cell * kVec * NM + kVa * NM + lane];
+            sA[wv][1][lane] = anet[(size_t)slot * NM + lane];
+        }
+        double f1 = 0.0, f2 = 0.0, g1[NM], g2[NM];
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            g1[i] = 0.0;
+            g2[i] = 0.0;
+        }
+        for (int ch = 0; ch < P.nqPad / kChunk; ++ch) {
+            __syncthreads();
+            for (int e = tid; e < kChunk * NM; e += kBlock) {
+                const int rr = e / NM, cc = e - rr * NM;
+                sM[rr * NMS + cc] = Mg[(size_t)ch * kChunk * NM + e];
+            }
+            if (tid < kChunk)
+                sW[tid] = wg[(size_t)ch * kChunk + tid];
+            __syncthreads();
+            if (cell >= 0) {
+                const double *rowp = &sM[lane * NMS];
+                double x1 = 0.0, x2 = 0.0;
+#pragma unroll
+                for (int i = 0; i < NM; ++i) {
+                    x1 = fma(sA[wv][0][i], rowp[i], x1);
+                    x2 = fma(sA[wv][1][i], rowp[i], x2);
+                }
+                const double e1 = sW[lane] * exp(x1), e2 = sW[lane] * exp(x2);
+                f1 = f1 + e1;
+                f2 = f2 + e2;
+#pragma unroll
+                for (int i = 0; i < NM; ++i) {
+                    g1[i] = fma(e1, rowp[i], g1[i]);
+                    g2[i] = fma(e2, rowp[i], g2[i]);
+                }
+            }
+        }
+        if (cell < 0)
+            continue;
+        const double *v = S.vec + (size_t)cell * kVec * NM;
+        double n1 = 0.0, n2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            const double ui = v[kVu * NM + i];
+            const double d1 = wave_sum(g1[i]) - ui, d2 = wave_sum(g2[i]) - ui;
+            n1 = fma(d1, d1, n1);
+            n2 = fma(d2, d2, n2);
+        }
+        const bool take = n2 < n1;  // false when the proposal overflowed (NaN or inf)
+        const double fs = wave_sum(take ? f2 : f1);
+        if (take && lane < NM) {
+            const double an = sA[wv][1][lane];
+            double *vw = S.vec + (size_t)cell * kVec * NM;
+            vw[kVa * NM + lane] = an;
+            vw[kVa0 * NM + lane] = an;
+            vw[kVaOld * NM + lane] = an;
+        }
+        if (lane == 0) {
+            if (take)
+                S.ints[(size_t)cell * kInt + kIflags] |= C3D_FLAG_WARM;
+            if (P.mixed) {
+                const double lf = log(fs);
+                S.scal[(size_t)cell * kScal + kSshift] = (lf > -700.0 && lf < 700.0) ? lf : 0.0;
+            }
+        }
     }
 }
 
@@ -1097,6 +1532,51 @@ __global__ __launch_bounds__(kBlock) void k_hessian_fp64(int64_t ncells, int nqP
     }
 }
 
+// H only, mixed mode (FP32 matrix cores, zero shift): one wave per cell.
+template <int NM>
+__global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nqPad,
+                                                         const double *__restrict__ Mg,
+                                                         const double *__restrict__ wg,
+                                                         const double *__restrict__ alpha_g,
+                                                         double *hs)
+{
+    constexpr int NT = (NM + 15) / 16;
+    constexpr int NB = NT * (NT + 1) / 2;
+    constexpr int NMS = NM | 1;
+    __shared__ double sM[kChunk * NMS];
+    __shared__ double sW[kChunk];
+    __shared__ double sA[kWavesPerBlock][NM];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t cell = (int64_t)blockIdx.x * kWavesPerBlock + wv;
+    const bool have = cell < ncells;
+    if (lane < NM)
+        sA[wv][lane] = have ? alpha_g[cell * NM + lane] : 0.0;
+    f4 acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+        acc[b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    double fpart = 0.0, gpart[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        gpart[i] = 0.0;
+    for (int c = 0; c < nqPad / kChunk; ++c) {
+        __syncthreads();
+        for (int e = tid; e < kChunk * NM; e += kBlock) {
+            int rr = e / NM, cc = e - rr * NM;
+            sM[rr * NMS + cc] = Mg[(size_t)c * kChunk * NM + e];
+        }
+        if (tid < kChunk)
+            sW[tid] = wg[(size_t)c * kChunk + tid];
+        __syncthreads();
+        if (have)
+            chunk_mixed<NM>(sM, sW[lane], sA[wv], 0.0, acc, fpart, gpart, lane);
+    }
+    if (!have)
+        return;
+    double gsl;
+    finish_mixed<NM>(acc, fpart, gpart, 0.0, hs + cell * NM * NM, gsl, lane);
+}
+
 // The dense part of one Newton iteration for a batch of given Hessians (one wave each):
 // info = dpotrf, rcond = dpocon(norm1(H)), d = -H^{-1} g (custom or LAPACK solve order).
 template <int NM>
@@ -1176,7 +1656,7 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
                        c3d_cell_stats *stats, int maxPasses, hipStream_t s)
 {
     const size_t n = (size_t)P.ncells;
-    size_t bytes = n * (kVec * NM + kScal + (size_t)NM * NM + 1) * sizeof(double) +
+    size_t bytes = n * (kVec * NM + kScal + (size_t)NM * NM + 1 + NM) * sizeof(double) +
                    n * (kInt + 2) * sizeof(int) + 4 * sizeof(int);
     hipError_t e = grow(ctx->closureState, bytes);
     if (e != hipSuccess)
@@ -1187,11 +1667,13 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
     S.scal = S.vec + n * kVec * NM;
     S.H = S.scal + n * kScal;
     S.fdir = S.H + n * NM * NM;
-    S.ints = (int *)(S.fdir + n);
+    S.gs = S.fdir + n;
+    S.ints = (int *)(S.gs + n * NM);
     int *W0 = S.ints + n * kInt;
     int *W1 = W0 + n;
     int *counts = W1 + n;  // [0],[1] worklist sizes
     const bool ref = ctx->precision == C3D_PREC_FP64_REFORDER;
+    const bool mixed = P.mixed != 0;
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, counts + 0, 0);
     ctx->evUsed = 0;
     ctx->evSpans.clear();
@@ -1205,6 +1687,29 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
     const unsigned gInit = (unsigned)((n * 64 + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_init<NM>, dim3(gInit), dim3(kBlock), 0, s, P, u, alpha, S, W0, counts,
                        stats, ctx->d_status);
+    if (ctx->warm && ctx->mlp.have) {
+        const size_t xb = (n * C3D_MLP_IN * sizeof(float) + 255) / 256 * 256;
+        e = grow(ctx->warmBuf, xb + n * NM * sizeof(double));
+        if (e != hipSuccess)
+            return e;
+        float *X = (float *)ctx->warmBuf.p;
+        double *anet = (double *)((char *)ctx->warmBuf.p + xb);
+        hipEvent_t *ev = ctx->profiling ? take_events(ctx, 2, 2) : nullptr;
+        if (ctx->profiling && !ev)
+            return hipErrorOutOfMemory;
+        if (ev)
+            (void)hipEventRecord(ev[0], s);
+        hipLaunchKernelGGL(k_warm_features<NM>, dim3((unsigned)((n * C3D_MLP_IN + 255) / 256)),
+                           dim3(256), 0, s, (int64_t)n, S, W0, counts,
+                           (const float *)ctx->mlp.normDev.p, X);
+        e = mlp_infer(ctx, (int64_t)n, X, anet, s);
+        if (e != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(k_warm_select<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P,
+                           ctx->d_M, ctx->d_wpad, S, W0, counts, anet);
+        if (ev)
+            (void)hipEventRecord(ev[1], s);
+    }
     int cur = 0;
     int64_t left = (int64_t)n;  // upper bound on the listed cells until the first read-back
     int pass = 0;
@@ -1219,17 +1724,20 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
             if (ctx->profiling && !ev)
                 return hipErrorOutOfMemory;
             if (ev)
-                hipEventRecord(ev[0], s);
+                (void)hipEventRecord(ev[0], s);
+#define C3D_TAIL(R, X)                                                                     \
+    hipLaunchKernelGGL((k_tail<NM, R, X>), dim3((unsigned)left), dim3(kWave), 0, s, P, ctx->d_M, \
+                       ctx->d_wpad, S, Wc, nc, Wn, nn, u, alpha, stats, ctx->d_status, K,      \
+                       cntTail)
             if (ref)
-                hipLaunchKernelGGL((k_tail<NM, true>), dim3((unsigned)left), dim3(kWave), 0, s,
-                                   P, ctx->d_M, ctx->d_wpad, S, Wc, nc, Wn, nn, u, alpha, stats,
-                                   ctx->d_status, K, cntTail);
+                C3D_TAIL(true, false);
+            else if (mixed)
+                C3D_TAIL(false, true);
             else
-                hipLaunchKernelGGL((k_tail<NM, false>), dim3((unsigned)left), dim3(kWave), 0, s,
-                                   P, ctx->d_M, ctx->d_wpad, S, Wc, nc, Wn, nn, u, alpha, stats,
-                                   ctx->d_status, K, cntTail);
+                C3D_TAIL(false, false);
+#undef C3D_TAIL
             if (ev)
-                hipEventRecord(ev[1], s);
+                (void)hipEventRecord(ev[1], s);
             pass += K;
             ++ctx->lastTailLaunches;
         } else {
@@ -1237,21 +1745,30 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
             if (ctx->profiling && !ev)
                 return hipErrorOutOfMemory;
             if (ev)
-                hipEventRecord(ev[0], s);
+                (void)hipEventRecord(ev[0], s);
             if (ref)
                 hipLaunchKernelGGL(k_sweep_ref<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s,
                                    P, ctx->d_M, ctx->d_wpad, S, Wc, nc);
+            else if (mixed)
+                hipLaunchKernelGGL(k_sweep_mixed<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0,
+                                   s, P, ctx->d_M, ctx->d_wpad, S, Wc, nc);
             else
                 hipLaunchKernelGGL(k_sweep<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P,
                                    ctx->d_M, ctx->d_wpad, S, Wc, nc);
             if (ev) {
-                hipEventRecord(ev[1], s);
-                hipEventRecord(ev[2], s);
+                (void)hipEventRecord(ev[1], s);
+                (void)hipEventRecord(ev[2], s);
             }
-            hipLaunchKernelGGL(k_step<NM>, dim3(grid_cap(ctx, 16)), dim3(kBlock), 0, s, P, S, Wc,
-                               nc, Wn, nn, u, alpha, stats, ctx->d_status, cntBulk);
+            if (ref)
+                hipLaunchKernelGGL((k_step<NM, true>), dim3(grid_cap(ctx, 16)), dim3(kBlock), 0,
+                                   s, P, S, Wc, nc, Wn, nn, u, alpha, stats, ctx->d_status,
+                                   cntBulk);
+            else
+                hipLaunchKernelGGL((k_step<NM, false>), dim3(grid_cap(ctx, 16)), dim3(kBlock),
+                                   0, s, P, S, Wc, nc, Wn, nn, u, alpha, stats, ctx->d_status,
+                                   cntBulk);
             if (ev)
-                hipEventRecord(ev[3], s);
+                (void)hipEventRecord(ev[3], s);
             ++pass;
         }
         ctx->lastPasses = pass;
@@ -1291,6 +1808,9 @@ hipError_t launch_closure_fp64(c3d_ctx_s *ctx, int64_t ncells, double *u, double
     P.m0 = ctx->cc.m0;
     P.isoAlpha0 = ctx->cc.isoAlpha0;
     P.minf = ctx->cc.minf;
+    P.mixed = ctx->precision == C3D_PREC_MIXED;
+    P.mixedMaxIter = ctx->mixedMaxIter;
+    P.mixedCondMax = ctx->mixedCondMax;
     if (ncells <= 0)
         return hipSuccess;
     if (ncells > INT32_MAX / 2)
@@ -1329,6 +1849,18 @@ hipError_t launch_hessian_fp64(c3d_ctx_s *ctx, int64_t ncells, const double *alp
     if (ncells <= 0)
         return hipSuccess;
     dim3 grid((unsigned)((ncells + kWavesPerBlock - 1) / kWavesPerBlock));
+    if (ctx->precision == C3D_PREC_MIXED) {
+#define C3D_HM(NMV) hipLaunchKernelGGL(k_hessian_mixed<NMV>, grid, dim3(kBlock), 0, s, ncells, ctx->nqPad, ctx->d_M, ctx->d_wpad, alpha, hs)
+        switch (ctx->t.nm) {
+        case 4: C3D_HM(4); break;
+        case 9: C3D_HM(9); break;
+        case 16: C3D_HM(16); break;
+        case 25: C3D_HM(25); break;
+        default: return hipErrorInvalidValue;
+        }
+#undef C3D_HM
+        return hipGetLastError();
+    }
     switch (ctx->t.nm) {
     case 4: hipLaunchKernelGGL(k_hessian_fp64<4>, grid, dim3(kBlock), 0, s, ncells, ctx->nqPad, ctx->d_M, ctx->d_wpad, alpha, hs); break;
     case 9: hipLaunchKernelGGL(k_hessian_fp64<9>, grid, dim3(kBlock), 0, s, ncells, ctx->nqPad, ctx->d_M, ctx->d_wpad, alpha, hs); break;

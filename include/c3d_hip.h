@@ -44,11 +44,17 @@ extern "C" {
 #define C3D_FLAG_U0_NONPOS 2
 #define C3D_FLAG_LADDER 4
 #define C3D_FLAG_BUDGET 8      /* cell left unsolved by the pass budget */
+#define C3D_FLAG_FP64_FALLBACK 16 /* mixed mode: the cell was solved by the FP64 path from
+                                     its input (the mixed iteration declined it) */
 
 /* arithmetic modes of a context (c3d_set_precision) */
 #define C3D_PREC_FP64 0          /* FP64; quadrature sums on the matrix cores (default) */
 #define C3D_PREC_FP64_REFORDER 1 /* FP64 in the reference's operation order with a
                                     deterministic exp/log: bit-reproducible on a CPU */
+#define C3D_PREC_MIXED 2         /* closure: Hessian on the FP32 matrix cores under a
+                                    log-sum-exp shift; exponents, f, gradient, Cholesky,
+                                    condition estimate and stopping test in FP64; cells
+                                    that turn ill-conditioned restart in the FP64 path */
 
 typedef struct c3d_ctx_s *c3d_ctx;
 
@@ -157,6 +163,13 @@ int c3d_closure(c3d_ctx ctx, int64_t ncells, double *u, double *alpha,
  * No counterpart in the reference (whose results depend on the host libm). */
 int c3d_set_precision(c3d_ctx ctx, int mode);
 
+/* Parameters of C3D_PREC_MIXED: a cell leaves the FP32-Hessian iteration and restarts from
+ * its input in the FP64 path when its condition estimate exceeds cond_max (default 1e5),
+ * its factorisation or line search fails, or it has taken max_iter iterations (default 40).
+ * Such a cell then takes exactly the FP64 path's decisions (regularisation included) and is
+ * flagged C3D_FLAG_FP64_FALLBACK.  No counterpart in the reference. */
+int c3d_set_mixed(c3d_ctx ctx, double cond_max, int max_iter);
+
 /* Kernel-time accounting of c3d_closure (HIP events on the closure's stream around every
  * launch), for throughput and roofline reporting. */
 typedef struct {
@@ -167,6 +180,7 @@ typedef struct {
     double sweep_ms;          /* summed duration of the bulk sweep kernels */
     double step_ms;           /* summed duration of the bulk Newton-step kernels */
     double tail_ms;           /* summed duration of the fused tail kernels */
+    double warm_ms;           /* warm start: features, network, selection */
 } c3d_closure_profile;
 
 int c3d_set_profiling(c3d_ctx ctx, int enable);
@@ -186,7 +200,8 @@ int c3d_set_pass_budget(c3d_ctx ctx, int max_passes);
  * disables the fused kernel.  Defaults 8192 and 32.  No counterpart in the reference. */
 int c3d_set_tail(c3d_ctx ctx, int max_cells, int passes_per_launch);
 
-/* H(alpha) for ncells cells (solveHMn semantics), device pointers, hs [ncells][nm][nm]. */
+/* H(alpha) for ncells cells (solveHMn semantics), device pointers, hs [ncells][nm][nm].
+ * Under C3D_PREC_MIXED the sums run on the FP32 matrix cores (relative error ~1e-6). */
 int c3d_hessian(c3d_ctx ctx, int64_t ncells, const double *alpha, double *hs, int useGaunt,
                 int useGauntSparse, void *stream);
 
@@ -235,6 +250,65 @@ int c3d_update(c3d_ctx ctx, int nx, int ny, int nz, double *u, double *alpha,
                const double *sigmaT, const double *sigmaS, double dx, double dy, double dz,
                double theta, double dt, int solver, int pnUseKinetic,
                const c3d_closure_opts *opts, c3d_cell_stats *stats, void *stream);
+
+/* ======================================================================
+ * 3. Learned warm start (no counterpart in the reference)
+ * ======================================================================
+ * A multilayer perceptron 25 -> 128 -> 128 -> 25 with GELU activations maps the
+ * standardised moments x_k = (u_k/u0 - x_mean_k)/x_std_k (k < nm, zero beyond) of a cell to
+ * a standardised initial alpha of the u0-scaled dual problem,
+ * alpha_k = alpha_mean_k + alpha_std_k * y_k.  Parameters (FP32, C3D_MLP_NPARAM):
+ * W1[25][128] b1[128] W2[128][128] b2[128] W3[128][25] b3[25], z_j = b_j + sum_i x_i W[i][j].
+ * With the warm start enabled, c3d_closure evaluates the network for every cell (FP16
+ * weights on the matrix cores) and starts Newton from the proposal only if it has a smaller
+ * ||grad h||_2 (FP64) than the caller's alpha; otherwise from the caller's alpha. */
+#define C3D_MLP_IN 25
+#define C3D_MLP_HID 128
+#define C3D_MLP_NPARAM 23065
+#define C3D_FLAG_WARM 32 /* the closure started from the network's proposal */
+
+typedef struct {
+    float x_mean[C3D_MLP_IN], x_std[C3D_MLP_IN];
+    float alpha_mean[C3D_MLP_IN], alpha_std[C3D_MLP_IN];
+} c3d_mlp_norm;
+
+/* Train the context's network on n (u, alpha) pairs (device, [n][nm], e.g. a closure's
+ * output): normalisation from the first 65536 pairs, Glorot-uniform initialisation from
+ * `seed`, then mse_steps Adam steps on the mean squared error of the standardised alpha
+ * (step size lr, cosine decay) followed by dual_steps Adam steps on the mean dual gap
+ * f(alpha_net) - f(alpha_pair) of the u0-scaled problems (step size 0.3 lr, cosine decay;
+ * its gradient with respect to alpha is the moment residual, evaluated in FP64 on the
+ * context's quadrature).  Minibatches of `batch` pairs are drawn by a counter RNG.
+ * loss_first / loss_last (host, may be NULL) receive the first loss of the first phase run
+ * and the last loss of the last one.  Synchronous. */
+int c3d_mlp_train(c3d_ctx ctx, int64_t n, const double *u, const double *alpha, int mse_steps,
+                  int dual_steps, int batch, float lr, uint64_t seed, float *loss_first,
+                  float *loss_last, void *stream);
+/* Copy the trained parameters (host, C3D_MLP_NPARAM floats) and normalisation out / in. */
+int c3d_mlp_get(c3d_ctx ctx, float *params, c3d_mlp_norm *norm);
+int c3d_mlp_set(c3d_ctx ctx, const float *params, const c3d_mlp_norm *norm);
+/* Enable (1) or disable (0) the warm start in c3d_closure (needs a trained or set network). */
+int c3d_set_warmstart(c3d_ctx ctx, int enable);
+/* Building blocks, device pointers (for checking the kernels):
+ * features X [n][25] (and targets T [n][25] if alpha and T are not NULL) of (u, alpha);
+ * Y [n][25] = net(X), FP32 (fp16 = 0) or the FP16 matrix-core inference path (fp16 = 1,
+ * then with the context's own network; Y is written as alpha_mean + alpha_std*net in FP64
+ * to the double array Yd [n][nm] instead);
+ * loss (device float) and grad (device, C3D_MLP_NPARAM) of the mean squared error over
+ * outputs k < nm. */
+int c3d_mlp_features(c3d_ctx ctx, int64_t n, const double *u, const double *alpha,
+                     const c3d_mlp_norm *norm, float *X, float *T, void *stream);
+int c3d_mlp_forward(c3d_ctx ctx, int64_t n, const float *params, const float *X, float *Y,
+                    void *stream);
+int c3d_mlp_infer(c3d_ctx ctx, int64_t n, const float *X, double *Yd, void *stream);
+int c3d_mlp_loss_grad(c3d_ctx ctx, int64_t n, const float *params, const float *X,
+                      const float *T, float *grad, float *loss, void *stream);
+/* The dual-gap loss of the second training phase without the constant f*: loss = mean over
+ * rows of f(alpha(net(X_i))) for the u0-scaled moments of u [n][nm] (device), alpha =
+ * alpha_mean + alpha_std * net (norm, host), and its gradient (device, C3D_MLP_NPARAM). */
+int c3d_mlp_dual_loss_grad(c3d_ctx ctx, int64_t n, const float *params, const float *X,
+                           const double *u, const c3d_mlp_norm *norm, float *grad, float *loss,
+                           void *stream);
 
 /* Synthetic closure batch on the device: per cell s ~ U[smin,smax], alpha*_k ~ N(0,s^2),
  * u = sum_q w_q m_q exp(alpha*.m_q), alpha0 from init.cpp:265.  SplitMix64 counter RNG
