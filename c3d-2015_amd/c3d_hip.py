@@ -30,6 +30,7 @@ class ClosureProfile(C.Structure):
 
 MLP_IN, MLP_HID, MLP_NPARAM = 25, 128, 23065
 FLAG_WARM = 32
+FLAG_FP64_FINISH = 64
 
 
 class MlpNorm(C.Structure):
@@ -76,7 +77,7 @@ SIGNATURES = [
     ("c3d_get_closure_profile", C.c_int, [_vp, C.POINTER(ClosureProfile)]),
     ("c3d_set_tail", C.c_int, [_vp, _i32, _i32]),
     ("c3d_set_precision", C.c_int, [_vp, C.c_int]),
-    ("c3d_set_mixed", C.c_int, [_vp, _d, C.c_int]),
+    ("c3d_set_mixed", C.c_int, [_vp, _d, C.c_int, _d]),
     ("c3d_set_pass_budget", C.c_int, [_vp, C.c_int]),
     ("c3d_hessian", C.c_int, [_vp, _i64, _vp, _vp, C.c_int, C.c_int, _vp]),
     ("c3d_dense_solve", C.c_int, [C.c_int, _i64] + [_vp] * 6 + [C.c_int, _vp]),
@@ -204,8 +205,9 @@ class Ctx:
     def set_precision(self, mode):
         return check("c3d_set_precision", self.L.c3d_set_precision(self.h, mode))
 
-    def set_mixed(self, cond_max=1e5, max_iter=40):
-        return check("c3d_set_mixed", self.L.c3d_set_mixed(self.h, cond_max, max_iter))
+    def set_mixed(self, cond_max=1e8, max_iter=40, switch_tol=1e-4):
+        return check("c3d_set_mixed", self.L.c3d_set_mixed(self.h, cond_max, max_iter,
+                                                           switch_tol))
 
     def set_pass_budget(self, n):
         return check("c3d_set_pass_budget", self.L.c3d_set_pass_budget(self.h, n))

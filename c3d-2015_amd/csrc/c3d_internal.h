@@ -76,8 +76,9 @@ struct c3d_ctx_s {
     c3d::DevBuf warmBuf;       // features and proposals of the closure's cells
     int maxPasses = 100000;     // sweep passes per closure call before cells are reported
     int precision = C3D_PREC_FP64;
-    double mixedCondMax = 1e5;  // C3D_PREC_MIXED hand-over to the FP64 path
+    double mixedCondMax = 1e8;  // C3D_PREC_MIXED hand-over to the FP64 path
     int mixedMaxIter = 40;
+    double mixedSwitch = 1e-4;
     // profiling of the last closure call
     bool profiling = false;
     std::vector<hipEvent_t> evPool;

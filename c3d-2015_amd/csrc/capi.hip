@@ -315,12 +315,13 @@ int c3d_set_precision(c3d_ctx ctx, int mode)
     return C3D_OK;
 }
 
-int c3d_set_mixed(c3d_ctx ctx, double cond_max, int max_iter)
+int c3d_set_mixed(c3d_ctx ctx, double cond_max, int max_iter, double switch_tol)
 {
-    if (!ctx || !(cond_max >= 1.0) || max_iter < 1)
+    if (!ctx || !(cond_max >= 1.0) || max_iter < 1 || !(switch_tol > 0.0))
         return C3D_ERR_ARG;
     ctx->mixedCondMax = cond_max;
     ctx->mixedMaxIter = max_iter;
+    ctx->mixedSwitch = switch_tol;
     return C3D_OK;
 }
 

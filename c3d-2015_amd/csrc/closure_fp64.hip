@@ -58,7 +58,6 @@ struct State {
     int *ints;     // [ncells][kInt]
     double *H;     // [ncells][nm][nm]
     double *fdir;  // [ncells]  sum_q w_q e_q
-    double *gs;    // [ncells][nm]  sum_q w_q e_q m_q (mixed mode only)
 };
 
 struct KParams {
@@ -66,12 +65,33 @@ struct KParams {
     int nq, nqPad, maxIter, unbatched;
     double tol, tolGamma, condHMax;
     double m0, isoAlpha0, minf;
-    // mixed mode: a cell leaves the FP32-Hessian iteration for the FP64 path (restarting
-    // from its input) on condest > mixedCondMax, a failed factorisation or line search, or
-    // more than mixedMaxIter iterations
+    // mixed mode: a cell leaves the FP32-Hessian iteration for the FP64 path, restarting
+    // from its input on condest > mixedCondMax, a failed factorisation or line search, or
+    // mixedMaxIter iterations; continuing from its iterate once ||g|| <= mixedSwitch
     int mixed, mixedMaxIter;
-    double mixedCondMax;
+    double mixedCondMax, mixedSwitch;
 };
+
+// The worklists of one pass generation: cells in the mixed mode's FP32-Hessian phase and
+// cells on the FP64 path (every cell outside the mixed mode).  Slot s of a pass is
+// Wm[s] for s < *nm, else Wf[s - *nm].
+struct Lists {
+    int *Wm, *Wf;
+    int *nm, *nf;
+};
+
+__device__ __forceinline__ int list_cell(const Lists &L, int slot, int nWm)
+{
+    return slot < nWm ? L.Wm[slot] : L.Wf[slot - nWm];
+}
+
+__device__ __forceinline__ void list_append(const Lists &L, bool mixedPhase, int c)
+{
+    if (mixedPhase)
+        L.Wm[atomicAdd(L.nm, 1)] = c;
+    else
+        L.Wf[atomicAdd(L.nf, 1)] = c;
+}
 
 __device__ __forceinline__ double rdl(double v, int src)
 {
@@ -357,8 +377,7 @@ struct Dense {
 template <int NM>
 __global__ __launch_bounds__(kBlock) void k_init(KParams P, const double *__restrict__ u_g,
                                                   const double *__restrict__ alpha_g, State S,
-                                                  int *W, int *nW, c3d_cell_stats *stats,
-                                                  int *status)
+                                                  Lists L, c3d_cell_stats *stats, int *status)
 {
     const int lane = threadIdx.x & 63;
     const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -415,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_init(KParams P, const double *__rest
         in[kIflags] = 0;
         in[kIphase] = kPhaseH;
         in[kImaxIt] = P.maxIter;
-        W[atomicAdd(nW, 1)] = (int)c;
+        list_append(L, P.mixed != 0, (int)c);
     }
 }
 
@@ -660,46 +679,52 @@ __global__ __launch_bounds__(kBlock) void k_sweep_ref(KParams P, const double *_
 }
 
 // ---------------------------------------------------------------- mixed-precision sweep
-// One cell per wave.  FP64 on the vector units: the exponent alpha.m_q (FMA chain), the
-// shifted weight w_q e^{alpha.m_q - shift}, sum_q w e and sum_q w e m_q (so f and the
-// gradient, hence the stopping test, keep FP64 accuracy).  The Hessian on the FP32 matrix
-// cores (V_MFMA_F32_16X16X4_F32, 4 ordinates per instruction) from the shifted weights;
-// the shift is log sum_q w e at the cell's last Newton iterate (a log-sum-exp shift), so the
-// FP32 operands stay near unit scale.  FP32 accumulator register r of lane l holds
-// D[4(l/16) + r][l%16].
+// Phase one of the mixed mode.  FP64 on the vector units: the exponent alpha.m_q (FMA chain)
+// and the shifted weight w_q e^{alpha.m_q - shift}, summed in FP64 for f (so the Armijo
+// test keeps FP64 resolution).  The Hessian on the FP32 matrix cores
+// (V_MFMA_F32_16X16X4_F32, 4 ordinates per instruction) from the shifted weights, 4 cells
+// per wave sharing every B operand; the shift is log sum_q w e at the cell's last Newton
+// iterate (a log-sum-exp shift), so the FP32 operands stay near unit scale.  The gradient
+// of this phase is H[0][:]/m0 - u (FP32 accuracy); once it falls below the hand-over
+// tolerance the cell finishes on the FP64 path, whose sweeps decide the stopping test.
+// FP32 accumulator register r of lane l holds D[4(l/16) + r][l%16].
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-template <int NM>
+template <int NM, int CW>
 __device__ __forceinline__ void chunk_mixed(const double *sMc, double wl, const double *sAc,
-                                            double shift,
-                                            f4 (&acc)[(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
-                                            double &fpart, double (&gpart)[NM], int lane)
+                                            const double *shift,
+                                            f4 (&acc)[CW][(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
+                                            double (&fpart)[CW], int lane)
 {
     constexpr int NT = (NM + 15) / 16;
     constexpr int NMS = NM | 1;
     const int gq = lane >> 4, rq = lane & 15;
     const double *rowp = &sMc[lane * NMS];
-    double x = 0.0;
+    float wef[CW];
 #pragma unroll
-    for (int i = 0; i < NM; ++i)
-        x = fma(sAc[i], rowp[i], x);
-    const double we = wl * exp(x - shift);
-    fpart = fpart + we;
+    for (int c = 0; c < CW; ++c) {
+        double x = 0.0;
 #pragma unroll
-    for (int i = 0; i < NM; ++i)
-        gpart[i] = fma(we, rowp[i], gpart[i]);
-    const float wef = (float)we;
+        for (int i = 0; i < NM; ++i)
+            x = fma(sAc[c * NM + i], rowp[i], x);
+        const double we = wl * exp(x - shift[c]);
+        fpart[c] = fpart[c] + we;
+        wef[c] = (float)we;
+    }
 #pragma unroll
     for (int k = 0; k < kChunk / 4; ++k) {
         const int ql = 4 * k + gq;
         const double *mr = &sMc[ql * NMS];
         const float b0 = (rq < NM) ? (float)mr[rq] : 0.0f;
         const float b1 = (NT == 2 && 16 + rq < NM) ? (float)mr[16 + rq] : 0.0f;
-        const float wq = __shfl(wef, ql);
-        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b0, b0, acc[0], 0, 0, 0);
-        if constexpr (NT == 2) {
-            acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b1, b0, acc[1], 0, 0, 0);
-            acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b1, b1, acc[2], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            const float wq = __shfl(wef[c], ql);
+            acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b0, b0, acc[c][0], 0, 0, 0);
+            if constexpr (NT == 2) {
+                acc[c][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b1, b0, acc[c][1], 0, 0, 0);
+                acc[c][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b1, b1, acc[c][2], 0, 0, 0);
+            }
         }
     }
 }
@@ -722,23 +747,12 @@ __device__ __forceinline__ double f_only_mixed(const KParams &P, const double *_
     return wave_sum(fpart) * exp(shift);
 }
 
-// Results of a mixed sweep: fdir and the gradient sums (lane i < NM gets component i) are
-// returned; H = e^shift * H32 is written row-major to H.
+// H = e^shift * H32, row-major, lower triangle mirrored
 template <int NM>
-__device__ __forceinline__ double finish_mixed(const f4 *acc, double fpart, const double *gpart,
-                                               double shift, double *H, double &gsLane, int lane)
+__device__ __forceinline__ void store_H32(const f4 *acc, double es, double *H, int lane)
 {
     constexpr int NT = (NM + 15) / 16;
     constexpr int NB = NT * (NT + 1) / 2;
-    const double es = exp(shift);
-    const double fdir = wave_sum(fpart) * es;
-    gsLane = 0.0;
-#pragma unroll
-    for (int i = 0; i < NM; ++i) {
-        const double t = wave_sum(gpart[i]) * es;
-        if (lane == i)
-            gsLane = t;
-    }
     const int gq = lane >> 4, rq = lane & 15;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -753,12 +767,9 @@ __device__ __forceinline__ double finish_mixed(const f4 *acc, double fpart, cons
             }
         }
     }
-    return fdir;
 }
 
-// Sweep pass of the mixed mode: one cell per wave, 4 cells per block sharing each LDS
-// chunk.  Cells the mixed iteration handed to the FP64 path are swept exactly as k_sweep
-// sweeps them (same bits).
+// Sweep pass of the mixed phase: 4 cells per wave, 16 per block sharing each LDS chunk.
 template <int NM>
 __global__ __launch_bounds__(kBlock) void k_sweep_mixed(KParams P, const double *__restrict__ Mg,
                                                          const double *__restrict__ wg, State S,
@@ -768,35 +779,38 @@ __global__ __launch_bounds__(kBlock) void k_sweep_mixed(KParams P, const double 
     constexpr int NT = (NM + 15) / 16;
     constexpr int NB = NT * (NT + 1) / 2;
     constexpr int NMS = NM | 1;
+    constexpr int CW = kCellsPerWave;
     __shared__ double sM[kChunk * NMS];
     __shared__ double sW[kChunk];
-    __shared__ double sA[kWavesPerBlock][NM];
+    __shared__ double sA[kWavesPerBlock][CW * NM];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nW = uniform_int(*nWp);
-    const int nBlocksNeeded = (nW + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int groups = (nW + CW - 1) / CW;
+    const int nBlocksNeeded = (groups + kWavesPerBlock - 1) / kWavesPerBlock;
     for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
-        const int slot = blk * kWavesPerBlock + wv;
-        const int cell = slot < nW ? uniform_int(W[slot]) : -1;
-        bool mixedCell = false;
-        double shift = 0.0;
-        if (cell >= 0) {
-            mixedCell = !(S.ints[(size_t)cell * kInt + kIflags] & C3D_FLAG_FP64_FALLBACK);
-            shift = S.scal[(size_t)cell * kScal + kSshift];
+        const int grp = blk * kWavesPerBlock + wv;
+        int cells[CW];
+        double shift[CW];
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            const int slot = grp * CW + c;
+            cells[c] = (grp < groups && slot < nW) ? W[slot] : -1;
+            shift[c] = cells[c] >= 0 ? S.scal[(size_t)cells[c] * kScal + kSshift] : 0.0;
             if (lane < NM)
-                sA[wv][lane] = S.vec[(size_t)cell * kVec * NM + kVa * NM + lane];
+                sA[wv][c * NM + lane] =
+                    cells[c] >= 0 ? S.vec[(size_t)cells[c] * kVec * NM + kVa * NM + lane] : 0.0;
         }
-        d4 acc64[1][NB];
-        f4 acc32[NB];
+        f4 acc[CW][NB];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            acc64[0][b] = d4{0.0, 0.0, 0.0, 0.0};
-            acc32[b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-        }
-        double fpart[1] = {0.0};
-        double gpart[NM];
+        for (int c = 0; c < CW; ++c)
 #pragma unroll
-        for (int i = 0; i < NM; ++i)
-            gpart[i] = 0.0;
+            for (int b = 0; b < NB; ++b)
+                acc[c][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        double fpart[CW];
+#pragma unroll
+        for (int c = 0; c < CW; ++c)
+            fpart[c] = 0.0;
+        const bool any = cells[0] >= 0;
         for (int ch = 0; ch < P.nqPad / kChunk; ++ch) {
             __syncthreads();
             for (int e = tid; e < kChunk * NM; e += kBlock) {
@@ -806,28 +820,18 @@ __global__ __launch_bounds__(kBlock) void k_sweep_mixed(KParams P, const double 
             if (tid < kChunk)
                 sW[tid] = wg[(size_t)ch * kChunk + tid];
             __syncthreads();
-            if (cell >= 0) {
-                if (mixedCell)
-                    chunk_mixed<NM>(sM, sW[lane], sA[wv], shift, acc32, fpart[0], gpart, lane);
-                else
-                    chunk_mfma<NM, 1>(sM, sW[lane], sA[wv], acc64, fpart, lane);
-            }
+            if (any)
+                chunk_mixed<NM, CW>(sM, sW[lane], sA[wv], shift, acc, fpart, lane);
         }
-        if (cell < 0)
-            continue;
-        double *H = S.H + (size_t)cell * NM * NM;
-        if (mixedCell) {
-            double gsl;
-            const double fd = finish_mixed<NM>(acc32, fpart[0], gpart, shift, H, gsl, lane);
-            if (lane < NM)
-                S.gs[(size_t)cell * NM + lane] = gsl;
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            const double fs = wave_sum(fpart[c]);
+            if (cells[c] < 0)
+                continue;
+            const double es = exp(shift[c]);
             if (lane == 0)
-                S.fdir[cell] = fd;
-        } else {
-            const double fs = wave_sum(fpart[0]);
-            if (lane == 0)
-                S.fdir[cell] = fs;
-            store_H<NM>(acc64[0], H, lane);
+                S.fdir[cells[c]] = fs * es;
+            store_H32<NM>(acc[c], es, S.H + (size_t)cells[c] * NM * NM, lane);
         }
     }
 }
@@ -844,7 +848,7 @@ struct Cell {
     // mixed mode and not yet handed to the FP64 path
     __device__ __forceinline__ bool mixed(const KParams &P) const
     {
-        return P.mixed && !(flags & C3D_FLAG_FP64_FALLBACK);
+        return P.mixed && !(flags & (C3D_FLAG_FP64_FALLBACK | C3D_FLAG_FP64_FINISH));
     }
 
     __device__ __forceinline__ void load(const State &S, int c, int lane)
@@ -976,14 +980,13 @@ struct Cell {
     }
 
     // One step of the state machine with the sweep result at the current alpha (H row-major,
-    // in LDS or global memory; fdir = sum_q w_q e_q; gs = sum_q w_q e_q m_q, read in mixed
-    // mode only): either the Armijo test of a line-search
+    // in LDS or global memory; fdir = sum_q w_q e_q): either the Armijo test of a line-search
     // trial, or one Newton iteration of solveTheRest / optScaled.  sT: NM x (NM+1) doubles of
     // LDS for the transposed factor.  Returns 0 to continue, 1 converged, 2 ladder exhausted.
     // H is not read when a line-search trial is rejected (it may then be null).
     template <bool EXACT>
     __device__ __forceinline__ int advance(const KParams &P, const double *H, double fdir,
-                                           const double *gs, double *sT, int lane)
+                                           double *sT, int lane)
     {
         constexpr int LD = NM + 1;
         const bool own = lane < NM;
@@ -1015,13 +1018,11 @@ struct Cell {
         ++iter;
         ++tot;
         const bool mx = mixed(P);
-        if (mx) {  // fobjMn from the FP64 sums (fobj.cpp:135-164)
-            g = own ? gs[lane] - uu : 0.0;
+        g = own ? D.row[0] / P.m0 - uu : 0.0;  // fobjMnUseH
+        if (mx)  // f from the FP64 sum, as the line search evaluates it (fobj.cpp:135-164)
             f = fdir;
-        } else {  // fobjMnUseH
-            g = own ? D.row[0] / P.m0 - uu : 0.0;
+        else
             f = rdl(D.row[0], 0) / (P.m0 * P.m0);
-        }
 #pragma unroll
         for (int i = 0; i < NM; ++i)
             f = f - rdl(a, i) * rdl(uu, i);
@@ -1092,10 +1093,17 @@ struct Cell {
         for (int i = 0; i < NM; ++i)
             da1 = da1 + fabs(rdl(d, i));
         gamma = det_exp(2.0 * da1 * P.minf);
-        if (err <= rtol && gamma <= tolGamma)
-            return 1;
-        if (err <= rtol && gamma > tolGamma)
-            ++igam;
+        if (mx) {
+            // the stopping test belongs to the FP64 path: hand the cell over once its
+            // FP32-accurate gradient is small, after taking this Newton step
+            if (err <= P.mixedSwitch)
+                flags |= C3D_FLAG_FP64_FINISH;
+        } else {
+            if (err <= rtol && gamma <= tolGamma)
+                return 1;
+            if (err <= rtol && gamma > tolGamma)
+                ++igam;
+        }
         aOld = a;  // linesearch.cpp:29-54
         maxF = 0.0;
 #pragma unroll
@@ -1123,8 +1131,7 @@ struct Cell {
 
 // ---------------------------------------------------------------- step pass
 template <int NM, bool EXACT>
-__global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, const int *__restrict__ W,
-                                                  const int *__restrict__ nWp, int *Wn, int *nWn,
+__global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, Lists Lc, Lists Ln,
                                                   double *u_g, double *alpha_g,
                                                   c3d_cell_stats *stats, int *status,
                                                   unsigned long long *sweepCount)
@@ -1132,7 +1139,8 @@ __global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, const int *
     constexpr int LD = NM + 1;
     __shared__ double sT[kWavesPerBlock][NM * LD];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int nW = uniform_int(*nWp);
+    const int nWm = uniform_int(*Lc.nm);
+    const int nW = nWm + uniform_int(*Lc.nf);
     if (blockIdx.x == 0 && threadIdx.x == 0 && sweepCount)
         atomicAdd(sweepCount, (unsigned long long)nW);
     const int nBlocksNeeded = (nW + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -1140,17 +1148,17 @@ __global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, const int *
         const int slot = blk * kWavesPerBlock + wv;
         if (slot >= nW)
             continue;
-        const int c = uniform_int(W[slot]);
+        const int c = uniform_int(list_cell(Lc, slot, nWm));
         Cell<NM> C;
         C.load(S, c, lane);
         const int done = C.template advance<EXACT>(P, S.H + (size_t)c * NM * NM, S.fdir[c],
-                                   S.gs + (size_t)c * NM, sT[wv], lane);
+                                                   sT[wv], lane);
         if (done) {
             C.finish(done, c, u_g, alpha_g, stats, status, lane);
         } else {
             C.store(S, c, lane);
             if (lane == 0)
-                Wn[atomicAdd(nWn, 1)] = c;
+                list_append(Ln, C.mixed(P), c);
         }
     }
 }
@@ -1199,9 +1207,7 @@ __device__ __forceinline__ double f_only(const KParams &P, const double *__restr
 template <int NM, bool REF, bool MIXED>
 __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restrict__ Mg,
                                                  const double *__restrict__ wg, State S,
-                                                 const int *__restrict__ W,
-                                                 const int *__restrict__ nWp, int *Wn, int *nWn,
-                                                 double *u_g, double *alpha_g,
+                                                 Lists Lc, Lists Ln, double *u_g, double *alpha_g,
                                                  c3d_cell_stats *stats, int *status, int passes,
                                                  unsigned long long *sweepCount)
 {
@@ -1209,23 +1215,22 @@ __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restr
     constexpr int NB = NT * (NT + 1) / 2;
     constexpr int NMS = NM | 1;
     static_assert(!(REF && MIXED), "one arithmetic mode");
-    static_assert(NM * NM + NM * (NM + 1) + NM <= kChunk * NMS,
-                  "H, factor and gradient alias the M chunk");
+    static_assert(NM * NM + NM * (NM + 1) <= kChunk * NMS, "H and factor alias the M chunk");
     __shared__ double sM[kChunk * NMS];
     __shared__ double sW[kChunk];
     __shared__ double sE[kChunk];
     __shared__ double sA[NM];
     const int lane = threadIdx.x;
-    const int nW = uniform_int(*nWp);
+    const int nWm = uniform_int(*Lc.nm);
+    const int nW = nWm + uniform_int(*Lc.nf);
     const int slot = blockIdx.x;
     if (slot >= nW)
         return;
-    const int c = uniform_int(W[slot]);
+    const int c = uniform_int(list_cell(Lc, slot, nWm));
     Cell<NM> C;
     C.load(S, c, lane);
     double *sH = sM;  // valid between a sweep and the next one
     double *sT = sM + NM * NM;
-    double *sG = sT + NM * (NM + 1);
     const int nch = P.nqPad / kChunk;
     int done = 0;
     int sweeps = 0;
@@ -1238,7 +1243,7 @@ __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restr
             const double ft = mx ? f_only_mixed<NM>(P, Mg, wg, sA, C.shift, lane)
                                  : f_only<NM, REF>(P, Mg, wg, sA, sW, sE, lane);
             if (!C.ls_accepts(ft)) {
-                done = C.template advance<REF>(P, nullptr, ft, nullptr, sT, lane);
+                done = C.template advance<REF>(P, nullptr, ft, sT, lane);
                 wave_lds_sync();
                 if (done)
                     break;
@@ -1268,17 +1273,14 @@ __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restr
             fdir = f;
         } else {
             d4 acc[1][NB];
-            f4 acc32[NB];
+            f4 acc32[1][NB];
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 acc[0][b] = d4{0.0, 0.0, 0.0, 0.0};
-                acc32[b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+                acc32[0][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
             }
             double fpart[1] = {0.0};
-            double gpart[NM];
-#pragma unroll
-            for (int i = 0; i < NM; ++i)
-                gpart[i] = 0.0;
+            const double shift[1] = {C.shift};
             double pre[NM];
             double prew = wg[lane];
 #pragma unroll
@@ -1302,16 +1304,15 @@ __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restr
                         pre[k] = Mg[base * NM + lane + kWave * k];
                 }
                 if (mx)
-                    chunk_mixed<NM>(sM, wl, sA, C.shift, acc32, fpart[0], gpart, lane);
+                    chunk_mixed<NM, 1>(sM, wl, sA, shift, acc32, fpart, lane);
                 else
                     chunk_mfma<NM, 1>(sM, wl, sA, acc, fpart, lane);
             }
             wave_lds_sync();
             if (mx) {
-                double gsl;
-                fdir = finish_mixed<NM>(acc32, fpart[0], gpart, C.shift, sH, gsl, lane);
-                if (lane < NM)
-                    sG[lane] = gsl;
+                const double es = exp(C.shift);
+                fdir = wave_sum(fpart[0]) * es;
+                store_H32<NM>(acc32[0], es, sH, lane);
             } else {
                 fdir = wave_sum(fpart[0]);
                 store_H<NM>(acc[0], sH, lane);
@@ -1319,7 +1320,7 @@ __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restr
         }
         wave_lds_sync();
         ++sweeps;
-        done = C.template advance<REF>(P, sH, fdir, sG, sT, lane);
+        done = C.template advance<REF>(P, sH, fdir, sT, lane);
         wave_lds_sync();
         if (done)
             break;
@@ -1331,7 +1332,7 @@ __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restr
     } else {
         C.store(S, c, lane);
         if (lane == 0)
-            Wn[atomicAdd(nWn, 1)] = c;
+            list_append(Ln, C.mixed(P), c);
     }
 }
 
@@ -1339,8 +1340,7 @@ __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restr
 // Features of the listed cells for the network, row = worklist slot: x_k = (u_k/u0 - xm_k)/
 // xs_k from the state's u0-scaled moments (k < NM; zero beyond and for slots >= *nWp).
 template <int NM>
-__global__ void k_warm_features(int64_t rows, State S, const int *__restrict__ W,
-                                const int *__restrict__ nWp, const float *__restrict__ norm,
+__global__ void k_warm_features(int64_t rows, State S, Lists L, const float *__restrict__ norm,
                                 float *X)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1349,8 +1349,9 @@ __global__ void k_warm_features(int64_t rows, State S, const int *__restrict__ W
     const int64_t slot = e / C3D_MLP_IN;
     const int k = (int)(e % C3D_MLP_IN);
     float x = 0.0f;
-    if (slot < *nWp && k < NM) {
-        const double uk = S.vec[(size_t)W[slot] * kVec * NM + kVu * NM + k];
+    const int nWm = *L.nm;
+    if (slot < nWm + *L.nf && k < NM) {
+        const double uk = S.vec[(size_t)list_cell(L, (int)slot, nWm) * kVec * NM + kVu * NM + k];
         x = ((float)uk - norm[k]) / norm[C3D_MLP_IN + k];
     }
     X[e] = x;
@@ -1363,20 +1364,19 @@ __global__ void k_warm_features(int64_t rows, State S, const int *__restrict__ W
 template <int NM>
 __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double *__restrict__ Mg,
                                                          const double *__restrict__ wg, State S,
-                                                         const int *__restrict__ W,
-                                                         const int *__restrict__ nWp,
-                                                         const double *__restrict__ anet)
+                                                         Lists L, const double *__restrict__ anet)
 {
     constexpr int NMS = NM | 1;
     __shared__ double sM[kChunk * NMS];
     __shared__ double sW[kChunk];
     __shared__ double sA[kWavesPerBlock][2][NM];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int nW = uniform_int(*nWp);
+    const int nWm = uniform_int(*L.nm);
+    const int nW = nWm + uniform_int(*L.nf);
     const int nBlocksNeeded = (nW + kWavesPerBlock - 1) / kWavesPerBlock;
     for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
         const int slot = blk * kWavesPerBlock + wv;
-        const int cell = slot < nW ? uniform_int(W[slot]) : -1;
+        const int cell = slot < nW ? uniform_int(list_cell(L, slot, nWm)) : -1;
         if (cell >= 0 && lane < NM) {
             sA[wv][0][lane] = S.vec[(size_t)cell * kVec * NM + kVa * NM + lane];
             sA[wv][1][lane] = anet[(size_t)slot * NM + lane];
@@ -1447,16 +1447,16 @@ __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double 
 
 // Cells still listed after the pass budget: report them and leave their input untouched.
 template <int NM>
-__global__ void k_budget(const int *__restrict__ W, const int *__restrict__ nWp,
-                         c3d_cell_stats *stats, int *status)
+__global__ void k_budget(Lists L, c3d_cell_stats *stats, int *status)
 {
-    const int nW = *nWp;
+    const int nWm = *L.nm;
+    const int nW = nWm + *L.nf;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nW; s += gridDim.x * blockDim.x) {
         if (s == 0)
             atomicOr(status, C3D_FLAG_BUDGET);
         if (stats) {
             c3d_cell_stats s0{0.0, 0.0, 0.0, 0, 0, 0, C3D_FLAG_BUDGET};
-            stats[W[s]] = s0;
+            stats[list_cell(L, s, nWm)] = s0;
         }
     }
 }
@@ -1551,14 +1551,12 @@ __global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nq
     const bool have = cell < ncells;
     if (lane < NM)
         sA[wv][lane] = have ? alpha_g[cell * NM + lane] : 0.0;
-    f4 acc[NB];
+    f4 acc[1][NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b)
-        acc[b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    double fpart = 0.0, gpart[NM];
-#pragma unroll
-    for (int i = 0; i < NM; ++i)
-        gpart[i] = 0.0;
+        acc[0][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    double fpart[1] = {0.0};
+    const double shift[1] = {0.0};
     for (int c = 0; c < nqPad / kChunk; ++c) {
         __syncthreads();
         for (int e = tid; e < kChunk * NM; e += kBlock) {
@@ -1569,12 +1567,11 @@ __global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nq
             sW[tid] = wg[(size_t)c * kChunk + tid];
         __syncthreads();
         if (have)
-            chunk_mixed<NM>(sM, sW[lane], sA[wv], 0.0, acc, fpart, gpart, lane);
+            chunk_mixed<NM, 1>(sM, sW[lane], sA[wv], shift, acc, fpart, lane);
     }
     if (!have)
         return;
-    double gsl;
-    finish_mixed<NM>(acc, fpart, gpart, 0.0, hs + cell * NM * NM, gsl, lane);
+    store_H32<NM>(acc[0], 1.0, hs + cell * NM * NM, lane);
 }
 
 // The dense part of one Newton iteration for a batch of given Hessians (one wave each):
@@ -1631,6 +1628,11 @@ __global__ __launch_bounds__(kBlock) void k_dense_solve(int64_t n, const double 
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
+__global__ void k_set_int2(int *p, int v)
+{
+    p[0] = v;
+    p[1] = v;
+}
 __global__ void k_zero_u64(unsigned long long *p) { p[threadIdx.x] = 0ull; }
 
 unsigned grid_cap(c3d_ctx_s *ctx, int perCU) { return (unsigned)(ctx->numCU * perCU); }
@@ -1656,8 +1658,8 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
                        c3d_cell_stats *stats, int maxPasses, hipStream_t s)
 {
     const size_t n = (size_t)P.ncells;
-    size_t bytes = n * (kVec * NM + kScal + (size_t)NM * NM + 1 + NM) * sizeof(double) +
-                   n * (kInt + 2) * sizeof(int) + 4 * sizeof(int);
+    size_t bytes = n * (kVec * NM + kScal + (size_t)NM * NM + 1) * sizeof(double) +
+                   n * (kInt + 4) * sizeof(int) + 8 * sizeof(int);
     hipError_t e = grow(ctx->closureState, bytes);
     if (e != hipSuccess)
         return e;
@@ -1667,14 +1669,19 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
     S.scal = S.vec + n * kVec * NM;
     S.H = S.scal + n * kScal;
     S.fdir = S.H + n * NM * NM;
-    S.gs = S.fdir + n;
-    S.ints = (int *)(S.gs + n * NM);
-    int *W0 = S.ints + n * kInt;
-    int *W1 = W0 + n;
-    int *counts = W1 + n;  // [0],[1] worklist sizes
+    S.ints = (int *)(S.fdir + n);
+    int *Wbase = S.ints + n * kInt;  // 4 lists of n: generation 0 (mixed, fp64), 1 (...)
+    int *counts = Wbase + 4 * n;     // [2g] mixed, [2g+1] fp64 of generation g
+    Lists Lg[2];
+    for (int g = 0; g < 2; ++g) {
+        Lg[g].Wm = Wbase + (2 * g) * n;
+        Lg[g].Wf = Wbase + (2 * g + 1) * n;
+        Lg[g].nm = counts + 2 * g;
+        Lg[g].nf = counts + 2 * g + 1;
+    }
     const bool ref = ctx->precision == C3D_PREC_FP64_REFORDER;
     const bool mixed = P.mixed != 0;
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, counts + 0, 0);
+    hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, s, counts, 0);
     ctx->evUsed = 0;
     ctx->evSpans.clear();
     ctx->lastPasses = 0;
@@ -1685,7 +1692,7 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
     if (ctx->profiling)
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(2), 0, s, ctx->d_sweepCount);
     const unsigned gInit = (unsigned)((n * 64 + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_init<NM>, dim3(gInit), dim3(kBlock), 0, s, P, u, alpha, S, W0, counts,
+    hipLaunchKernelGGL(k_init<NM>, dim3(gInit), dim3(kBlock), 0, s, P, u, alpha, S, Lg[0],
                        stats, ctx->d_status);
     if (ctx->warm && ctx->mlp.have) {
         const size_t xb = (n * C3D_MLP_IN * sizeof(float) + 255) / 256 * 256;
@@ -1700,13 +1707,13 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
         if (ev)
             (void)hipEventRecord(ev[0], s);
         hipLaunchKernelGGL(k_warm_features<NM>, dim3((unsigned)((n * C3D_MLP_IN + 255) / 256)),
-                           dim3(256), 0, s, (int64_t)n, S, W0, counts,
+                           dim3(256), 0, s, (int64_t)n, S, Lg[0],
                            (const float *)ctx->mlp.normDev.p, X);
         e = mlp_infer(ctx, (int64_t)n, X, anet, s);
         if (e != hipSuccess)
             return e;
         hipLaunchKernelGGL(k_warm_select<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P,
-                           ctx->d_M, ctx->d_wpad, S, W0, counts, anet);
+                           ctx->d_M, ctx->d_wpad, S, Lg[0], anet);
         if (ev)
             (void)hipEventRecord(ev[1], s);
     }
@@ -1714,9 +1721,8 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
     int64_t left = (int64_t)n;  // upper bound on the listed cells until the first read-back
     int pass = 0;
     while (pass < maxPasses) {
-        int *Wc = cur ? W1 : W0, *Wn = cur ? W0 : W1;
-        int *nc = counts + cur, *nn = counts + (1 - cur);
-        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, nn, 0);
+        const Lists &Lc = Lg[cur], &Ln = Lg[1 - cur];
+        hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, s, Ln.nm, 0);
         const bool tail = ctx->tailCells > 0 && left <= ctx->tailCells;
         if (tail) {
             const int K = std::min(ctx->tailPasses, maxPasses - pass);
@@ -1727,8 +1733,7 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
                 (void)hipEventRecord(ev[0], s);
 #define C3D_TAIL(R, X)                                                                     \
     hipLaunchKernelGGL((k_tail<NM, R, X>), dim3((unsigned)left), dim3(kWave), 0, s, P, ctx->d_M, \
-                       ctx->d_wpad, S, Wc, nc, Wn, nn, u, alpha, stats, ctx->d_status, K,      \
-                       cntTail)
+                       ctx->d_wpad, S, Lc, Ln, u, alpha, stats, ctx->d_status, K, cntTail)
             if (ref)
                 C3D_TAIL(true, false);
             else if (mixed)
@@ -1746,27 +1751,25 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
                 return hipErrorOutOfMemory;
             if (ev)
                 (void)hipEventRecord(ev[0], s);
+            if (mixed)
+                hipLaunchKernelGGL(k_sweep_mixed<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0,
+                                   s, P, ctx->d_M, ctx->d_wpad, S, Lc.Wm, Lc.nm);
             if (ref)
                 hipLaunchKernelGGL(k_sweep_ref<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s,
-                                   P, ctx->d_M, ctx->d_wpad, S, Wc, nc);
-            else if (mixed)
-                hipLaunchKernelGGL(k_sweep_mixed<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0,
-                                   s, P, ctx->d_M, ctx->d_wpad, S, Wc, nc);
+                                   P, ctx->d_M, ctx->d_wpad, S, Lc.Wf, Lc.nf);
             else
                 hipLaunchKernelGGL(k_sweep<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P,
-                                   ctx->d_M, ctx->d_wpad, S, Wc, nc);
+                                   ctx->d_M, ctx->d_wpad, S, Lc.Wf, Lc.nf);
             if (ev) {
                 (void)hipEventRecord(ev[1], s);
                 (void)hipEventRecord(ev[2], s);
             }
             if (ref)
                 hipLaunchKernelGGL((k_step<NM, true>), dim3(grid_cap(ctx, 16)), dim3(kBlock), 0,
-                                   s, P, S, Wc, nc, Wn, nn, u, alpha, stats, ctx->d_status,
-                                   cntBulk);
+                                   s, P, S, Lc, Ln, u, alpha, stats, ctx->d_status, cntBulk);
             else
                 hipLaunchKernelGGL((k_step<NM, false>), dim3(grid_cap(ctx, 16)), dim3(kBlock),
-                                   0, s, P, S, Wc, nc, Wn, nn, u, alpha, stats, ctx->d_status,
-                                   cntBulk);
+                                   0, s, P, S, Lc, Ln, u, alpha, stats, ctx->d_status, cntBulk);
             if (ev)
                 (void)hipEventRecord(ev[3], s);
             ++pass;
@@ -1774,19 +1777,18 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
         ctx->lastPasses = pass;
         cur = 1 - cur;
         if (tail || (pass & 7) == 0) {
-            int lft = 0;
-            e = hipMemcpyAsync(&lft, counts + cur, sizeof(int), hipMemcpyDeviceToHost, s);
+            int lft[2] = {0, 0};
+            e = hipMemcpyAsync(lft, counts + 2 * cur, sizeof(lft), hipMemcpyDeviceToHost, s);
             if (e == hipSuccess)
                 e = hipStreamSynchronize(s);
             if (e != hipSuccess)
                 return e;
-            if (lft == 0)
+            left = (int64_t)lft[0] + lft[1];
+            if (left == 0)
                 return hipGetLastError();
-            left = lft;
         }
     }
-    hipLaunchKernelGGL(k_budget<NM>, dim3(64), dim3(256), 0, s, cur ? W1 : W0, counts + cur,
-                       stats, ctx->d_status);
+    hipLaunchKernelGGL(k_budget<NM>, dim3(64), dim3(256), 0, s, Lg[cur], stats, ctx->d_status);
     return hipGetLastError();
 }
 
@@ -1811,6 +1813,7 @@ hipError_t launch_closure_fp64(c3d_ctx_s *ctx, int64_t ncells, double *u, double
     P.mixed = ctx->precision == C3D_PREC_MIXED;
     P.mixedMaxIter = ctx->mixedMaxIter;
     P.mixedCondMax = ctx->mixedCondMax;
+    P.mixedSwitch = ctx->mixedSwitch;
     if (ncells <= 0)
         return hipSuccess;
     if (ncells > INT32_MAX / 2)

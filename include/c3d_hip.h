@@ -46,15 +46,19 @@ extern "C" {
 #define C3D_FLAG_BUDGET 8      /* cell left unsolved by the pass budget */
 #define C3D_FLAG_FP64_FALLBACK 16 /* mixed mode: the cell was solved by the FP64 path from
                                      its input (the mixed iteration declined it) */
+#define C3D_FLAG_FP64_FINISH 64   /* mixed mode: the FP32-Hessian phase reached the hand-over
+                                     tolerance and FP64 iterations finished the cell */
 
 /* arithmetic modes of a context (c3d_set_precision) */
 #define C3D_PREC_FP64 0          /* FP64; quadrature sums on the matrix cores (default) */
 #define C3D_PREC_FP64_REFORDER 1 /* FP64 in the reference's operation order with a
                                     deterministic exp/log: bit-reproducible on a CPU */
-#define C3D_PREC_MIXED 2         /* closure: Hessian on the FP32 matrix cores under a
-                                    log-sum-exp shift; exponents, f, gradient, Cholesky,
-                                    condition estimate and stopping test in FP64; cells
-                                    that turn ill-conditioned restart in the FP64 path */
+#define C3D_PREC_MIXED 2         /* closure: a first phase with the Hessian (and the
+                                    gradient H[0][:]/m0) on the FP32 matrix cores under a
+                                    log-sum-exp shift, exponents, f, Cholesky and condition
+                                    estimate in FP64; each cell then finishes on the FP64
+                                    path, which makes the stopping test; cells that turn
+                                    ill-conditioned restart from their input on it */
 
 typedef struct c3d_ctx_s *c3d_ctx;
 
@@ -163,12 +167,14 @@ int c3d_closure(c3d_ctx ctx, int64_t ncells, double *u, double *alpha,
  * No counterpart in the reference (whose results depend on the host libm). */
 int c3d_set_precision(c3d_ctx ctx, int mode);
 
-/* Parameters of C3D_PREC_MIXED: a cell leaves the FP32-Hessian iteration and restarts from
- * its input in the FP64 path when its condition estimate exceeds cond_max (default 1e5),
- * its factorisation or line search fails, or it has taken max_iter iterations (default 40).
- * Such a cell then takes exactly the FP64 path's decisions (regularisation included) and is
- * flagged C3D_FLAG_FP64_FALLBACK.  No counterpart in the reference. */
-int c3d_set_mixed(c3d_ctx ctx, double cond_max, int max_iter);
+/* Parameters of C3D_PREC_MIXED.  A cell leaves the FP32-Hessian phase and restarts from its
+ * input on the FP64 path when its condition estimate exceeds cond_max (default 1e8), its
+ * factorisation or line search fails, or it has taken max_iter iterations (default 40);
+ * it then takes exactly the FP64 path's decisions (regularisation included) and is flagged
+ * C3D_FLAG_FP64_FALLBACK.  It continues from its iterate on the FP64 path once its gradient
+ * norm falls below switch_tol (default 1e-4; C3D_FLAG_FP64_FINISH).  No counterpart in the
+ * reference. */
+int c3d_set_mixed(c3d_ctx ctx, double cond_max, int max_iter, double switch_tol);
 
 /* Kernel-time accounting of c3d_closure (HIP events on the closure's stream around every
  * launch), for throughput and roofline reporting. */

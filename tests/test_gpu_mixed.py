@@ -1,13 +1,14 @@
-"""GPU checks of the mixed-precision closure (C3D_PREC_MIXED): the Hessian on the FP32
-matrix cores under a log-sum-exp shift, exponents / f / gradient / Cholesky / condition
-estimate / stopping test in FP64, and the hand-over of ill-conditioned cells to the FP64
-path from their input.
+"""GPU checks of the mixed-precision closure (C3D_PREC_MIXED): a first phase with the
+Hessian on the FP32 matrix cores under a log-sum-exp shift (exponents, f, Cholesky and
+condition estimate in FP64), the hand-over of each cell to the FP64 path once its gradient
+is small (the FP64 sweeps make the stopping test), and the restart of ill-conditioned
+cells from their input on the FP64 path.
 
 Bars, against the oracle (oracle/c3d_oracle.c, FP64 restatement of src/opt/*) and the
 product's own FP64 mode on identical inputs:
   * the mixed Hessian is within 1e-5 of the FP64 one (relative to max|H|);
-  * every cell the mixed iteration finishes meets the FP64 stopping test
-    (||grad h||_2 <= tol in FP64, u0-scaled) and reproduces its moments to 1.5e-8 u0;
+  * every converged cell meets the FP64 stopping test (||grad h||_2 <= tol in FP64,
+    u0-scaled) and reproduces its moments to 1.5e-8 u0;
   * a cell handed to the FP64 path ends bit-identical to the FP64 mode's result;
   * the regularisation rung agrees with the oracle's, and the moments written back agree
     with the oracle's to 1e-6 u0, on >= 99.5% of the cells (moments of cells whose rung
@@ -78,13 +79,14 @@ def test_closure_mixed_against_oracle_and_fp64(P, O, dev, N, Q, n):
     for k in ("totalIter", "iterGamma", "regIndex", "r", "err", "gamma"):
         assert np.array_equal(sm[k][fb], sf[k][fb]), k
     assert np.array_equal(sm["flags"][fb] & ~P.FLAG_FP64_FALLBACK, sf["flags"][fb])
-    # cells the mixed iteration finished: FP64 stopping test, moments, tolerance ball
+    # the other cells: FP64 stopping test, moments, tolerance ball
     mx = ~fb
-    assert (sm["regIndex"][mx] == 0).all() and (sm["err"][mx] <= 1e-8).all()
-    res = moment_residual(T, um[mx], am[mx])
+    conv = mx & ((sm["flags"] & P.FLAG_LADDER) == 0)
+    assert (sm["err"][conv] <= 1e-8).all()
+    res = moment_residual(T, um[conv], am[conv])
     assert res.max() <= 1.5e-8, res.max()
     same_rung = sm["regIndex"] == so["regIndex"]
-    ok = same_rung & (so["regIndex"] == 0) & mx
+    ok = same_rung & (so["regIndex"] == 0) & conv
     dist = np.linalg.norm(am - ao, axis=1)
     ball = 2.5 * 1e-8 * np.abs(u[:, 0]) / np.maximum(hess_lmin(T, ao), 1e-300)
     assert (dist[ok] <= ball[ok]).all(), (dist / ball)[ok].max()
@@ -93,8 +95,9 @@ def test_closure_mixed_against_oracle_and_fp64(P, O, dev, N, Q, n):
     frac_rung = same_rung.mean()
     frac_mom = (du <= 1e-6).mean()
     it_m, it_f = sm["totalIter"][mx].mean(), sf["totalIter"].mean()
-    print(f"fallback {fb.mean():.4f} rung-agree {frac_rung:.4f} moments<=1e-6 {frac_mom:.4f} "
-          f"iters mixed {it_m:.2f} fp64 {it_f:.2f}")
+    fin = ((sm["flags"] & P.FLAG_FP64_FINISH) != 0).mean()
+    print(f"fallback {fb.mean():.4f} finish {fin:.4f} rung-agree {frac_rung:.4f} "
+          f"moments<=1e-6 {frac_mom:.4f} iters mixed {it_m:.2f} fp64 {it_f:.2f}")
     assert frac_rung >= 0.995 and frac_mom >= 0.995
     assert rc_m == rc_f
 
