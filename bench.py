@@ -15,9 +15,9 @@ timed) on --train-pairs (u, alpha) pairs that the FP64 closure solves from a dif
 The plain FP64 path is always timed beside the chosen one (same inputs, same process), and
 the chosen path's outputs are compared with its outputs cell by cell.
 
-One JSON line: value = cells solved per second by the chosen path; roofline of the FP64
-sweep kernel (HIP events on the closure's stream, cell-sweeps counted on the device);
-the CPU oracle timed on a bounded sample of the same inputs.
+One JSON line: value = cells solved per second by the chosen path; roofline of the path's
+matrix-core sweep kernel (HIP events on the closure's stream, cell-sweeps counted on the
+device); the CPU oracle timed on a bounded sample of the same inputs.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--path mixed+nn] [--cpu-sample 65536]
 """
@@ -34,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = ("M_N cell-updates/s at ‖∇h‖≤1e-8: batched mixed+NN vs plain FP64, "
           "one GPU")
 FP64_MFMA_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+FP32_MFMA_PEAK = 157.3  # TFLOP/s, MI355X dense FP32 matrix
 
 
 def front_end():
@@ -79,6 +80,10 @@ def main():
     ap.add_argument("--dual-steps", type=int, default=3000, help="dual-gap steps")
     ap.add_argument("--train-batch", type=int, default=8192)
     ap.add_argument("--train-lr", type=float, default=3e-3)
+    ap.add_argument("--workload", choices=("closure", "step"), default="closure",
+                    help="closure: the metric's batch (default); step: M_2 SSP-RK2 time "
+                         "stepping on a 128^3 grid (kinetic flux roofline)")
+    ap.add_argument("--grid", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the closure is "
@@ -89,6 +94,8 @@ def main():
     import torch
     P = front_end()
     dev = torch.device("cuda:0")
+    if args.workload == "step":
+        return step_workload(args, P, torch, dev)
     N, Q = 3, 24
     n = args.cells
     ctx = P.Ctx(N, Q)
@@ -172,20 +179,33 @@ def main():
     agree = {"moments_within_1e-6": round(float((du <= 1e-6).mean()), 6),
              "rung_agreement": round(float((main_r["reg"] == base_r["reg"]).mean()), 6)}
 
-    # roofline of the FP64 sweep kernel: H = M^T diag(w e) M, 2 nq nm^2 flops per cell-sweep
-    bp = base_r["prof"]
+    # roofline of the path's matrix-core sweep kernel: H = M^T diag(w e) M, 2 nq nm^2 flops
+    # per cell-sweep; the mixed phase runs it on the FP32 matrix cores, the FP64 path (and
+    # the FP64 finish of the mixed cells) on the FP64 ones
     flops_per_sweep = 2.0 * nq * nm * nm
-    achieved = flops_per_sweep * bp["cell_sweeps"] / (bp["sweep_ms"] * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK,
-                "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK, 4),
-                "traffic": None, "kernel": "k_sweep<16> (FP64 MFMA Hessian sweep, fp64 path)",
-                "flops_per_cell_sweep": flops_per_sweep,
-                "cell_sweeps_per_step": int(bp["cell_sweeps"]),
-                "sweep_kernel_ms_per_step": round(bp["sweep_ms"], 3)}
+
+    def sweep_roofline(prof, mixed_kernel):
+        cs = prof["mixed_cell_sweeps"] if mixed_kernel else prof["cell_sweeps"]
+        ms = prof["mixed_sweep_ms"] if mixed_kernel else prof["sweep_ms"]
+        peak = FP32_MFMA_PEAK if mixed_kernel else FP64_MFMA_PEAK
+        ach = flops_per_sweep * cs / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+        return {"bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 4), "traffic": None,
+                "kernel": ("k_sweep_mixed<16> (FP32 MFMA Hessian, mixed phase)" if mixed_kernel
+                           else "k_sweep<16> (FP64 MFMA Hessian)"),
+                "flops_per_cell_sweep": flops_per_sweep, "cell_sweeps_per_step": int(cs),
+                "kernel_ms_per_step": round(ms, 3)}
+
+    mixed_main = args.path.startswith("mixed")
+    roofline = sweep_roofline(main_r["prof"], mixed_main)
+    if mixed_main:
+        roofline["fp64_sweep_in_path"] = sweep_roofline(main_r["prof"], False)
+    roofline["fp64_path_sweep"] = sweep_roofline(base_r["prof"], False)
 
     def kernel_ms(r):
         p = r["prof"]
-        out = {k: round(p[k], 3) for k in ("sweep_ms", "step_ms", "tail_ms", "warm_ms")}
+        out = {k: round(p[k], 3) for k in ("mixed_sweep_ms", "sweep_ms", "step_ms", "tail_ms",
+                                            "warm_ms")}
         out.update(passes=int(p["passes"]), tail_launches=int(p["tail_launches"]))
         return out
 
@@ -228,6 +248,100 @@ def main():
                    "agreement_with_fp64": agree,
                    "warm_start_training": train},
         "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def step_workload(args, P, torch, dev):
+    """M_2 (9 moments), Q = 8 (128 ordinates), periodic n^3 grid on [-1,1]^3, pulse
+    rho = max(exp(-|x|^2/(2 0.05^2)), 1e-4) (1 + 0.01 U[-1,1]), sigma_s = 1, sigma_a = 0,
+    minmod slopes (theta = 1), CFL 0.5 (dt of init.cpp:274), SSP-RK2 with the closure on
+    every cell (the reference's default batched semantics, tol 1e-8), FP64.  A step is one
+    full update() (update.cpp:185-275) on the device; the state keeps evolving across
+    steps.  Also timed: the kinetic flux alone (ansatz + limited upwind stencil), against
+    its algorithmic HBM bytes with the ansatz materialised."""
+    N, Q, n = 2, 8, args.grid
+    ctx = P.Ctx(N, Q)
+    nm, nq = ctx.nm, ctx.nq
+    ng = (n + 4) ** 3
+    ni = n ** 3
+    d = 2.0 / n
+    u = torch.empty((ng, nm), dtype=torch.float64, device=dev)
+    a = torch.empty_like(u)
+    sT = torch.empty(ng, dtype=torch.float64, device=dev)
+    sS = torch.empty_like(sT)
+    ctx.init_grid(n, n, n, u, sT, sS, alpha=a, ic=1, sigma=0.05, floor=1e-4, noise=0.01,
+                  seed=args.seed, cross=1.0)
+    theta = 1.0
+    dt = 0.5 * (2.0 / (2.0 + theta)) / (3.0 / d)
+    opts = P.closure_opts(maxIter=100, tol=1e-8, tolGamma=1.1, condHMax=1e10)
+    stats = P.alloc_stats(2 * ng, dev)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        ctx.update(n, n, n, u, a, sT, sS, d, d, d, theta, dt, 1, 0, opts, stats=stats)
+
+    for _ in range(args.warmup):
+        step()
+    rc = ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    rc = rc or ctx.sync()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    st = P.stats_to_numpy(stats)
+    # the kinetic flux alone, HIP events on the stream it is launched on
+    flux = torch.zeros_like(u)
+    reps = 5
+    ctx.flux_kinetic(n, n, n, a, flux, d, d, d, theta, stream=stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        ctx.flux_kinetic(n, n, n, a, flux, d, d, d, theta, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    flux_ms = e0.elapsed_time(e1) / reps
+    flux_bytes = 8.0 * (ng * nm + 2.0 * ng * nq + ni * nm)
+    gbs = flux_bytes / (flux_ms * 1e-3) / 1e9
+    cpu = None
+    if args.cpu_sample > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        T = O.Tables(N, Q)
+        m = 32
+        dm = 2.0 / m
+        uc, sTc, sSc = O.init_grid(m, m, m, nm, ic=1, sigma=0.05, floor=1e-4, noise=0.01,
+                                   seed=args.seed)
+        ac = O.init_alpha(uc)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        ngc = uc.shape[0]
+        dtc = 0.5 * (2.0 / 3.0) / (3.0 / dm)
+        t0 = time.perf_counter()
+        O.lib().orc_update(m, m, m, nm, T.nm2, nq, 1, 0, uc, ac, np.zeros_like(uc),
+                           np.zeros_like(uc), np.zeros(ngc * nq), sTc, sSc, O.C.byref(T.c),
+                           T.ops, O.C.byref(O.opts()), dm, dm, dm, theta, dtc, None, threads)
+        secs = time.perf_counter() - t0
+        cpu = {"value": round(m ** 3 / secs, 1), "unit": "cell-updates/s", "cores": threads,
+               "kind": "port", "sample": f"one SSP-RK2 step of the oracle on the same "
+                                         f"problem at 32^3, {secs:.1f} s"}
+    out = {
+        "metric": "M_2 SSP-RK2 interior cell-updates/s (time stepping, one GPU)",
+        "value": round(ni / (ms / 1e3), 1), "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"M_2 time stepping, {n}^3 periodic grid, Q=8 (128 ordinates), "
+                               "pulse sigma 0.05 + 1% noise, theta 1, CFL 0.5, closure tol 1e-8",
+                   "grid": n, "moment_order": N, "quad_order": Q, "dt": dt, "status": rc,
+                   "closure_iters_mean": round(float(st["totalIter"].mean()), 3),
+                   "closure_iters_max": int(st["totalIter"].max())},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(gbs / 8000.0, 4), "traffic": None,
+                     "kernel": "kinetic flux (k_ansatz + k_stencil), ansatz materialised",
+                     "bytes_per_launch": flux_bytes, "ms_per_launch": round(flux_ms, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -23,9 +23,9 @@ _d = C.c_double
 
 
 class ClosureProfile(C.Structure):
-    _fields_ = [("cell_sweeps", _i64), ("tail_cell_sweeps", _i64), ("passes", _i32),
-                ("tail_launches", _i32), ("sweep_ms", _d), ("step_ms", _d), ("tail_ms", _d),
-                ("warm_ms", _d)]
+    _fields_ = [("cell_sweeps", _i64), ("tail_cell_sweeps", _i64), ("mixed_cell_sweeps", _i64),
+                ("passes", _i32), ("tail_launches", _i32), ("sweep_ms", _d),
+                ("mixed_sweep_ms", _d), ("step_ms", _d), ("tail_ms", _d), ("warm_ms", _d)]
 
 
 MLP_IN, MLP_HID, MLP_NPARAM = 25, 128, 23065
@@ -91,6 +91,8 @@ SIGNATURES = [
     ("c3d_update", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _d, _d, _d,
                              _d, _d, C.c_int, C.c_int, C.POINTER(ClosureOpts), _vp, _vp]),
     ("c3d_make_closure_batch", C.c_int, [_vp, _i64, C.c_uint64, _d, _d, _vp, _vp, _vp, _vp]),
+    ("c3d_init_grid", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_int, _d, _d, _d,
+                                C.c_uint64, _d, _vp, _vp, _vp, _vp, _vp]),
     ("c3d_mlp_train", C.c_int, [_vp, _i64, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float,
                                 C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_float), _vp]),
     ("c3d_mlp_dual_loss_grad", C.c_int, [_vp, _i64, _vp, _vp, _vp, C.POINTER(MlpNorm), _vp, _vp,
@@ -256,6 +258,18 @@ class Ctx:
     def make_closure_batch(self, n, seed, smin, smax, u, alpha0, alphaStar=None, stream=None):
         return check("c3d_make_closure_batch", self.L.c3d_make_closure_batch(
             self.h, n, seed, smin, smax, _p(u), _p(alpha0), _p(alphaStar), _stream(stream)))
+
+    def init_grid(self, nx, ny, nz, u, sigmaT, sigmaS, alpha=None, a=(-1.0, -1.0, -1.0),
+                  d=None, ic=1, sigma=0.1, floor=1e-4, noise=0.0, seed=0, cross=1.0,
+                  stream=None):
+        import numpy as np
+        if d is None:
+            d = ((1.0 - a[0]) / nx, (1.0 - a[1]) / ny, (1.0 - a[2]) / nz)
+        av = np.ascontiguousarray(a, dtype=np.float64)
+        dv = np.ascontiguousarray(d, dtype=np.float64)
+        return check("c3d_init_grid", self.L.c3d_init_grid(
+            self.h, nx, ny, nz, av.ctypes.data, dv.ctypes.data, ic, sigma, floor, noise, seed,
+            cross, _p(u), _p(alpha), _p(sigmaT), _p(sigmaS), _stream(stream)))
 
     # ---- learned warm start
     def mlp_train(self, u, alpha, mse_steps, dual_steps=0, batch=4096, lr=2e-3, seed=1,

@@ -139,6 +139,10 @@ hipError_t launch_euler(int nx, int ny, int nz, int nm, double *u, const double 
                         const double *sigmaT, const double *sigmaS, double dt, hipStream_t s);
 hipError_t launch_rk2(int nx, int ny, int nz, int nm, double *u, double *uOld, int kind,
                       hipStream_t s);
+hipError_t launch_init_grid(int nx, int ny, int nz, int nm, const double *a, const double *d,
+                            int ic, double sigma, double floorv, double noise, uint64_t seed,
+                            double cross, double *u, double *alpha, double *sigmaT,
+                            double *sigmaS, hipStream_t s);
 hipError_t launch_make_batch(c3d_ctx_s *ctx, int64_t n, uint64_t seed, double smin,
                              double smax, double *u, double *alpha0, double *alphaStar,
                              hipStream_t s);

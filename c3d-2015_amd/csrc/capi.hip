@@ -76,7 +76,7 @@ int finish_ctx(c3d_ctx_s *ctx)
     UP(d_pnOps, t.pnOps);
 #undef UP
     if (e == hipSuccess)
-        e = hipMalloc((void **)&ctx->d_sweepCount, 2 * sizeof(unsigned long long));
+        e = hipMalloc((void **)&ctx->d_sweepCount, 3 * sizeof(unsigned long long));
     if (e == hipSuccess)
         e = hipMalloc((void **)&ctx->d_status, sizeof(int));
     if (e == hipSuccess)
@@ -274,20 +274,23 @@ int c3d_get_closure_profile(c3d_ctx ctx, c3d_closure_profile *out)
         return C3D_OK;
     if (hipStreamSynchronize(ctx->lastStream) != hipSuccess)
         return C3D_ERR_HIP;
-    unsigned long long n[2] = {0, 0};
+    unsigned long long n[3] = {0, 0, 0};
     if (hipMemcpy(n, ctx->d_sweepCount, sizeof(n), hipMemcpyDeviceToHost) != hipSuccess)
         return C3D_ERR_HIP;
     out->cell_sweeps = (int64_t)n[0];
     out->tail_cell_sweeps = (int64_t)n[1];
+    out->mixed_cell_sweeps = (int64_t)n[2];
     for (const auto &sp : ctx->evSpans) {
         const int k = sp.second;
-        float a = 0.f, b = 0.f;
+        float a = 0.f, b = 0.f, c = 0.f;
         if (sp.first == 0) {
             if (hipEventElapsedTime(&a, ctx->evPool[k], ctx->evPool[k + 1]) != hipSuccess ||
-                hipEventElapsedTime(&b, ctx->evPool[k + 2], ctx->evPool[k + 3]) != hipSuccess)
+                hipEventElapsedTime(&b, ctx->evPool[k + 1], ctx->evPool[k + 2]) != hipSuccess ||
+                hipEventElapsedTime(&c, ctx->evPool[k + 2], ctx->evPool[k + 3]) != hipSuccess)
                 return C3D_ERR_HIP;
-            out->sweep_ms += a;
-            out->step_ms += b;
+            out->mixed_sweep_ms += a;
+            out->sweep_ms += b;
+            out->step_ms += c;
         } else {
             if (hipEventElapsedTime(&a, ctx->evPool[k], ctx->evPool[k + 1]) != hipSuccess)
                 return C3D_ERR_HIP;
@@ -435,6 +438,19 @@ int c3d_mlp_dual_loss_grad(c3d_ctx ctx, int64_t n, const float *params, const fl
     return hip_status(e);
 }
 
+int c3d_init_grid(c3d_ctx ctx, int nx, int ny, int nz, const double *a, const double *d, int ic,
+                  double sigma, double floor_value, double noise, uint64_t seed,
+                  double cross_section, double *u, double *alpha, double *sigmaT,
+                  double *sigmaS, void *stream)
+{
+    if (!ctx || nx < 1 || ny < 1 || nz < 1 || !a || !d || ic < 0 || ic > 2 || !u || !sigmaT ||
+        !sigmaS || (ic != 2 && !(sigma > 0.0)))
+        return C3D_ERR_ARG;
+    return hip_status(launch_init_grid(nx, ny, nz, ctx->t.nm, a, d, ic, sigma, floor_value,
+                                       noise, seed, cross_section, u, alpha, sigmaT, sigmaS,
+                                       (hipStream_t)stream));
+}
+
 int c3d_set_pass_budget(c3d_ctx ctx, int max_passes)
 {
     if (!ctx || max_passes < 1)
@@ -565,7 +581,10 @@ int c3d_update(c3d_ctx ctx, int nx, int ny, int nz, double *u, double *alpha,
         if (e != hipSuccess)
             break;
         if (solver == 1) {
-            e = launch_closure_fp64(ctx, (int64_t)ng, u, alpha, *opts,
+            c3d_closure_opts oc = *opts;
+            if (oc.unbatched)
+                oc.tolGamma = 1.1;  // solveOptimization passes 1.1 to opt(), update.cpp:57
+            e = launch_closure_fp64(ctx, (int64_t)ng, u, alpha, oc,
                                     stats ? stats + st * ng : nullptr, s);
             if (e == hipSuccess)
                 e = launch_ansatz(ctx, nx, ny, nz, alpha, A, 0, s);

@@ -93,6 +93,56 @@ __device__ __forceinline__ void list_append(const Lists &L, bool mixedPhase, int
         L.Wf[atomicAdd(L.nf, 1)] = c;
 }
 
+// Appends of a block to the next pass's lists, gathered in LDS and flushed with one global
+// atomic per list and batch (a global atomic per cell on one counter serialises at the L2).
+// push: one lane per wave, at most kWavesPerBlock pushes between two flush() calls; flush:
+// called by the whole block at a block-uniform point.
+constexpr int kAggCap = 256;
+struct BlockAppender {
+    int *buf;   // LDS [2][kAggCap]
+    int *cnt;   // LDS [2]
+    int *base;  // LDS [2]
+    __device__ __forceinline__ void init()
+    {
+        if (threadIdx.x < 2)
+            cnt[threadIdx.x] = 0;
+        __syncthreads();
+    }
+    __device__ __forceinline__ void push(bool mixedPhase, int c)
+    {
+        const int l = mixedPhase ? 0 : 1;
+        buf[l * kAggCap + atomicAdd(&cnt[l], 1)] = c;
+    }
+    // after a barrier: flush if forced or if another round of pushes might overflow
+    __device__ __forceinline__ void flush(const Lists &L, bool force, int perRound)
+    {
+        __syncthreads();
+        const int c0 = cnt[0], c1 = cnt[1];
+        if (!force && c0 <= kAggCap - perRound && c1 <= kAggCap - perRound)
+            return;
+        if (threadIdx.x == 0) {
+            base[0] = c0 ? atomicAdd(L.nm, c0) : 0;
+            base[1] = c1 ? atomicAdd(L.nf, c1) : 0;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < c0; t += blockDim.x)
+            L.Wm[base[0] + t] = buf[t];
+        for (int t = threadIdx.x; t < c1; t += blockDim.x)
+            L.Wf[base[1] + t] = buf[kAggCap + t];
+        __syncthreads();
+        if (threadIdx.x < 2)
+            cnt[threadIdx.x] = 0;
+        __syncthreads();
+    }
+};
+
+#define C3D_BLOCK_APPENDER(name)                                                          \
+    __shared__ int name##_buf[2 * kAggCap];                                                \
+    __shared__ int name##_cnt[2];                                                          \
+    __shared__ int name##_base[2];                                                         \
+    BlockAppender name{name##_buf, name##_cnt, name##_base};                               \
+    name.init()
+
 __device__ __forceinline__ double rdl(double v, int src)
 {
     int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
@@ -163,14 +213,22 @@ struct LaneVec {
 };
 
 // Wave-resident factor: lane i < NM holds row i (L after Cholesky; L(i,i) in row[i]) and
-// col[j] = L(j,i) for j > i, for the transposed solves.  EXACT divides by the diagonal as
+// the wave's LDS copy of the factor (sT[i ld + j] = L(i,j), j <= i) serves the transposed
+// solves.  EXACT divides by the diagonal as
 // LAPACK/BLAS do; otherwise the triangular solves multiply by reciprocals of the diagonal
 // computed once per factor (dinv, lane i: 1/L(i,i)), which shortens their critical path.
 template <int NM, bool EXACT = true>
 struct Dense {
     double row[NM];
-    double col[NM];
+    const double *sT = nullptr;  // LDS copy of L, row-major with leading dimension ld
+    int ld = 0;
     double dinv;
+
+    // L(j, lane) for j > lane (the column of L below the diagonal, read from LDS)
+    __device__ __forceinline__ double col(int j, int lane) const
+    {
+        return (j > lane && lane < NM) ? sT[j * ld + lane] : 0.0;
+    }
 
     __device__ __forceinline__ double diag(int j) const { return rdl(row[j], j); }
 
@@ -202,8 +260,10 @@ struct Dense {
         return 0;
     }
 
-    __device__ __forceinline__ void build_columns(double *sT, int ld, int lane)
+    __device__ __forceinline__ void build_columns(double *sTw, int ldw, int lane)
     {
+        sT = sTw;
+        ld = ldw;
         if constexpr (!EXACT) {
             double djj = 1.0;
 #pragma unroll
@@ -211,21 +271,12 @@ struct Dense {
                 djj = (lane == j) ? row[j] : djj;
             dinv = 1.0 / djj;
         }
+        wave_lds_sync();
         if (lane < NM) {
 #pragma unroll
             for (int j = 0; j < NM; ++j)
                 if (j <= lane)
-                    sT[lane * ld + j] = row[j];
-        }
-        wave_lds_sync();
-        if (lane < NM) {
-#pragma unroll
-            for (int i = 0; i < NM; ++i)
-                col[i] = (i > lane) ? sT[i * ld + lane] : 0.0;
-        } else {
-#pragma unroll
-            for (int i = 0; i < NM; ++i)
-                col[i] = 0.0;
+                    sTw[lane * ldw + j] = row[j];
         }
         wave_lds_sync();
     }
@@ -255,7 +306,7 @@ struct Dense {
                 x = EXACT ? x / row[j] : x * dinv;
             double xj = rdl(x, j);
             if (lane < j)
-                x = x - col[j] * xj;
+                x = x - col(j, lane) * xj;
         }
     }
     __device__ __forceinline__ void potrs_custom(double &d, int lane)  // utils.cpp:71-86
@@ -274,7 +325,7 @@ struct Dense {
                 d = EXACT ? d / row[j] : d * dinv;
             double dj = rdl(d, j);
             if (lane < j)
-                d = d - col[j] * dj;
+                d = d - col(j, lane) * dj;
         }
     }
     // LAPACK dpotrs('L'): dtrsm(L) then dtrsm(L^T), reference BLAS order (opt.cpp:285)
@@ -296,7 +347,7 @@ struct Dense {
             double temp = rdl(d, i);
 #pragma unroll
             for (int k = i + 1; k < NM; ++k)
-                temp = temp - rdl(col[k], i) * rdl(d, k);
+                temp = temp - sT[k * ld + i] * rdl(d, k);
             temp = EXACT ? temp / diag(i) : temp * rdl(dinv, i);
             if (lane == i)
                 d = temp;
@@ -375,14 +426,11 @@ struct Dense {
 // ---------------------------------------------------------------- init pass
 // preProcess (optBatch.cpp:312-331 / opt.cpp:336-346) and the first worklist.
 template <int NM>
-__global__ __launch_bounds__(kBlock) void k_init(KParams P, const double *__restrict__ u_g,
-                                                  const double *__restrict__ alpha_g, State S,
-                                                  Lists L, c3d_cell_stats *stats, int *status)
+__device__ __forceinline__ void init_cell(const KParams &P, const double *__restrict__ u_g,
+                                          const double *__restrict__ alpha_g, const State &S,
+                                          int64_t c, c3d_cell_stats *stats, int *status,
+                                          BlockAppender &app, int lane)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (c >= P.ncells)
-        return;
     const bool own = lane < NM;
     double uu = own ? u_g[c * NM + lane] : 0.0;
     double a = own ? alpha_g[c * NM + lane] : 0.0;
@@ -434,9 +482,26 @@ __global__ __launch_bounds__(kBlock) void k_init(KParams P, const double *__rest
         in[kIflags] = 0;
         in[kIphase] = kPhaseH;
         in[kImaxIt] = P.maxIter;
-        list_append(L, P.mixed != 0, (int)c);
+        app.push(P.mixed != 0, (int)c);
     }
 }
+
+template <int NM>
+__global__ __launch_bounds__(kBlock) void k_init(KParams P, const double *__restrict__ u_g,
+                                                  const double *__restrict__ alpha_g, State S,
+                                                  Lists L, c3d_cell_stats *stats, int *status)
+{
+    C3D_BLOCK_APPENDER(app);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t nBlocksNeeded = (P.ncells + kWavesPerBlock - 1) / kWavesPerBlock;
+    for (int64_t blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
+        const int64_t c = blk * kWavesPerBlock + wv;
+        if (c < P.ncells)
+            init_cell<NM>(P, u_g, alpha_g, S, c, stats, status, app, lane);
+        app.flush(L, blk + gridDim.x >= nBlocksNeeded, kWavesPerBlock);
+    }
+}
+
 
 // ---------------------------------------------------------------- sweep building blocks
 // Matrix-core accumulation of one LDS chunk of kChunk ordinates for CW cells: lane l owns
@@ -990,7 +1055,7 @@ struct Cell {
     {
         constexpr int LD = NM + 1;
         const bool own = lane < NM;
-        const double tolGamma = P.unbatched ? 1.1 : P.tolGamma;
+        const double tolGamma = P.tolGamma;  // opt.cpp:149 / optBatch.cpp:268
         if (phase == kPhaseLS) {
             if (ls_accepts(fdir)) {
                 tprev = tls;  // accepted; H of this sweep belongs to the new alpha
@@ -1141,25 +1206,29 @@ __global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, Lists Lc, L
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nWm = uniform_int(*Lc.nm);
     const int nW = nWm + uniform_int(*Lc.nf);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && sweepCount)
-        atomicAdd(sweepCount, (unsigned long long)nW);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && sweepCount) {  // [0] FP64 sweeps, [2] mixed
+        atomicAdd(sweepCount, (unsigned long long)(nW - nWm));
+        atomicAdd(sweepCount + 2, (unsigned long long)nWm);
+    }
+    C3D_BLOCK_APPENDER(app);
     const int nBlocksNeeded = (nW + kWavesPerBlock - 1) / kWavesPerBlock;
     for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
         const int slot = blk * kWavesPerBlock + wv;
-        if (slot >= nW)
-            continue;
-        const int c = uniform_int(list_cell(Lc, slot, nWm));
-        Cell<NM> C;
-        C.load(S, c, lane);
-        const int done = C.template advance<EXACT>(P, S.H + (size_t)c * NM * NM, S.fdir[c],
-                                                   sT[wv], lane);
-        if (done) {
-            C.finish(done, c, u_g, alpha_g, stats, status, lane);
-        } else {
-            C.store(S, c, lane);
-            if (lane == 0)
-                list_append(Ln, C.mixed(P), c);
+        if (slot < nW) {
+            const int c = uniform_int(list_cell(Lc, slot, nWm));
+            Cell<NM> C;
+            C.load(S, c, lane);
+            const int done = C.template advance<EXACT>(P, S.H + (size_t)c * NM * NM, S.fdir[c],
+                                                       sT[wv], lane);
+            if (done) {
+                C.finish(done, c, u_g, alpha_g, stats, status, lane);
+            } else {
+                C.store(S, c, lane);
+                if (lane == 0)
+                    app.push(C.mixed(P), c);
+            }
         }
+        app.flush(Ln, blk + gridDim.x >= nBlocksNeeded, kWavesPerBlock);
     }
 }
 
@@ -1358,18 +1427,19 @@ __global__ void k_warm_features(int64_t rows, State S, Lists L, const float *__r
 }
 
 // The gradient-decrease test: for each listed cell, ||grad h||_2 at the caller's alpha and at
-// the network's proposal anet[slot] (FP64 sums, one wave per cell); the proposal becomes the
-// starting point (alpha, alpha0 and the line-search anchor) only if its gradient is smaller.
-// Either way the mixed mode's first shift is log sum_q w e at the chosen point.
+// the network's proposal anet[slot] (one wave per cell, sums in FP32: the test only ranks
+// two candidates, and a proposal whose exponentials overflow loses it); the proposal becomes
+// the starting point (alpha, alpha0 and the line-search anchor) only if its gradient is
+// smaller.  Either way the mixed mode's first shift is log sum_q w e at the chosen point.
 template <int NM>
 __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double *__restrict__ Mg,
                                                          const double *__restrict__ wg, State S,
                                                          Lists L, const double *__restrict__ anet)
 {
     constexpr int NMS = NM | 1;
-    __shared__ double sM[kChunk * NMS];
-    __shared__ double sW[kChunk];
-    __shared__ double sA[kWavesPerBlock][2][NM];
+    __shared__ float sM[kChunk * NMS];
+    __shared__ float sW[kChunk];
+    __shared__ float sA[kWavesPerBlock][2][NM];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int nWm = uniform_int(*L.nm);
     const int nW = nWm + uniform_int(*L.nf);
@@ -1378,57 +1448,65 @@ __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double 
         const int slot = blk * kWavesPerBlock + wv;
         const int cell = slot < nW ? uniform_int(list_cell(L, slot, nWm)) : -1;
         if (cell >= 0 && lane < NM) {
-            sA[wv][0][lane] = S.vec[(size_t)cell * kVec * NM + kVa * NM + lane];
-            sA[wv][1][lane] = anet[(size_t)slot * NM + lane];
+            sA[wv][0][lane] = (float)S.vec[(size_t)cell * kVec * NM + kVa * NM + lane];
+            sA[wv][1][lane] = (float)anet[(size_t)slot * NM + lane];
         }
-        double f1 = 0.0, f2 = 0.0, g1[NM], g2[NM];
+        float f1 = 0.0f, f2 = 0.0f, g1[NM], g2[NM];
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            g1[i] = 0.0;
-            g2[i] = 0.0;
+            g1[i] = 0.0f;
+            g2[i] = 0.0f;
         }
         for (int ch = 0; ch < P.nqPad / kChunk; ++ch) {
             __syncthreads();
             for (int e = tid; e < kChunk * NM; e += kBlock) {
                 const int rr = e / NM, cc = e - rr * NM;
-                sM[rr * NMS + cc] = Mg[(size_t)ch * kChunk * NM + e];
+                sM[rr * NMS + cc] = (float)Mg[(size_t)ch * kChunk * NM + e];
             }
             if (tid < kChunk)
-                sW[tid] = wg[(size_t)ch * kChunk + tid];
+                sW[tid] = (float)wg[(size_t)ch * kChunk + tid];
             __syncthreads();
             if (cell >= 0) {
-                const double *rowp = &sM[lane * NMS];
-                double x1 = 0.0, x2 = 0.0;
+                const float *rowp = &sM[lane * NMS];
+                float x1 = 0.0f, x2 = 0.0f;
 #pragma unroll
                 for (int i = 0; i < NM; ++i) {
-                    x1 = fma(sA[wv][0][i], rowp[i], x1);
-                    x2 = fma(sA[wv][1][i], rowp[i], x2);
+                    x1 = fmaf(sA[wv][0][i], rowp[i], x1);
+                    x2 = fmaf(sA[wv][1][i], rowp[i], x2);
                 }
-                const double e1 = sW[lane] * exp(x1), e2 = sW[lane] * exp(x2);
-                f1 = f1 + e1;
-                f2 = f2 + e2;
+                const float e1 = sW[lane] * expf(x1), e2 = sW[lane] * expf(x2);
+                f1 += e1;
+                f2 += e2;
 #pragma unroll
                 for (int i = 0; i < NM; ++i) {
-                    g1[i] = fma(e1, rowp[i], g1[i]);
-                    g2[i] = fma(e2, rowp[i], g2[i]);
+                    g1[i] = fmaf(e1, rowp[i], g1[i]);
+                    g2[i] = fmaf(e2, rowp[i], g2[i]);
                 }
             }
         }
         if (cell < 0)
             continue;
         const double *v = S.vec + (size_t)cell * kVec * NM;
-        double n1 = 0.0, n2 = 0.0;
+        float n1 = 0.0f, n2 = 0.0f;
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            const double ui = v[kVu * NM + i];
-            const double d1 = wave_sum(g1[i]) - ui, d2 = wave_sum(g2[i]) - ui;
-            n1 = fma(d1, d1, n1);
-            n2 = fma(d2, d2, n2);
+            float t1 = g1[i], t2 = g2[i];
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                t1 += __shfl_xor(t1, o);
+                t2 += __shfl_xor(t2, o);
+            }
+            const float ui = (float)v[kVu * NM + i];
+            n1 = fmaf(t1 - ui, t1 - ui, n1);
+            n2 = fmaf(t2 - ui, t2 - ui, n2);
         }
         const bool take = n2 < n1;  // false when the proposal overflowed (NaN or inf)
-        const double fs = wave_sum(take ? f2 : f1);
+        float fs = take ? f2 : f1;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1)
+            fs += __shfl_xor(fs, o);
         if (take && lane < NM) {
-            const double an = sA[wv][1][lane];
+            const double an = anet[(size_t)slot * NM + lane];
             double *vw = S.vec + (size_t)cell * kVec * NM;
             vw[kVa * NM + lane] = an;
             vw[kVa0 * NM + lane] = an;
@@ -1438,7 +1516,7 @@ __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double 
             if (take)
                 S.ints[(size_t)cell * kInt + kIflags] |= C3D_FLAG_WARM;
             if (P.mixed) {
-                const double lf = log(fs);
+                const double lf = log((double)fs);
                 S.scal[(size_t)cell * kScal + kSshift] = (lf > -700.0 && lf < 700.0) ? lf : 0.0;
             }
         }
@@ -1690,10 +1768,9 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
     unsigned long long *cntBulk = ctx->profiling ? ctx->d_sweepCount : nullptr;
     unsigned long long *cntTail = ctx->profiling ? ctx->d_sweepCount + 1 : nullptr;
     if (ctx->profiling)
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(2), 0, s, ctx->d_sweepCount);
-    const unsigned gInit = (unsigned)((n * 64 + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_init<NM>, dim3(gInit), dim3(kBlock), 0, s, P, u, alpha, S, Lg[0],
-                       stats, ctx->d_status);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(3), 0, s, ctx->d_sweepCount);
+    hipLaunchKernelGGL(k_init<NM>, dim3(grid_cap(ctx, 16)), dim3(kBlock), 0, s, P, u, alpha, S,
+                       Lg[0], stats, ctx->d_status);
     if (ctx->warm && ctx->mlp.have) {
         const size_t xb = (n * C3D_MLP_IN * sizeof(float) + 255) / 256 * 256;
         e = grow(ctx->warmBuf, xb + n * NM * sizeof(double));
@@ -1754,16 +1831,16 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
             if (mixed)
                 hipLaunchKernelGGL(k_sweep_mixed<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0,
                                    s, P, ctx->d_M, ctx->d_wpad, S, Lc.Wm, Lc.nm);
+            if (ev)
+                (void)hipEventRecord(ev[1], s);
             if (ref)
                 hipLaunchKernelGGL(k_sweep_ref<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s,
                                    P, ctx->d_M, ctx->d_wpad, S, Lc.Wf, Lc.nf);
             else
                 hipLaunchKernelGGL(k_sweep<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P,
                                    ctx->d_M, ctx->d_wpad, S, Lc.Wf, Lc.nf);
-            if (ev) {
-                (void)hipEventRecord(ev[1], s);
+            if (ev)
                 (void)hipEventRecord(ev[2], s);
-            }
             if (ref)
                 hipLaunchKernelGGL((k_step<NM, true>), dim3(grid_cap(ctx, 16)), dim3(kBlock), 0,
                                    s, P, S, Lc, Ln, u, alpha, stats, ctx->d_status, cntBulk);

@@ -352,6 +352,55 @@ __device__ __forceinline__ double normal01(uint64_t seed, uint64_t idx, uint64_t
     return sqrt(-2.0 * log(1.0 - u1)) * cos(2.0 * M_PI * u2);
 }
 
+// Initial state over all cells including ghosts (init.cpp:480-506), cell centres at
+// a + (i - 2 + 1/2) d with ghosts at their own unwrapped centres (init.cpp:483-485):
+//   ic 0: the reference Gaussian, u0 = max(floor, 2 sqrt(pi) G_sigma(x)) with the normalised
+//         G (init.cpp:27-34, 493-495);
+//   ic 1: pulse rho = max(exp(-|x|^2/(2 sigma^2)), floor) (1 + noise U[-1,1]), u0 = 2 sqrt(pi)
+//         rho (counter RNG stream 7 on the cell index);
+//   ic 2: the smooth state of init.cpp:497-502.
+// sigma_t = sigma_s = crossSection; alpha (if not null) from init.cpp:262-268.
+__global__ __launch_bounds__(kTB) void k_init_grid(int nx, int ny, int nz, int nm, double ax,
+                                                   double ay, double az, double dx, double dy,
+                                                   double dz, int ic, double sigma, double floorv,
+                                                   double noise, uint64_t seed, double cross,
+                                                   double *u, double *alpha, double *sigmaT,
+                                                   double *sigmaS)
+{
+    const int64_t NX = nx + 4, NY = ny + 4, NZ = nz + 4;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= NX * NY * NZ)
+        return;
+    const int k = (int)(c % NZ), j = (int)((c / NZ) % NY), i = (int)(c / (NZ * NY));
+    const double x = ax + (i - 2) * dx + dx / 2.0;
+    const double y = ay + (j - 2) * dy + dy / 2.0;
+    const double z = az + (k - 2) * dz + dz / 2.0;
+    sigmaT[c] = cross;
+    sigmaS[c] = cross;
+    double u0, u123 = 0.0;
+    if (ic == 0) {
+        const double f1 = 1.0 / (sigma * sqrt(2.0 * M_PI));
+        const double g = f1 * f1 * f1 * exp(-(x * x + y * y + z * z) / (2.0 * sigma * sigma));
+        u0 = fmax(floorv, g * 2.0 * sqrt(M_PI));
+    } else if (ic == 1) {
+        double rho = fmax(exp(-(x * x + y * y + z * z) / (2.0 * sigma * sigma)), floorv);
+        if (noise != 0.0)
+            rho = rho * (1.0 + noise * (2.0 * uniform01(seed, (uint64_t)c, 7) - 1.0));
+        u0 = 2.0 * sqrt(M_PI) * rho;
+    } else {
+        u0 = (2.0 + cos(2 * M_PI * x) * cos(2 * M_PI * y) * cos(2 * M_PI * z)) * 2.0 * sqrt(M_PI);
+        u123 = 0.3 * u0;
+    }
+    u[c * nm] = u0;
+    for (int m = 1; m < nm; ++m)
+        u[c * nm + m] = (m <= 3) ? u123 : 0.0;
+    if (alpha) {
+        alpha[c * nm] = sqrt(4.0 * M_PI) * log(u0 / sqrt(4.0 * M_PI));
+        for (int m = 1; m < nm; ++m)
+            alpha[c * nm + m] = 0.0;
+    }
+}
+
 template <int NM>
 __global__ __launch_bounds__(kTB) void k_make_batch(int64_t n, int nq, const double *__restrict__ M,
                                                      const double *__restrict__ w, uint64_t seed,
@@ -501,6 +550,18 @@ hipError_t launch_rk2(int nx, int ny, int nz, int nm, double *u, double *uOld, i
     const int64_t n = (int64_t)nx * ny * nz * nm;
     hipLaunchKernelGGL(k_rk2, dim3(blocks_for(n, 256, 1 << 20)), dim3(256), 0, s, G, nm, u, uOld,
                        kind);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_grid(int nx, int ny, int nz, int nm, const double *a, const double *d,
+                            int ic, double sigma, double floorv, double noise, uint64_t seed,
+                            double cross, double *u, double *alpha, double *sigmaT,
+                            double *sigmaS, hipStream_t s)
+{
+    const int64_t ng = (int64_t)(nx + 4) * (ny + 4) * (nz + 4);
+    hipLaunchKernelGGL(k_init_grid, dim3((unsigned)((ng + kTB - 1) / kTB)), dim3(kTB), 0, s, nx,
+                       ny, nz, nm, a[0], a[1], a[2], d[0], d[1], d[2], ic, sigma, floorv, noise,
+                       seed, cross, u, alpha, sigmaT, sigmaS);
     return hipGetLastError();
 }
 

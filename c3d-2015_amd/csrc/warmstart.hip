@@ -47,7 +47,9 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x)
 // B(k,j) = B[k sbk + j sbj], C[i ldc + j].  64x64 tile per 256-thread block, 4x4 per
 // thread, K in steps of 16 through LDS; each output is summed in k order by one thread.
 // Epilogues: 0 store; 1 Z = acc + bias_j, store Z and gelu(Z) (second array); 2 store
-// acc * gelu'(Zin(i,j)) (Zin with the same layout as C).
+// acc * gelu'(Zin(i,j)) (Zin with the same layout as C).  Split K: block z sums k in
+// [z kLen, (z+1) kLen) and stores at C + z M ldc (epilogue 0 only); k_sum_splits adds the
+// partial products in z order.
 constexpr int kBM = 64, kBN = 64, kBK = 16;
 
 template <int EPI>
@@ -56,23 +58,26 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float *
                                               const float *__restrict__ B, int64_t sbk,
                                               int64_t sbj, float *C, int64_t ldc,
                                               const float *__restrict__ bias, float *C2,
-                                              const float *__restrict__ Zin)
+                                              const float *__restrict__ Zin, int kLen)
 {
     __shared__ float As[kBK][kBM + 4];
     __shared__ float Bs[kBK][kBN + 4];
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     const int i0 = blockIdx.y * kBM, j0 = blockIdx.x * kBN;
+    const int kBeg = blockIdx.z * kLen;
+    const int kEnd = min(K, kBeg + kLen);
+    C += (int64_t)blockIdx.z * M * ldc;
     float acc[4][4] = {};
-    for (int k0 = 0; k0 < K; k0 += kBK) {
+    for (int k0 = kBeg; k0 < kEnd; k0 += kBK) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int e = tid + 256 * r;  // 1024 = kBK * kBM
             const int kk = e / kBM, ii = e % kBM;
             const int gi = i0 + ii, gk = k0 + kk;
-            As[kk][ii] = (gi < M && gk < K) ? A[gi * sai + gk * sak] : 0.0f;
+            As[kk][ii] = (gi < M && gk < kEnd) ? A[gi * sai + gk * sak] : 0.0f;
             const int kb = e / kBN, jj = e % kBN;
             const int gj = j0 + jj, gkb = k0 + kb;
-            Bs[kb][jj] = (gj < N && gkb < K) ? B[gkb * sbk + gj * sbj] : 0.0f;
+            Bs[kb][jj] = (gj < N && gkb < kEnd) ? B[gkb * sbk + gj * sbj] : 0.0f;
         }
         __syncthreads();
 #pragma unroll
@@ -113,6 +118,18 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float *
             }
         }
     }
+}
+
+// C[e] = sum_z P[z][e], z in order (the split-K partial products)
+__global__ void k_sum_splits(int n, int splits, const float *__restrict__ Pp, float *C)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n)
+        return;
+    float s = 0.0f;
+    for (int z = 0; z < splits; ++z)
+        s += Pp[(int64_t)z * n + e];
+    C[e] = s;
 }
 
 // out[j] = sum_i X[i ld + j], i < rows, in a fixed tree order (one block per column)
@@ -509,15 +526,33 @@ void gemm(hipStream_t s, int M, int N, int K, const float *A, int64_t sai, int64
 {
     dim3 grid((N + kBN - 1) / kBN, (M + kBM - 1) / kBM);
     hipLaunchKernelGGL(k_gemm<EPI>, grid, dim3(256), 0, s, M, N, K, A, sai, sak, B, sbk, sbj, C,
-                       ldc, bias, C2, Zin);
+                       ldc, bias, C2, Zin, K);
+}
+
+// C[M][N] (ldc = N) with the reduction over K split into pieces of kLen (weight gradients:
+// small M, N and K = the batch); partial products in Pp [splits][M][N]
+constexpr int kSplitLen = 256, kMaxSplits = 64;
+void gemm_splitk(hipStream_t s, int M, int N, int K, const float *A, int64_t sai, int64_t sak,
+                 const float *B, int64_t sbk, int64_t sbj, float *C, float *Pp)
+{
+    const int splits = std::min(kMaxSplits, std::max(1, (K + kSplitLen - 1) / kSplitLen));
+    const int kLen = (K + splits - 1) / splits;
+    dim3 grid((N + kBN - 1) / kBN, (M + kBM - 1) / kBM, splits);
+    hipLaunchKernelGGL(k_gemm<0>, grid, dim3(256), 0, s, M, N, K, A, sai, sak, B, sbk, sbj, Pp,
+                       (int64_t)N, nullptr, nullptr, nullptr, kLen);
+    hipLaunchKernelGGL(k_sum_splits, dim3((M * N + 255) / 256), dim3(256), 0, s, M * N, splits,
+                       Pp, C);
 }
 
 // Workspace of one forward/backward pass over b rows
 struct Acts {
-    float *Z1, *A1, *Z2, *A2, *Y, *dY, *dA, *dB, *part, *loss;
+    float *Z1, *A1, *Z2, *A2, *Y, *dY, *dA, *dB, *part, *loss, *splitk;
 };
 
-size_t acts_floats(int64_t b) { return (size_t)b * (4 * kHid + 2 * kIn + 2 * kHid) + 1024 + 1; }
+size_t acts_floats(int64_t b)
+{
+    return (size_t)b * (4 * kHid + 2 * kIn + 2 * kHid) + 1024 + 1 + (size_t)kMaxSplits * kHid * kHid;
+}
 
 Acts carve(float *w, int64_t b)
 {
@@ -532,6 +567,7 @@ Acts carve(float *w, int64_t b)
     a.dB = a.dA + b * kHid;
     a.part = a.dB + b * kHid;
     a.loss = a.part + 1024;
+    a.splitk = a.loss + 1;
     return a;
 }
 
@@ -564,15 +600,15 @@ void backward(hipStream_t s, int64_t b, const float *p, const float *X, float *g
 {
     const int M = (int)b;
     // layer 3
-    gemm<0>(s, kHid, kIn, M, a.A2, 1, kHid, a.dY, kIn, 1, grad + oW3, kIn);
+    gemm_splitk(s, kHid, kIn, M, a.A2, 1, kHid, a.dY, kIn, 1, grad + oW3, a.splitk);
     hipLaunchKernelGGL(k_colsum, dim3(kIn), dim3(256), 0, s, M, kIn, a.dY, grad + oB3);
     gemm<2>(s, M, kHid, kIn, a.dY, kIn, 1, p + oW3, 1, kIn, a.dA, kHid, nullptr, nullptr, a.Z2);
     // layer 2 (dA holds dZ2)
-    gemm<0>(s, kHid, kHid, M, a.A1, 1, kHid, a.dA, kHid, 1, grad + oW2, kHid);
+    gemm_splitk(s, kHid, kHid, M, a.A1, 1, kHid, a.dA, kHid, 1, grad + oW2, a.splitk);
     hipLaunchKernelGGL(k_colsum, dim3(kHid), dim3(256), 0, s, M, kHid, a.dA, grad + oB2);
     gemm<2>(s, M, kHid, kHid, a.dA, kHid, 1, p + oW2, 1, kHid, a.dB, kHid, nullptr, nullptr, a.Z1);
     // layer 1 (dB holds dZ1)
-    gemm<0>(s, kIn, kHid, M, X, 1, kIn, a.dB, kHid, 1, grad + oW1, kHid);
+    gemm_splitk(s, kIn, kHid, M, X, 1, kIn, a.dB, kHid, 1, grad + oW1, a.splitk);
     hipLaunchKernelGGL(k_colsum, dim3(kHid), dim3(256), 0, s, M, kHid, a.dB, grad + oB1);
 }
 

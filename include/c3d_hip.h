@@ -66,7 +66,8 @@ typedef struct c3d_ctx_s *c3d_ctx;
  *   unbatched = 0: Optimization::opt semantics (optBatch.cpp:143-282): absolute tol,
  *                  tolGamma from the deck, the custom dpotrs (utils.cpp:71-86).
  *   unbatched = 1: opt()/optScaled semantics (opt.cpp:139-355): tol relative to
- *                  ||u/u0||_2, tolGamma fixed at 1.1 (update.cpp:57), LAPACK dpotrs order. */
+ *                  ||u/u0||_2 (OPTIONS.tolRel), tolGamma from the options (update() passes
+ *                  1.1, update.cpp:57; c3d_update does the same), LAPACK dpotrs order. */
 typedef struct {
     int32_t maxIter;        /* MN_MAX_ITER */
     int32_t useGaunt;       /* MN_USE_GAUNT_COEF */
@@ -179,14 +180,16 @@ int c3d_set_mixed(c3d_ctx ctx, double cond_max, int max_iter, double switch_tol)
 /* Kernel-time accounting of c3d_closure (HIP events on the closure's stream around every
  * launch), for throughput and roofline reporting. */
 typedef struct {
-    int64_t cell_sweeps;      /* cell evaluations of H(alpha) by the bulk sweep kernel */
-    int64_t tail_cell_sweeps; /* cell evaluations of H(alpha) inside the fused tail kernel */
-    int32_t passes;           /* worklist passes (a tail launch counts its pass limit) */
+    int64_t cell_sweeps;       /* cell evaluations of H(alpha) by the FP64 sweep kernel */
+    int64_t tail_cell_sweeps;  /* cell evaluations of H(alpha) inside the fused tail kernel */
+    int64_t mixed_cell_sweeps; /* cell evaluations of H(alpha) by the mixed-phase sweep kernel */
+    int32_t passes;            /* worklist passes (a tail launch counts its pass limit) */
     int32_t tail_launches;
-    double sweep_ms;          /* summed duration of the bulk sweep kernels */
-    double step_ms;           /* summed duration of the bulk Newton-step kernels */
-    double tail_ms;           /* summed duration of the fused tail kernels */
-    double warm_ms;           /* warm start: features, network, selection */
+    double sweep_ms;           /* summed duration of the FP64 sweep kernels */
+    double mixed_sweep_ms;     /* summed duration of the mixed-phase sweep kernels */
+    double step_ms;            /* summed duration of the Newton-step kernels */
+    double tail_ms;            /* summed duration of the fused tail kernels */
+    double warm_ms;            /* warm start: features, network, selection */
 } c3d_closure_profile;
 
 int c3d_set_profiling(c3d_ctx ctx, int enable);
@@ -315,6 +318,17 @@ int c3d_mlp_loss_grad(c3d_ctx ctx, int64_t n, const float *params, const float *
 int c3d_mlp_dual_loss_grad(c3d_ctx ctx, int64_t n, const float *params, const float *X,
                            const double *u, const c3d_mlp_norm *norm, float *grad, float *loss,
                            void *stream);
+
+/* Initial state of a grid on the device (init.cpp:480-506 and 262-268): u [ng][nm], alpha
+ * [ng][nm] (may be NULL), sigmaT/sigmaS [ng] = crossSection, over all (nx+4)(ny+4)(nz+4)
+ * cells with centres a + (i - 2 + 1/2) d.  ic 0: the reference's normalised Gaussian with
+ * u0 = max(floor, 2 sqrt(pi) G); ic 1: rho = max(exp(-|x|^2/(2 sigma^2)), floor) times
+ * (1 + noise U[-1,1]) from the counter RNG (seed), u0 = 2 sqrt(pi) rho; ic 2: the smooth
+ * state (init.cpp:497-502).  a, d: 3 doubles (host). */
+int c3d_init_grid(c3d_ctx ctx, int nx, int ny, int nz, const double *a, const double *d, int ic,
+                  double sigma, double floor_value, double noise, uint64_t seed,
+                  double cross_section, double *u, double *alpha, double *sigmaT,
+                  double *sigmaS, void *stream);
 
 /* Synthetic closure batch on the device: per cell s ~ U[smin,smax], alpha*_k ~ N(0,s^2),
  * u = sum_q w_q m_q exp(alpha*.m_q), alpha0 from init.cpp:265.  SplitMix64 counter RNG

@@ -830,7 +830,7 @@ int orc_closure_cell(int nm, int nm2, int nq, double *u, double *alpha, const or
     double tstep = 1.0, r = 0.0, err = 0.0, gamma = 0.0;
     int maxIter = o->maxIter;
     const int unb = o->unbatched;
-    double tolGamma = unb ? 1.1 : o->tolGamma; /* update.cpp:57 hard-codes 1.1 */
+    double tolGamma = o->tolGamma; /* opt.cpp:149; update() passes 1.1 when unbatched */
     double rtol = unb ? orc_dnrm2(nm, u) * o->tol : o->tol;
 
     for (;;) {
@@ -1160,6 +1160,9 @@ int orc_update(int nx, int ny, int nz, int nm, int nm2, int nq, int solverMn,
 {
     int errs = 0;
     int64_t ng = (int64_t)(nx + NG) * (ny + NG) * (nz + NG);
+    orc_opts oc = *o;
+    if (oc.unbatched)
+        oc.tolGamma = 1.1; /* solveOptimization passes 1.1 to opt(), update.cpp:57 */
     for (int i = 2; i <= nx + 1; i++)
         for (int j = 2; j <= ny + 1; j++)
             for (int k = 2; k <= nz + 1; k++)
@@ -1168,7 +1171,7 @@ int orc_update(int nx, int ny, int nz, int nm, int nm2, int nq, int solverMn,
     for (int st = 0; st < 2; st++) {
         orc_ghost_fill(nx, ny, nz, nm, u);
         if (solverMn) {
-            errs |= orc_closure(ng, nm, nm2, nq, u, alpha, o, t,
+            errs |= orc_closure(ng, nm, nm2, nq, u, alpha, &oc, t,
                                 stats ? stats + st * ng : NULL, nthreads);
             orc_flux_kinetic(nx, ny, nz, nm, nq, alpha, flux, t, dx, dy, dz, theta, 0, ansatz,
                              nthreads);

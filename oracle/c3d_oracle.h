@@ -40,7 +40,7 @@ typedef struct {
     int32_t useGauntSparse;   /* MN_USE_GAUNT_SPARSE */
     int32_t unbatched;        /* 0: optBatch.cpp semantics (MN_BATCHED true), 1: opt.cpp semantics */
     double  tol;              /* MN_TOL (absolute in batched mode, relative in unbatched) */
-    double  tolGamma;         /* MN_TOL_GAMMA (batched); unbatched uses 1.1 (update.cpp:57) */
+    double  tolGamma;         /* MN_TOL_GAMMA (batched) / OPTIONS.tolGamma (unbatched; update.cpp:57 passes 1.1) */
     double  condHMax;         /* MN_COND_H_MAX */
 } orc_opts;
 

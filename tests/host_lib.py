@@ -1,0 +1,95 @@
+"""ctypes access to libc3d_host.so (the reference-signature host layer, include/c3d_host.h)."""
+import ctypes as C
+import os
+
+from conftest import ROOT, product
+
+# mangled names of the C++ entry points (Itanium ABI), as a reference build links them
+CXX_SYMBOLS = {
+    "opt": "_Z3optiiiPdS_7OPTIONSS_S_S_P4OUTS",
+    "update": "_Z6updated",
+    "communicateBoundaries": "_Z21communicateBoundariesv",
+    "solveFluxKinetic": "_Z16solveFluxKineticPdS_S_ddd",
+    "solveFluxMoment": "_Z15solveFluxMomentPdS_ddd",
+    "fobj_init": "_Z9fobj_initi",
+    "linesearch_init": "_Z15linesearch_initi",
+    "Optimization::Optimization": "_ZN12OptimizationC1EiiiiPdS0_",
+    "Optimization::preProcess": "_ZN12Optimization10preProcessEi",
+    "Optimization::postProcess": "_ZN12Optimization11postProcessEi",
+    "Optimization::opt": "_ZN12Optimization3optEiii",
+    "InputDeckReader::readInputDeck": "_ZN15InputDeckReader13readInputDeckEPKc",
+    "InputDeckReader::getValue(char*)": "_ZN15InputDeckReader8getValueEPKcPc",
+    "InputDeckReader::getValue(int*)": "_ZN15InputDeckReader8getValueEPKcPi",
+    "InputDeckReader::getValue(double*)": "_ZN15InputDeckReader8getValueEPKcPd",
+    "InputDeckReader::getValue(bool*)": "_ZN15InputDeckReader8getValueEPKcPb",
+    "InputDeckReader::print": "_ZN15InputDeckReader5printEv",
+}
+C_SYMBOLS = ("c3d_host_deck_value", "c3d_host_set_precision", "c3d_host_optimize_cells")
+
+
+class OPTIONS(C.Structure):
+    _fields_ = [("maxIter", C.c_int), ("tolAbs", C.c_double), ("tolRel", C.c_double),
+                ("tolGamma", C.c_double), ("condHMax", C.c_double), ("useGaunt", C.c_bool),
+                ("useGauntSparse", C.c_bool)]
+
+
+class OUTS(C.Structure):
+    _fields_ = [("r", C.c_double), ("iter", C.c_int), ("iterGamma", C.c_int),
+                ("gamma", C.c_double), ("normG", C.c_double)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        product().load()  # libc3d_hip.so first (the host library links it)
+        path = os.path.join(ROOT, "c3d-2015_amd", "libc3d_host.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libc3d_host.so not built (make -C c3d-2015_amd)")
+        # local scope: its reference-named symbols must not interpose on other libraries
+        L = C.CDLL(path, mode=C.RTLD_LOCAL)
+        L.c3d_host_deck_value.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
+        L.c3d_host_deck_value.restype = C.c_int
+        L.c3d_host_set_precision.argtypes = [C.c_int]
+        L.c3d_host_set_precision.restype = C.c_int
+        L.c3d_host_optimize_cells.argtypes = [C.c_int64, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_void_p]
+        L.c3d_host_optimize_cells.restype = C.c_int
+        fn = getattr(L, CXX_SYMBOLS["opt"])
+        fn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, OPTIONS, C.c_void_p,
+                       C.c_void_p, C.c_void_p, C.POINTER(OUTS)]
+        fn.restype = None
+        getattr(L, CXX_SYMBOLS["update"]).argtypes = [C.c_double]
+        getattr(L, CXX_SYMBOLS["communicateBoundaries"]).argtypes = []
+        getattr(L, CXX_SYMBOLS["solveFluxKinetic"]).argtypes = [C.c_void_p] * 3 + [C.c_double] * 3
+        getattr(L, CXX_SYMBOLS["solveFluxMoment"]).argtypes = [C.c_void_p] * 2 + [C.c_double] * 3
+        _lib = L
+    return _lib
+
+
+class Globals:
+    """Sets the reference-named globals of libc3d_host.so and keeps the arrays alive."""
+
+    def __init__(self, L):
+        self.L = L
+        self.keep = []
+
+    def int(self, name, v):
+        C.c_int.in_dll(self.L, name).value = int(v)
+
+    def bool(self, name, v):
+        C.c_bool.in_dll(self.L, name).value = bool(v)
+
+    def double(self, name, v):
+        C.c_double.in_dll(self.L, name).value = float(v)
+
+    def ints4(self, name, vals):
+        arr = (C.c_int * 4).in_dll(self.L, name)
+        for k, v in enumerate(vals):
+            arr[k] = int(v)
+
+    def ptr(self, name, a):
+        self.keep.append(a)
+        C.c_void_p.in_dll(self.L, name).value = None if a is None else a.ctypes.data

@@ -168,3 +168,31 @@ def test_update_fp64_matrix_core_mode(P, O, dev, T2):
     print(f"density rel {rho:.2e} moments rel {rel:.2e}")
     assert rho <= 1e-5 and rel <= 2e-3, (rho, rel)
     ctx.close()
+
+
+@pytest.mark.parametrize("ic,noise", [(0, 0.0), (1, 0.0), (1, 0.01), (2, 0.0)])
+def test_init_grid_matches_oracle(P, O, dev, ic, noise):
+    """Device initial state (init.cpp:480-506, 262-268) against the oracle's: the
+    counter-RNG noise is identical, exp/log/cos differ by at most an ulp."""
+    import torch
+    T = O.Tables(2, 8)
+    nx, ny, nz = 9, 7, 8
+    sigma = 0.2 if ic == 0 else 0.25
+    u, sT, sS = O.init_grid(nx, ny, nz, T.nm, ic=ic, sigma=sigma, floor=1e-4, noise=noise,
+                            seed=42, cross=1.5)
+    alpha = O.init_alpha(u)
+    ng = u.shape[0]
+    ctx = P.Ctx(tables=T)
+    ud = torch.zeros((ng, T.nm), dtype=torch.float64, device=dev)
+    ad = torch.zeros_like(ud)
+    st = torch.zeros(ng, dtype=torch.float64, device=dev)
+    ss = torch.zeros_like(st)
+    ctx.init_grid(nx, ny, nz, ud, st, ss, alpha=ad, ic=ic, sigma=sigma, floor=1e-4,
+                  noise=noise, seed=42, cross=1.5)
+    torch.cuda.synchronize()
+    ug, ag = ud.cpu().numpy(), ad.cpu().numpy()
+    ctx.close()
+    assert np.array_equal(st.cpu().numpy(), sT) and np.array_equal(ss.cpu().numpy(), sS)
+    assert np.allclose(ug, u, rtol=2e-15, atol=0)
+    assert np.allclose(ag, alpha, rtol=0, atol=4e-15 * np.abs(alpha).max())
+    assert (u[:, 0] > 0).all()

@@ -1,0 +1,146 @@
+"""GPU parity of the host-level drop-in (libc3d_host.so, include/c3d_host.h): the
+reference-signature entry points driven through the reference-named globals, in the
+reference-order mode, equal the CPU oracle bit for bit (oracle in exp mode 1):
+update() (update.cpp:185-275), opt() (opt.cpp:330-355), Optimization::opt via the
+solveBatchOptimization pattern (update.cpp:105-178), solveFluxKinetic, solveFluxMoment and
+communicateBoundaries."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import host_lib as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid_globals(L, O, T, nx, ny, nz, u, alpha, sT, sS, d, theta, solver_mn=True, batched=True):
+    G = H.Globals(L)
+    G.int("g_numMoments", T.nm)
+    G.int("g_numMoments2", T.nm2)
+    G.int("g_numQuadPoints", T.nq)
+    G.int("g_nx", nx)
+    G.int("g_ny", ny)
+    G.int("g_nz", nz)
+    G.ints4("g_gX", [0, 2, nx + 1, nx + 3])
+    G.ints4("g_gY", [0, 2, ny + 1, ny + 3])
+    G.ints4("g_gZ", [0, 2, nz + 1, nz + 3])
+    for name in ("g_dx", "g_dy", "g_dz"):
+        G.double(name, d)
+    G.double("g_theta", theta)
+    G.int("g_solver", 1 if solver_mn else 0)
+    G.int("g_pnUseKineticFlux", 0)
+    for name, a in (("g_w", T.w), ("g_m", T.M), ("g_m2", T.M2), ("g_xi", T.xi),
+                    ("g_eta", T.eta), ("g_mu", T.mu), ("g_u", u), ("g_mnAlpha", alpha),
+                    ("g_sigmaT", sT), ("g_sigmaS", sS)):
+        assert a is None or (a.flags["C_CONTIGUOUS"] and a.dtype == np.float64)
+        G.ptr(name, a)
+    G.int("g_mnMaxIter", 100)
+    G.double("g_mnTol", 1e-8)
+    G.double("g_mnTolGamma", 1.1)
+    G.double("g_mnCondHMax", 1e10)
+    G.bool("g_mnBatched", batched)
+    hist = np.zeros(5, dtype=np.int32)
+    G.ptr("g_mnHistReg", hist)
+    G.int("g_mnNumDualSolves", 0)
+    G.int("g_numUpdates", 0)
+    return G, hist
+
+
+@pytest.mark.parametrize("batched", [True, False])
+def test_update_matches_oracle_bitwise(P, O, dev, batched):
+    L = H.load()
+    assert L.c3d_host_set_precision(P.PREC_FP64_REFORDER) == 0
+    T = O.Tables(2, 8)
+    nx, ny, nz = 6, 5, 7
+    d = 2.0 / nx
+    u, sT, sS = O.init_grid(nx, ny, nz, T.nm, ic=1, sigma=0.25, floor=1e-4)
+    alpha = O.init_alpha(u)
+    dt = 0.5 * (2 / 3) / (3 / d)
+    ug, ag = u.copy(), alpha.copy()
+    G, hist = _grid_globals(L, O, T, nx, ny, nz, ug, ag, sT, sS, d, 1.0, batched=batched)
+    ur, ar = u.copy(), alpha.copy()
+    ng = u.shape[0]
+    oo = O.opts(unbatched=not batched)
+    for step in range(2):
+        getattr(L, H.CXX_SYMBOLS["update"])(C.c_double(dt))
+        with O.exp_mode(1):
+            O.lib().orc_update(nx, ny, nz, T.nm, T.nm2, T.nq, 1, 0, ur, ar, np.zeros_like(u),
+                               np.zeros_like(u), np.zeros(ng * T.nq), sT, sS, O.C.byref(T.c),
+                               T.ops, O.C.byref(oo), d, d, d, 1.0, dt, None, 0)
+    assert np.array_equal(ug, ur) and np.array_equal(ag, ar)
+    assert C.c_int.in_dll(L, "g_numUpdates").value == 2
+    assert hist.sum() == 2 * 2 * ng == C.c_int.in_dll(L, "g_mnNumDualSolves").value
+
+
+def test_opt_single_cell_and_batched_cells_match_oracle(P, O, dev):
+    L = H.load()
+    assert L.c3d_host_set_precision(P.PREC_FP64_REFORDER) == 0
+    T = O.Tables(3, 12)
+    n = 40
+    a_star, u, a0 = O.closure_batch_input(T, n, 123, 0.5, 6.0)
+    # opt(): unbatched semantics, tolRel, tolGamma from the options
+    uo, ao = u.copy(), a0.copy()
+    with O.exp_mode(1):
+        rc, so = O.closure(T, uo, ao, O.opts(unbatched=True, tol=1e-9, tolGamma=1.1))
+    fn = getattr(L, H.CXX_SYMBOLS["opt"])
+    opts = H.OPTIONS(100, 1e-18, 1e-9, 1.1, 1e10, False, False)
+    for c in range(n):
+        uc, ac = u[c].copy(), a0[c].copy()
+        outs = H.OUTS()
+        fn(T.nm, T.nm2, T.nq, uc.ctypes.data, ac.ctypes.data, opts, T.w.ctypes.data,
+           T.M.ctypes.data, T.M2.ctypes.data, C.byref(outs))
+        assert np.array_equal(uc, uo[c]) and np.array_equal(ac, ao[c]), c
+        assert outs.iter == so["totalIter"][c] and outs.iterGamma == so["iterGamma"][c]
+        assert outs.r == so["r"][c] and outs.normG == so["err"][c] and outs.gamma == so["gamma"][c]
+    # Optimization::opt over c_stack1 -> c_stack3, batched semantics from the globals
+    G = H.Globals(L)
+    for name, a in (("g_w", T.w), ("g_m", T.M), ("g_m2", T.M2), ("g_xi", T.xi),
+                    ("g_eta", T.eta), ("g_mu", T.mu)):
+        G.ptr(name, a)
+    G.int("g_mnMaxIter", 100)
+    G.double("g_mnTol", 1e-8)
+    G.double("g_mnTolGamma", 1.1)
+    G.double("g_mnCondHMax", 1e10)
+    ub, ab = u.copy(), a0.copy()
+    it = np.zeros(n, dtype=np.int32)
+    rg = np.zeros(n, dtype=np.int32)
+    assert L.c3d_host_optimize_cells(n, T.nm, T.nm2, T.nq, ub.ctypes.data, ab.ctypes.data,
+                                     it.ctypes.data, rg.ctypes.data) == 0
+    uo, ao = u.copy(), a0.copy()
+    with O.exp_mode(1):
+        rc, so = O.closure(T, uo, ao, O.opts())
+    assert np.array_equal(ub, uo) and np.array_equal(ab, ao)
+    assert np.array_equal(it, so["totalIter"]) and np.array_equal(rg, so["regIndex"])
+
+
+def test_flux_and_boundary_entry_points_match_oracle(P, O, dev):
+    L = H.load()
+    assert L.c3d_host_set_precision(P.PREC_FP64_REFORDER) == 0
+    T = O.Tables(2, 8)
+    nx, ny, nz = 7, 6, 5
+    d = 2.0 / nx
+    u, sT, sS = O.init_grid(nx, ny, nz, T.nm, ic=1, sigma=0.3, floor=1e-3, noise=0.05, seed=3)
+    rng = np.random.default_rng(8)
+    u[:, 1:] = u[:, :1] * rng.uniform(-0.2, 0.2, (u.shape[0], T.nm - 1))
+    alpha = O.init_alpha(u) + np.pad(rng.normal(0, 0.3, (u.shape[0], T.nm - 1)), ((0, 0), (1, 0)))
+    alpha = np.ascontiguousarray(alpha)
+    ug = u.copy()
+    G, _ = _grid_globals(L, O, T, nx, ny, nz, ug, alpha, sT, sS, d, 1.5)
+    f1 = np.full_like(u, 7.0)
+    f2 = np.full_like(u, 7.0)
+    getattr(L, H.CXX_SYMBOLS["solveFluxKinetic"])(ug.ctypes.data, f1.ctypes.data,
+                                                   alpha.ctypes.data, d, d, d)
+    with O.exp_mode(1):
+        O.lib().orc_flux_kinetic(nx, ny, nz, T.nm, T.nq, alpha, f2, O.C.byref(T.c), d, d, d,
+                                 1.5, 0, np.zeros(u.shape[0]), 0)
+    assert np.array_equal(f1, f2)
+    f1 = np.full_like(u, 3.0)
+    f2 = np.full_like(u, 3.0)
+    getattr(L, H.CXX_SYMBOLS["solveFluxMoment"])(ug.ctypes.data, f1.ctypes.data, 0.2, 0.25, 0.3)
+    O.lib().orc_flux_moment(nx, ny, nz, T.nm, ug, f2, T.ops, 0.2, 0.25, 0.3, 0)
+    assert np.array_equal(f1, f2)
+    ur = ug.copy()
+    getattr(L, H.CXX_SYMBOLS["communicateBoundaries"])()
+    O.lib().orc_ghost_fill(nx, ny, nz, T.nm, ur)
+    assert np.array_equal(ug, ur) and not np.array_equal(ug, u)
