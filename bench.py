@@ -196,11 +196,26 @@ def main():
                 "flops_per_cell_sweep": flops_per_sweep, "cell_sweeps_per_step": int(cs),
                 "kernel_ms_per_step": round(ms, 3)}
 
+    # HBM bytes per dispatch of the sweep kernels, from the committed rocprofv3 PMC summary
+    # (FETCH_SIZE / WRITE_SIZE runs calibrated on a copy of known size)
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tpath):
+        for k, v in json.load(open(tpath))["kernels"].items():
+            traffic[k] = round(v["fetch_bytes_per_dispatch"] + v["write_bytes_per_dispatch"])
+
+    def with_traffic(r):
+        key = r["kernel"].split(" ")[0]
+        r["traffic"] = traffic.get(key)
+        if r["traffic"] is not None:
+            r["traffic_source"] = "profiles/closure_C1_pmc_traffic.md (avg bytes per dispatch)"
+        return r
+
     mixed_main = args.path.startswith("mixed")
-    roofline = sweep_roofline(main_r["prof"], mixed_main)
+    roofline = with_traffic(sweep_roofline(main_r["prof"], mixed_main))
     if mixed_main:
-        roofline["fp64_sweep_in_path"] = sweep_roofline(main_r["prof"], False)
-    roofline["fp64_path_sweep"] = sweep_roofline(base_r["prof"], False)
+        roofline["fp64_sweep_in_path"] = with_traffic(sweep_roofline(main_r["prof"], False))
+    roofline["fp64_path_sweep"] = with_traffic(sweep_roofline(base_r["prof"], False))
 
     def kernel_ms(r):
         p = r["prof"]
