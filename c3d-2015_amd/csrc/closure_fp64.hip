@@ -95,7 +95,7 @@ __device__ __forceinline__ void list_append(const Lists &L, bool mixedPhase, int
 
 // Appends of a block to the next pass's lists, gathered in LDS and flushed with one global
 // atomic per list and batch (a global atomic per cell on one counter serialises at the L2).
-// push: one lane per wave, at most kWavesPerBlock pushes between two flush() calls; flush:
+// push: one lane per cell, at most perRound pushes between two flush() calls; flush:
 // called by the whole block at a block-uniform point.
 constexpr int kAggCap = 256;
 struct BlockAppender {
@@ -166,23 +166,81 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-template <int NM>
+// A cell's lanes: the whole wave (G = 64, one cell per wave; broadcasts by readlane) or one
+// 16-lane row (G = 16, four cells per wave; broadcasts by DPP row_newbcast).  Lane indices
+// passed to the algebra below are indices within the group.  The arithmetic a cell sees is
+// the same in both layouts.
+template <int G>
+struct Grp;
+
+template <>
+struct Grp<64> {
+    static __device__ __forceinline__ double bc(double v, int j) { return rdl(v, j); }
+    static __device__ __forceinline__ double bc_dyn(double v, int j) { return rdl(v, j); }
+    static __device__ __forceinline__ bool all(bool p) { return __all(p); }
+};
+
+template <int J>
+__device__ __forceinline__ double row_bcast(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + J, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + J, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+template <>
+struct Grp<16> {
+    // j is a constant after unrolling; the switch folds away
+    static __device__ __forceinline__ double bc(double v, int j)
+    {
+        switch (j) {
+        case 0: return row_bcast<0>(v);
+        case 1: return row_bcast<1>(v);
+        case 2: return row_bcast<2>(v);
+        case 3: return row_bcast<3>(v);
+        case 4: return row_bcast<4>(v);
+        case 5: return row_bcast<5>(v);
+        case 6: return row_bcast<6>(v);
+        case 7: return row_bcast<7>(v);
+        case 8: return row_bcast<8>(v);
+        case 9: return row_bcast<9>(v);
+        case 10: return row_bcast<10>(v);
+        case 11: return row_bcast<11>(v);
+        case 12: return row_bcast<12>(v);
+        case 13: return row_bcast<13>(v);
+        case 14: return row_bcast<14>(v);
+        default: return row_bcast<15>(v);
+        }
+    }
+    // a lane index only known at run time (uniform within the row)
+    static __device__ __forceinline__ double bc_dyn(double v, int j)
+    {
+        return __shfl(v, (int)(__lane_id() & 48u) + j);
+    }
+    static __device__ __forceinline__ bool all(bool p)
+    {
+        const unsigned long long b = __ballot(p);
+        return ((b >> (__lane_id() & 48u)) & 0xFFFFull) == 0xFFFFull;
+    }
+};
+
+template <int NM, int G = 64>
 struct LaneVec {
     static __device__ __forceinline__ double dasum(double x)
     {
         double s = 0.0;
 #pragma unroll
         for (int i = 0; i < NM; ++i)
-            s = s + fabs(rdl(x, i));
+            s = s + fabs(Grp<G>::bc(x, i));
         return s;
     }
     static __device__ __forceinline__ int idamax(double x)
     {
         int im = 0;
-        double m = fabs(rdl(x, 0));
+        double m = fabs(Grp<G>::bc(x, 0));
 #pragma unroll
         for (int i = 1; i < NM; ++i) {
-            double v = fabs(rdl(x, i));
+            double v = fabs(Grp<G>::bc(x, i));
             if (v > m) {
                 m = v;
                 im = i;
@@ -195,7 +253,7 @@ struct LaneVec {
         double scale = 0.0, ssq = 1.0;
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            double xi = rdl(x, i);
+            double xi = Grp<G>::bc(x, i);
             if (xi != 0.0) {
                 double a = fabs(xi);
                 if (scale < a) {
@@ -217,7 +275,7 @@ struct LaneVec {
 // solves.  EXACT divides by the diagonal as
 // LAPACK/BLAS do; otherwise the triangular solves multiply by reciprocals of the diagonal
 // computed once per factor (dinv, lane i: 1/L(i,i)), which shortens their critical path.
-template <int NM, bool EXACT = true>
+template <int NM, bool EXACT = true, int G = 64>
 struct Dense {
     double row[NM];
     const double *sT = nullptr;  // LDS copy of L, row-major with leading dimension ld
@@ -230,7 +288,7 @@ struct Dense {
         return (j > lane && lane < NM) ? sT[j * ld + lane] : 0.0;
     }
 
-    __device__ __forceinline__ double diag(int j) const { return rdl(row[j], j); }
+    __device__ __forceinline__ double diag(int j) const { return Grp<G>::bc(row[j], j); }
 
     // dpotf2 'L': ajj = a(j,j) - ddot; column by dgemv then dscal(1/ajj)
     __device__ __forceinline__ int cholesky(int lane)
@@ -241,14 +299,14 @@ struct Dense {
 #pragma unroll
             for (int k = 0; k < j; ++k)
                 s = s + row[k] * row[k];
-            double ajj = rdl(row[j] - s, j);
+            double ajj = Grp<G>::bc(row[j] - s, j);
             if (!(ajj > 0.0))
                 return j + 1;
             ajj = sqrt(ajj);
             double y = row[j];
 #pragma unroll
             for (int k = 0; k < j; ++k) {
-                double temp = -rdl(row[k], j);
+                double temp = -Grp<G>::bc(row[k], j);
                 y = y + temp * row[k];
             }
             double rr = 1.0 / ajj;
@@ -285,12 +343,12 @@ struct Dense {
     {
 #pragma unroll
         for (int j = 0; j < NM; ++j) {
-            double xj = rdl(x, j);
+            double xj = Grp<G>::bc(x, j);
             if (xj != 0.0) {
                 if constexpr (EXACT)
                     xj = xj / diag(j);
                 else
-                    xj = xj * rdl(dinv, j);
+                    xj = xj * Grp<G>::bc(dinv, j);
                 if (lane == j)
                     x = xj;
                 else if (lane > j)
@@ -304,7 +362,7 @@ struct Dense {
         for (int j = NM - 1; j >= 0; --j) {
             if (lane == j)
                 x = EXACT ? x / row[j] : x * dinv;
-            double xj = rdl(x, j);
+            double xj = Grp<G>::bc(x, j);
             if (lane < j)
                 x = x - col(j, lane) * xj;
         }
@@ -315,7 +373,7 @@ struct Dense {
         for (int j = 0; j < NM; ++j) {
             if (lane == j)
                 d = EXACT ? d / row[j] : d * dinv;
-            double dj = rdl(d, j);
+            double dj = Grp<G>::bc(d, j);
             if (lane > j)
                 d = d - row[j] * dj;
         }
@@ -323,7 +381,7 @@ struct Dense {
         for (int j = NM - 1; j >= 0; --j) {
             if (lane == j)
                 d = EXACT ? d / row[j] : d * dinv;
-            double dj = rdl(d, j);
+            double dj = Grp<G>::bc(d, j);
             if (lane < j)
                 d = d - col(j, lane) * dj;
         }
@@ -333,9 +391,9 @@ struct Dense {
     {
 #pragma unroll
         for (int k = 0; k < NM; ++k) {
-            double dk = rdl(d, k);
+            double dk = Grp<G>::bc(d, k);
             if (dk != 0.0) {
-                dk = EXACT ? dk / diag(k) : dk * rdl(dinv, k);
+                dk = EXACT ? dk / diag(k) : dk * Grp<G>::bc(dinv, k);
                 if (lane == k)
                     d = dk;
                 else if (lane > k)
@@ -344,11 +402,11 @@ struct Dense {
         }
 #pragma unroll
         for (int i = NM - 1; i >= 0; --i) {
-            double temp = rdl(d, i);
+            double temp = Grp<G>::bc(d, i);
 #pragma unroll
             for (int k = i + 1; k < NM; ++k)
-                temp = temp - sT[k * ld + i] * rdl(d, k);
-            temp = EXACT ? temp / diag(i) : temp * rdl(dinv, i);
+                temp = temp - sT[k * ld + i] * Grp<G>::bc(d, k);
+            temp = EXACT ? temp / diag(i) : temp * Grp<G>::bc(dinv, i);
             if (lane == i)
                 d = temp;
         }
@@ -370,7 +428,7 @@ struct Dense {
             trsv_lt(x, lane);
             bool l120 = false;
             if (isave1 == 1) {
-                est = LaneVec<NM>::dasum(x);
+                est = LaneVec<NM, G>::dasum(x);
                 if (own) {
                     x = (x >= 0.0) ? 1.0 : -1.0;
                     isgn = (x >= 0.0) ? 1 : -1;
@@ -378,16 +436,16 @@ struct Dense {
                 isave1 = 2;
                 continue;
             } else if (isave1 == 2) {
-                isave2 = LaneVec<NM>::idamax(x);
+                isave2 = LaneVec<NM, G>::idamax(x);
                 isave3 = 2;
                 x = (lane == isave2) ? 1.0 : 0.0;
                 isave1 = 3;
                 continue;
             } else if (isave1 == 3) {
                 double estold = est;
-                est = LaneVec<NM>::dasum(x);
+                est = LaneVec<NM, G>::dasum(x);
                 bool same = !own || (((x >= 0.0) ? 1 : -1) == isgn);
-                if (__all(same) || est <= estold) {
+                if (Grp<G>::all(same) || est <= estold) {
                     l120 = true;
                 } else {
                     if (own) {
@@ -399,8 +457,8 @@ struct Dense {
                 }
             } else if (isave1 == 4) {
                 int jlast = isave2;
-                isave2 = LaneVec<NM>::idamax(x);
-                if (rdl(x, jlast) != fabs(rdl(x, isave2)) && isave3 < 5) {
+                isave2 = LaneVec<NM, G>::idamax(x);
+                if (Grp<G>::bc_dyn(x, jlast) != fabs(Grp<G>::bc_dyn(x, isave2)) && isave3 < 5) {
                     ++isave3;
                     x = (lane == isave2) ? 1.0 : 0.0;
                     isave1 = 3;
@@ -408,7 +466,7 @@ struct Dense {
                 }
                 l120 = true;
             } else {
-                double temp = 2.0 * (LaneVec<NM>::dasum(x) / (double)(3 * NM));
+                double temp = 2.0 * (LaneVec<NM, G>::dasum(x) / (double)(3 * NM));
                 if (temp > est)
                     est = temp;
                 break;
@@ -902,9 +960,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_mixed(KParams P, const double 
 }
 
 // ---------------------------------------------------------------- one cell's state machine
-// Register image of a cell's Newton state; vectors are distributed (lane i < NM holds
-// component i), scalars are wave-uniform.
-template <int NM>
+// Register image of a cell's Newton state over a group of G lanes; vectors are distributed
+// (group lane i < NM holds component i), scalars are uniform over the group.
+template <int NM, int G = 64>
 struct Cell {
     double a, a0, aOld, d, g, uu, uOrig, aIn;
     double u0, f, tls, tprev, maxF, gd, err, gamma, r, rtol, shift;
@@ -1001,7 +1059,7 @@ struct Cell {
         err = 0.0;
         gamma = 0.0;
         r = 0.0;
-        rtol = P.unbatched ? LaneVec<NM>::dnrm2(lane < NM ? uOrig : 0.0) * P.tol : P.tol;
+        rtol = P.unbatched ? LaneVec<NM, G>::dnrm2(lane < NM ? uOrig : 0.0) * P.tol : P.tol;
         shift = 0.0;
         iter = 0;
         tot = 0;
@@ -1040,7 +1098,7 @@ struct Cell {
         double fNew = fdir;
 #pragma unroll
         for (int i = 0; i < NM; ++i)
-            fNew = fNew - rdl(a, i) * rdl(uu, i);
+            fNew = fNew - Grp<G>::bc(a, i) * Grp<G>::bc(uu, i);
         return fNew <= f + 1e-3 * tls * gd;
     }
 
@@ -1073,7 +1131,7 @@ struct Cell {
             }
         }
         // ================================ one Newton iteration (solveTheRest)
-        Dense<NM, EXACT> D;
+        Dense<NM, EXACT, G> D;
 #pragma unroll
         for (int j = 0; j < NM; ++j)
             D.row[j] = own ? H[lane * NM + j] : 0.0;
@@ -1087,10 +1145,10 @@ struct Cell {
         if (mx)  // f from the FP64 sum, as the line search evaluates it (fobj.cpp:135-164)
             f = fdir;
         else
-            f = rdl(D.row[0], 0) / (P.m0 * P.m0);
+            f = Grp<G>::bc(D.row[0], 0) / (P.m0 * P.m0);
 #pragma unroll
         for (int i = 0; i < NM; ++i)
-            f = f - rdl(a, i) * rdl(uu, i);
+            f = f - Grp<G>::bc(a, i) * Grp<G>::bc(uu, i);
         double rs = 0.0;  // utils_norm1
 #pragma unroll
         for (int j = 0; j < NM; ++j)
@@ -1098,7 +1156,7 @@ struct Cell {
         double norm1 = 0.0;
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            double vv = rdl(rs, i);
+            double vv = Grp<G>::bc(rs, i);
             if (vv > norm1)
                 norm1 = vv;
         }
@@ -1120,7 +1178,7 @@ struct Cell {
             a0 = (lane == 0) ? P.isoAlpha0 : 0.0;  // isotropic restart, optBatch.cpp:208-218
             a = a0;
             if (P.unbatched)
-                rtol = LaneVec<NM>::dnrm2(uu) * P.tol;
+                rtol = LaneVec<NM, G>::dnrm2(uu) * P.tol;
             iter = 0;
             igam = 0;
             tprev = 1.0;
@@ -1134,7 +1192,7 @@ struct Cell {
                 r = kReg[reg];
                 uu = (lane == 0) ? 1.0 : (own ? uOrig * (1.0 - r) : 0.0);
                 if (P.unbatched)
-                    rtol = LaneVec<NM>::dnrm2(uu) * P.tol;
+                    rtol = LaneVec<NM, G>::dnrm2(uu) * P.tol;
                 a = a0;
                 igam = 0;
                 iter = 0;
@@ -1152,11 +1210,11 @@ struct Cell {
         else
             D.potrs_custom(d, lane);
         d = own ? -d : 0.0;
-        err = LaneVec<NM>::dnrm2(g);
+        err = LaneVec<NM, G>::dnrm2(g);
         double da1 = 0.0;
 #pragma unroll
         for (int i = 0; i < NM; ++i)
-            da1 = da1 + fabs(rdl(d, i));
+            da1 = da1 + fabs(Grp<G>::bc(d, i));
         gamma = det_exp(2.0 * da1 * P.minf);
         if (mx) {
             // the stopping test belongs to the FP64 path: hand the cell over once its
@@ -1173,14 +1231,14 @@ struct Cell {
         maxF = 0.0;
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            double temp = fabs(rdl(d, i)) / (fabs(rdl(aOld, i)) + 1e-100);
+            double temp = fabs(Grp<G>::bc(d, i)) / (fabs(Grp<G>::bc(aOld, i)) + 1e-100);
             if (temp > maxF)
                 maxF = temp;
         }
         gd = 0.0;
 #pragma unroll
         for (int i = 0; i < NM; ++i)
-            gd = gd + rdl(g, i) * rdl(d, i);
+            gd = gd + Grp<G>::bc(g, i) * Grp<G>::bc(d, i);
         tls = 1.0;
         if (tls * maxF > DBL_EPSILON / 2) {
             a = own ? aOld + tls * d : 0.0;
@@ -1195,15 +1253,21 @@ struct Cell {
 };
 
 // ---------------------------------------------------------------- step pass
+// One state-machine step per listed cell.  Orders up to M_3 (NM <= 16) take four cells per
+// wave, one per 16-lane row, so that the serial Cholesky / solve / condition-estimate chains
+// of four cells run side by side; larger orders take one cell per wave.  Two waves per SIMD
+// (at most 256 registers a lane) measured faster than one wave with no spills.
 template <int NM, bool EXACT>
-__global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, Lists Lc, Lists Ln,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_step(KParams P, State S, Lists Lc, Lists Ln,
                                                   double *u_g, double *alpha_g,
                                                   c3d_cell_stats *stats, int *status,
                                                   unsigned long long *sweepCount)
 {
+    constexpr int G = NM <= 16 ? 16 : kWave;
+    constexpr int kCellsPerBlock = kBlock / G;
     constexpr int LD = NM + 1;
-    __shared__ double sT[kWavesPerBlock][NM * LD];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __shared__ double sT[kCellsPerBlock][NM * LD];
+    const int lane = threadIdx.x & (G - 1), grp = threadIdx.x / G;
     const int nWm = uniform_int(*Lc.nm);
     const int nW = nWm + uniform_int(*Lc.nf);
     if (blockIdx.x == 0 && threadIdx.x == 0 && sweepCount) {  // [0] FP64 sweeps, [2] mixed
@@ -1211,15 +1275,17 @@ __global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, Lists Lc, L
         atomicAdd(sweepCount + 2, (unsigned long long)nWm);
     }
     C3D_BLOCK_APPENDER(app);
-    const int nBlocksNeeded = (nW + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int nBlocksNeeded = (nW + kCellsPerBlock - 1) / kCellsPerBlock;
     for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
-        const int slot = blk * kWavesPerBlock + wv;
+        const int slot = blk * kCellsPerBlock + grp;
         if (slot < nW) {
-            const int c = uniform_int(list_cell(Lc, slot, nWm));
-            Cell<NM> C;
+            int c = list_cell(Lc, slot, nWm);
+            if constexpr (G == kWave)
+                c = uniform_int(c);
+            Cell<NM, G> C;
             C.load(S, c, lane);
             const int done = C.template advance<EXACT>(P, S.H + (size_t)c * NM * NM, S.fdir[c],
-                                                       sT[wv], lane);
+                                                       sT[grp], lane);
             if (done) {
                 C.finish(done, c, u_g, alpha_g, stats, status, lane);
             } else {
@@ -1228,7 +1294,7 @@ __global__ __launch_bounds__(kBlock) void k_step(KParams P, State S, Lists Lc, L
                     app.push(C.mixed(P), c);
             }
         }
-        app.flush(Ln, blk + gridDim.x >= nBlocksNeeded, kWavesPerBlock);
+        app.flush(Ln, blk + gridDim.x >= nBlocksNeeded, kCellsPerBlock);
     }
 }
 
