@@ -76,6 +76,7 @@ SIGNATURES = [
     ("c3d_set_profiling", C.c_int, [_vp, C.c_int]),
     ("c3d_get_closure_profile", C.c_int, [_vp, C.POINTER(ClosureProfile)]),
     ("c3d_set_tail", C.c_int, [_vp, _i32, _i32]),
+    ("c3d_set_tail_wide", C.c_int, [_vp, _i32]),
     ("c3d_set_precision", C.c_int, [_vp, C.c_int]),
     ("c3d_set_mixed", C.c_int, [_vp, _d, C.c_int, _d]),
     ("c3d_set_pass_budget", C.c_int, [_vp, C.c_int]),
@@ -214,8 +215,10 @@ class Ctx:
     def set_pass_budget(self, n):
         return check("c3d_set_pass_budget", self.L.c3d_set_pass_budget(self.h, n))
 
-    def set_tail(self, max_cells, passes_per_launch=32):
-        return check("c3d_set_tail", self.L.c3d_set_tail(self.h, max_cells, passes_per_launch))
+    def set_tail(self, max_cells, passes_per_launch=32, wide_cells=None):
+        check("c3d_set_tail", self.L.c3d_set_tail(self.h, max_cells, passes_per_launch))
+        if wide_cells is not None:
+            check("c3d_set_tail_wide", self.L.c3d_set_tail_wide(self.h, wide_cells))
 
     def hessian(self, alpha, hs, stream=None):
         n = alpha.numel() // self.nm

@@ -91,6 +91,7 @@ struct c3d_ctx_s {
     // fused tail: taken once at most tailCells cells remain listed, tailPasses per launch
     int tailCells = 8192;
     int tailPasses = 32;
+    int tailWideCells = 512;  // four waves per cell (speculative line-search trials) below this
     std::mutex mtx;
 };
 

@@ -309,6 +309,14 @@ int c3d_set_tail(c3d_ctx ctx, int max_cells, int passes_per_launch)
     return C3D_OK;
 }
 
+int c3d_set_tail_wide(c3d_ctx ctx, int max_cells)
+{
+    if (!ctx || max_cells < 0)
+        return C3D_ERR_ARG;
+    ctx->tailWideCells = max_cells;
+    return C3D_OK;
+}
+
 int c3d_set_precision(c3d_ctx ctx, int mode)
 {
     if (!ctx || (mode != C3D_PREC_FP64 && mode != C3D_PREC_FP64_REFORDER &&

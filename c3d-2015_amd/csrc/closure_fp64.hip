@@ -1333,29 +1333,36 @@ __device__ __forceinline__ double f_only(const KParams &P, const double *__restr
 }
 
 // ---------------------------------------------------------------- fused tail pass
-// For the last few cells: one wave per cell runs up to `passes` sweep+step rounds in one
-// launch, with the cell's state in registers, its alpha, M chunks, H and the transposed
-// factor in the wave's own LDS.  The sweep performs the same operations in the same order as
-// k_sweep / k_sweep_ref / k_sweep_mixed do for that cell, so a cell's result does not depend
-// on which kernel finished it.  A line-search trial evaluates f alone first and sweeps H only
-// when the trial is accepted.  Launch: one 64-thread block per listed cell (grid >= *nWp).
-template <int NM, bool REF, bool MIXED>
-__global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restrict__ Mg,
-                                                 const double *__restrict__ wg, State S,
-                                                 Lists Lc, Lists Ln, double *u_g, double *alpha_g,
-                                                 c3d_cell_stats *stats, int *status, int passes,
-                                                 unsigned long long *sweepCount)
+// For the last few cells: one block per cell runs up to `passes` rounds in one launch, with
+// the cell's state in registers and its trial points, M chunks, H and the transposed factor
+// in LDS.  The sweep performs the same operations in the same order as k_sweep /
+// k_sweep_ref / k_sweep_mixed do for that cell, so a cell's result does not depend on which
+// kernel finished it.  A line-search trial needs f alone (H only once a trial is accepted).
+// With TW waves per block, wave w evaluates f at the step the search reaches after w more
+// rejections (tls / 2^w), so one round settles up to TW trials of the sequential search.
+// Every wave keeps an identical copy of the state machine (same inputs, same operations),
+// which keeps control flow uniform over the block; wave 0 sweeps H and writes the results.
+// Launch: TW * 64 threads per block, one block per listed cell (grid >= *nWp).
+template <int NM, bool REF, bool MIXED, int TW>
+__global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__restrict__ Mg,
+                                                      const double *__restrict__ wg, State S,
+                                                      Lists Lc, Lists Ln, double *u_g,
+                                                      double *alpha_g, c3d_cell_stats *stats,
+                                                      int *status, int passes,
+                                                      unsigned long long *sweepCount)
 {
     constexpr int NT = (NM + 15) / 16;
     constexpr int NB = NT * (NT + 1) / 2;
     constexpr int NMS = NM | 1;
     static_assert(!(REF && MIXED), "one arithmetic mode");
-    static_assert(NM * NM + NM * (NM + 1) <= kChunk * NMS, "H and factor alias the M chunk");
     __shared__ double sM[kChunk * NMS];
-    __shared__ double sW[kChunk];
-    __shared__ double sE[kChunk];
-    __shared__ double sA[NM];
-    const int lane = threadIdx.x;
+    __shared__ double sH[NM * NM];
+    __shared__ double sT[TW][NM * (NM + 1)];
+    __shared__ double sW[TW][kChunk];
+    __shared__ double sE[TW][kChunk];
+    __shared__ double sA[TW][NM];
+    __shared__ double sF[TW + 1];  // f of each wave's trial; [TW]: f of the sweep
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int nWm = uniform_int(*Lc.nm);
     const int nW = nWm + uniform_int(*Lc.nf);
     const int slot = blockIdx.x;
@@ -1364,102 +1371,123 @@ __global__ __launch_bounds__(kWave) void k_tail(KParams P, const double *__restr
     const int c = uniform_int(list_cell(Lc, slot, nWm));
     Cell<NM> C;
     C.load(S, c, lane);
-    double *sH = sM;  // valid between a sweep and the next one
-    double *sT = sM + NM * NM;
     const int nch = P.nqPad / kChunk;
     int done = 0;
     int sweeps = 0;
     for (int p = 0; p < passes; ++p) {
         const bool mx = MIXED && C.mixed(P);
-        if (lane < NM)
-            sA[lane] = C.a;
-        if (C.phase == kPhaseLS) {  // a trial first needs f only; H only if it is accepted
+        if (C.phase == kPhaseLS) {
+            // wave w: alpha at the step after w more halvings, formed as advance() forms it
+            double tw = C.tls;
+            for (int k = 0; k < wv; ++k)
+                tw = 0.5 * tw;
+            if (lane < NM)
+                sA[wv][lane] = (wv == 0) ? C.a : C.aOld + tw * C.d;
             wave_lds_sync();
-            const double ft = mx ? f_only_mixed<NM>(P, Mg, wg, sA, C.shift, lane)
-                                 : f_only<NM, REF>(P, Mg, wg, sA, sW, sE, lane);
-            if (!C.ls_accepts(ft)) {
-                done = C.template advance<REF>(P, nullptr, ft, sT, lane);
-                wave_lds_sync();
-                if (done)
+            const double ft = mx ? f_only_mixed<NM>(P, Mg, wg, sA[wv], C.shift, lane)
+                                 : f_only<NM, REF>(P, Mg, wg, sA[wv], sW[wv], sE[wv], lane);
+            if (lane == 0)
+                sF[wv] = ft;
+            __syncthreads();
+            bool accepted = false;
+            for (int k = 0; k < TW; ++k) {
+                const double fk = sF[k];
+                if (C.ls_accepts(fk)) {
+                    accepted = true;
                     break;
+                }
+                C.template advance<REF>(P, nullptr, fk, sT[wv], lane);  // halve, or give up
+                if (C.phase != kPhaseLS)
+                    break;
+            }
+            __syncthreads();  // sF and sA are rewritten by the next round
+            if (!accepted)
                 continue;
-            }
         }
-        double fdir;
-        if constexpr (REF) {
-            const RefEntries<NM> E(lane);
-            double h[RefEntries<NM>::NS];
+        if (wv == 0) {
+            double fdir;
+            if (lane < NM)
+                sA[0][lane] = C.a;
+            if constexpr (REF) {
+                const RefEntries<NM> E(lane);
+                double h[RefEntries<NM>::NS];
 #pragma unroll
-            for (int s2 = 0; s2 < RefEntries<NM>::NS; ++s2)
-                h[s2] = 0.0;
-            double f = 0.0;
-            for (int ch = 0; ch < nch; ++ch) {
-                wave_lds_sync();
-                for (int e = lane; e < kChunk * NM; e += kWave) {
-                    const int rr = e / NM, cc = e - rr * NM;
-                    sM[rr * NMS + cc] = Mg[(size_t)ch * kChunk * NM + e];
+                for (int s2 = 0; s2 < RefEntries<NM>::NS; ++s2)
+                    h[s2] = 0.0;
+                double f = 0.0;
+                for (int ch = 0; ch < nch; ++ch) {
+                    wave_lds_sync();
+                    for (int e = lane; e < kChunk * NM; e += kWave) {
+                        const int rr = e / NM, cc = e - rr * NM;
+                        sM[rr * NMS + cc] = Mg[(size_t)ch * kChunk * NM + e];
+                    }
+                    sW[0][lane] = wg[(size_t)ch * kChunk + lane];
+                    wave_lds_sync();
+                    chunk_ref<NM>(sM, sW[0], sE[0], sA[0], E, h, f,
+                                  min(kChunk, P.nq - ch * kChunk), lane);
                 }
-                sW[lane] = wg[(size_t)ch * kChunk + lane];
                 wave_lds_sync();
-                chunk_ref<NM>(sM, sW, sE, sA, E, h, f, min(kChunk, P.nq - ch * kChunk), lane);
-            }
-            wave_lds_sync();
-            store_H_ref<NM>(E, h, sH, lane);
-            fdir = f;
-        } else {
-            d4 acc[1][NB];
-            f4 acc32[1][NB];
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                acc[0][b] = d4{0.0, 0.0, 0.0, 0.0};
-                acc32[0][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-            }
-            double fpart[1] = {0.0};
-            const double shift[1] = {C.shift};
-            double pre[NM];
-            double prew = wg[lane];
-#pragma unroll
-            for (int k = 0; k < NM; ++k)
-                pre[k] = Mg[lane + kWave * k];
-            for (int ch = 0; ch < nch; ++ch) {
-                wave_lds_sync();
-#pragma unroll
-                for (int k = 0; k < NM; ++k) {
-                    const int e = lane + kWave * k;
-                    const int rr = e / NM, cc = e - rr * NM;
-                    sM[rr * NMS + cc] = pre[k];
-                }
-                const double wl = prew;
-                wave_lds_sync();
-                if (ch + 1 < nch) {
-                    const size_t base = (size_t)(ch + 1) * kChunk;
-                    prew = wg[base + lane];
-#pragma unroll
-                    for (int k = 0; k < NM; ++k)
-                        pre[k] = Mg[base * NM + lane + kWave * k];
-                }
-                if (mx)
-                    chunk_mixed<NM, 1>(sM, wl, sA, shift, acc32, fpart, lane);
-                else
-                    chunk_mfma<NM, 1>(sM, wl, sA, acc, fpart, lane);
-            }
-            wave_lds_sync();
-            if (mx) {
-                const double es = exp(C.shift);
-                fdir = wave_sum(fpart[0]) * es;
-                store_H32<NM>(acc32[0], es, sH, lane);
+                store_H_ref<NM>(E, h, sH, lane);
+                fdir = f;
             } else {
-                fdir = wave_sum(fpart[0]);
-                store_H<NM>(acc[0], sH, lane);
+                d4 acc[1][NB];
+                f4 acc32[1][NB];
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    acc[0][b] = d4{0.0, 0.0, 0.0, 0.0};
+                    acc32[0][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+                }
+                double fpart[1] = {0.0};
+                const double shift[1] = {C.shift};
+                double pre[NM];
+                double prew = wg[lane];
+#pragma unroll
+                for (int k = 0; k < NM; ++k)
+                    pre[k] = Mg[lane + kWave * k];
+                for (int ch = 0; ch < nch; ++ch) {
+                    wave_lds_sync();
+#pragma unroll
+                    for (int k = 0; k < NM; ++k) {
+                        const int e = lane + kWave * k;
+                        const int rr = e / NM, cc = e - rr * NM;
+                        sM[rr * NMS + cc] = pre[k];
+                    }
+                    const double wl = prew;
+                    wave_lds_sync();
+                    if (ch + 1 < nch) {
+                        const size_t base = (size_t)(ch + 1) * kChunk;
+                        prew = wg[base + lane];
+#pragma unroll
+                        for (int k = 0; k < NM; ++k)
+                            pre[k] = Mg[base * NM + lane + kWave * k];
+                    }
+                    if (mx)
+                        chunk_mixed<NM, 1>(sM, wl, sA[0], shift, acc32, fpart, lane);
+                    else
+                        chunk_mfma<NM, 1>(sM, wl, sA[0], acc, fpart, lane);
+                }
+                wave_lds_sync();
+                if (mx) {
+                    const double es = exp(C.shift);
+                    fdir = wave_sum(fpart[0]) * es;
+                    store_H32<NM>(acc32[0], es, sH, lane);
+                } else {
+                    fdir = wave_sum(fpart[0]);
+                    store_H<NM>(acc[0], sH, lane);
+                }
             }
+            if (lane == 0)
+                sF[TW] = fdir;
+            ++sweeps;
         }
-        wave_lds_sync();
-        ++sweeps;
-        done = C.template advance<REF>(P, sH, fdir, sT, lane);
-        wave_lds_sync();
+        __syncthreads();
+        done = C.template advance<REF>(P, sH, sF[TW], sT[wv], lane);
+        __syncthreads();  // sH and sF are rewritten by the next round
         if (done)
             break;
     }
+    if (wv != 0)
+        return;
     if (lane == 0 && sweepCount)
         atomicAdd(sweepCount, (unsigned long long)sweeps);
     if (done) {
@@ -1874,15 +1902,24 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
                 return hipErrorOutOfMemory;
             if (ev)
                 (void)hipEventRecord(ev[0], s);
-#define C3D_TAIL(R, X)                                                                     \
-    hipLaunchKernelGGL((k_tail<NM, R, X>), dim3((unsigned)left), dim3(kWave), 0, s, P, ctx->d_M, \
-                       ctx->d_wpad, S, Lc, Ln, u, alpha, stats, ctx->d_status, K, cntTail)
-            if (ref)
-                C3D_TAIL(true, false);
+#define C3D_TAIL(R, X, TW)                                                                 \
+    hipLaunchKernelGGL((k_tail<NM, R, X, TW>), dim3((unsigned)left), dim3(TW * kWave), 0, s, P, \
+                       ctx->d_M, ctx->d_wpad, S, Lc, Ln, u, alpha, stats, ctx->d_status, K,   \
+                       cntTail)
+            // few cells: four waves per cell settle up to four line-search trials a round
+            const bool wide = left <= ctx->tailWideCells;
+            if (ref && wide)
+                C3D_TAIL(true, false, 4);
+            else if (ref)
+                C3D_TAIL(true, false, 1);
+            else if (mixed && wide)
+                C3D_TAIL(false, true, 4);
             else if (mixed)
-                C3D_TAIL(false, true);
+                C3D_TAIL(false, true, 1);
+            else if (wide)
+                C3D_TAIL(false, false, 4);
             else
-                C3D_TAIL(false, false);
+                C3D_TAIL(false, false, 1);
 #undef C3D_TAIL
             if (ev)
                 (void)hipEventRecord(ev[1], s);

@@ -27,17 +27,20 @@ def _tables(O, N, Q):
 
 
 # worklist strategies of c3d_closure: launch pairs only, the fused tail kernel only (few
-# passes per launch, so cells resume across launches), and a switch part way through
-TAILS = ("bulk", "fused", "switch")
+# passes per launch, so cells resume across launches) in its four-wave form (speculative
+# line-search trials) or its one-wave form, and switches part way through
+TAILS = ("bulk", "fused", "fused1", "switch")
 
 
 def _set_tail(ctx, tail, n):
     if tail == "bulk":
         ctx.set_tail(0)
     elif tail == "fused":
-        ctx.set_tail(1 << 30, 7)
+        ctx.set_tail(1 << 30, 7, wide_cells=1 << 30)
+    elif tail == "fused1":
+        ctx.set_tail(1 << 30, 7, wide_cells=0)
     elif tail == "switch":
-        ctx.set_tail(max(1, n // 5), 3)
+        ctx.set_tail(max(1, n // 5), 3, wide_cells=max(1, n // 10))
 
 
 def _run_gpu_closure(P, dev, T, u, a, opts, precision=0, tail=None):
