@@ -62,6 +62,7 @@ struct c3d_ctx_s {
     // device tables
     double *d_w = nullptr, *d_xi = nullptr, *d_eta = nullptr, *d_mu = nullptr;
     double *d_M = nullptr;     // [nqPad][nm], rows >= nq zero
+    double *d_MT = nullptr;    // [nm][nqPad], the transpose of d_M
     double *d_wpad = nullptr;  // [nqPad], entries >= nq zero
     double *d_M2 = nullptr;    // [nqPad][nm2]
     double *d_beta = nullptr, *d_betaVal = nullptr, *d_pnOps = nullptr;

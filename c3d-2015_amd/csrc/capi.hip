@@ -63,6 +63,13 @@ int finish_ctx(c3d_ctx_s *ctx)
     UP(d_eta, t.eta);
     UP(d_mu, t.mu);
     UP(d_M, Mp);
+    {
+        std::vector<double> MTp((size_t)ctx->nqPad * t.nm, 0.0);
+        for (int q = 0; q < t.nq; ++q)
+            for (int i = 0; i < t.nm; ++i)
+                MTp[(size_t)i * ctx->nqPad + q] = t.M[(size_t)q * t.nm + i];
+        UP(d_MT, MTp);
+    }
     UP(d_wpad, wp);
     if (!t.M2.empty()) {
         std::vector<double> M2p((size_t)ctx->nqPad * t.nm2, 0.0);
@@ -158,8 +165,8 @@ int c3d_destroy(c3d_ctx ctx)
 {
     if (!ctx)
         return C3D_OK;
-    void *ptrs[] = {ctx->d_w, ctx->d_xi, ctx->d_eta, ctx->d_mu, ctx->d_M, ctx->d_wpad,
-                    ctx->d_M2, ctx->d_beta, ctx->d_betaVal, ctx->d_pnOps, ctx->d_betaColInd,
+    void *ptrs[] = {ctx->d_w, ctx->d_xi, ctx->d_eta, ctx->d_mu, ctx->d_M, ctx->d_MT,
+                    ctx->d_wpad, ctx->d_M2, ctx->d_beta, ctx->d_betaVal, ctx->d_pnOps, ctx->d_betaColInd,
                     ctx->d_betaRowPtr, ctx->d_sweepCount, ctx->d_status, ctx->ansatz.p,
                     ctx->uOld.p, ctx->flux.p, ctx->closureState.p};
     for (void *p : ptrs)

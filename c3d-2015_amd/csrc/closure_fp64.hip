@@ -852,24 +852,6 @@ __device__ __forceinline__ void chunk_mixed(const double *sMc, double wl, const 
     }
 }
 
-// f alone with chunk_mixed's operations (for line-search trials in the tail kernel)
-template <int NM>
-__device__ __forceinline__ double f_only_mixed(const KParams &P, const double *__restrict__ Mg,
-                                               const double *__restrict__ wg, const double *sA,
-                                               double shift, int lane)
-{
-    double fpart = 0.0;
-    for (int ch = 0; ch < P.nqPad / kChunk; ++ch) {
-        const size_t q = (size_t)ch * kChunk + lane;
-        double x = 0.0;
-#pragma unroll
-        for (int i = 0; i < NM; ++i)
-            x = fma(sA[i], Mg[q * NM + i], x);
-        fpart = fpart + wg[q] * exp(x - shift);
-    }
-    return wave_sum(fpart) * exp(shift);
-}
-
 // H = e^shift * H32, row-major, lower triangle mirrored
 template <int NM>
 __device__ __forceinline__ void store_H32(const f4 *acc, double es, double *H, int lane)
@@ -1298,38 +1280,90 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     }
 }
 
-// sum_q w_q exp(alpha.m_q) alone for one cell (one wave; alpha in LDS), with exactly the
-// operations the sweeps use for it: chunk_mfma's per-lane partials and final wave_sum, or
-// chunk_ref's ordinate-order sum.  A rejected line-search trial needs nothing else.
-template <int NM, bool REF>
-__device__ __forceinline__ double f_only(const KParams &P, const double *__restrict__ Mg,
+// f alone for one cell at kw <= KW trial points (one wave; the points in LDS, KW x NM), for
+// line-search trials in the tail kernel: sum_q w_q exp(alpha.m_q) with exactly the
+// operations the sweep kernels use for that sum, so that a cell's result does not depend on
+// which kernel evaluated it -- kFp64: chunk_mfma's exponent, per-lane partials over the
+// chunks and final wave_sum; kFRef: chunk_ref's deterministic exp and ordinate-order sum
+// (one point); kFMixed: chunk_mixed's fused exponent, shifted exp and e^shift rescale.  M is
+// read from the transposed table MT [NM][nqPad] (one coalesced load per component), two
+// chunks in flight, each chunk's M shared by the kw points.
+enum : int { kFp64 = 0, kFRef = 1, kFMixed = 2 };
+template <int NM, int MODE, int KW>
+__device__ __forceinline__ void f_trials(const KParams &P, const double *__restrict__ MT,
                                          const double *__restrict__ wg, const double *sA,
-                                         double *sW, double *sE, int lane)
+                                         int kw, double shift, double *sW, double *sE, int lane,
+                                         double (&out)[KW])
 {
+    static_assert(MODE != kFRef || KW == 1, "reference order: one point per wave");
     const int nch = P.nqPad / kChunk;
-    double fpart = 0.0;
-    for (int ch = 0; ch < nch; ++ch) {
+    const size_t ld = (size_t)P.nqPad;
+    double mA[NM], mB[NM], wA = 0.0, wB = 0.0;
+    auto load = [&](double(&m)[NM], double &w, int ch) {
         const size_t q = (size_t)ch * kChunk + lane;
-        double x = 0.0;
 #pragma unroll
         for (int i = 0; i < NM; ++i)
-            x = x + sA[i] * Mg[q * NM + i];
-        if constexpr (REF) {
-            wave_lds_sync();
-            sE[lane] = det_exp(x);
-            sW[lane] = wg[q];
-            wave_lds_sync();
-            const int nqc = min(kChunk, P.nq - ch * kChunk);
-            for (int k = 0; k < nqc; ++k)
-                fpart = fpart + sW[k] * sE[k];
-        } else {
-            fpart = fpart + wg[q] * exp(x);
+            m[i] = MT[i * ld + q];
+        w = wg[q];
+    };
+    double fpart[KW];
+#pragma unroll
+    for (int j = 0; j < KW; ++j)
+        fpart[j] = 0.0;
+    auto term = [&](const double(&m)[NM], double w, int ch) {
+#pragma unroll
+        for (int j = 0; j < KW; ++j) {
+            if (j >= kw)
+                break;
+            const double *a = sA + j * NM;
+            double x = 0.0;
+            if constexpr (MODE == kFMixed) {
+#pragma unroll
+                for (int i = 0; i < NM; ++i)
+                    x = fma(a[i], m[i], x);
+                fpart[j] = fpart[j] + w * exp(x - shift);
+            } else {
+#pragma unroll
+                for (int i = 0; i < NM; ++i)
+                    x = x + a[i] * m[i];
+                if constexpr (MODE == kFRef) {
+                    wave_lds_sync();
+                    sE[lane] = det_exp(x);
+                    sW[lane] = w;
+                    wave_lds_sync();
+                    const int nqc = min(kChunk, P.nq - ch * kChunk);
+                    for (int k = 0; k < nqc; ++k)
+                        fpart[j] = fpart[j] + sW[k] * sE[k];
+                } else {
+                    fpart[j] = fpart[j] + w * exp(x);
+                }
+            }
+        }
+    };
+    load(mA, wA, 0);
+    if (nch > 1)
+        load(mB, wB, 1);
+    for (int ch = 0; ch < nch; ch += 2) {
+        term(mA, wA, ch);
+        if (ch + 2 < nch)
+            load(mA, wA, ch + 2);
+        if (ch + 1 < nch) {
+            term(mB, wB, ch + 1);
+            if (ch + 3 < nch)
+                load(mB, wB, ch + 3);
         }
     }
-    if constexpr (REF)
-        return fpart;
-    else
-        return wave_sum(fpart);
+#pragma unroll
+    for (int j = 0; j < KW; ++j) {
+        if (j >= kw)
+            break;
+        if constexpr (MODE == kFRef)
+            out[j] = fpart[j];
+        else if constexpr (MODE == kFMixed)
+            out[j] = wave_sum(fpart[j]) * exp(shift);
+        else
+            out[j] = wave_sum(fpart[j]);
+    }
 }
 
 // ---------------------------------------------------------------- fused tail pass
@@ -1338,13 +1372,15 @@ __device__ __forceinline__ double f_only(const KParams &P, const double *__restr
 // in LDS.  The sweep performs the same operations in the same order as k_sweep /
 // k_sweep_ref / k_sweep_mixed do for that cell, so a cell's result does not depend on which
 // kernel finished it.  A line-search trial needs f alone (H only once a trial is accepted).
-// With TW waves per block, wave w evaluates f at the step the search reaches after w more
-// rejections (tls / 2^w), so one round settles up to TW trials of the sequential search.
-// Every wave keeps an identical copy of the state machine (same inputs, same operations),
-// which keeps control flow uniform over the block; wave 0 sweeps H and writes the results.
-// Launch: TW * 64 threads per block, one block per listed cell (grid >= *nWp).
+// The TW waves of a block evaluate f at the next trial points of the backtracking search
+// (tls, tls/2, tls/4, ...: one per wave, or KW per wave once the search has backtracked), so
+// one round settles up to TW * KW trials of the sequential search.  Every wave keeps an
+// identical copy of the state machine (same inputs, same operations), which keeps control
+// flow uniform over the block; wave 0 sweeps H and writes the results.  Launch: TW * 64
+// threads per block, one block per listed cell (grid >= *nWp).
 template <int NM, bool REF, bool MIXED, int TW>
 __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__restrict__ Mg,
+                                                      const double *__restrict__ MT,
                                                       const double *__restrict__ wg, State S,
                                                       Lists Lc, Lists Ln, double *u_g,
                                                       double *alpha_g, c3d_cell_stats *stats,
@@ -1354,14 +1390,17 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     constexpr int NT = (NM + 15) / 16;
     constexpr int NB = NT * (NT + 1) / 2;
     constexpr int NMS = NM | 1;
+    // trial points per wave in a wide round: as many as keep a round's latency near one f
+    // evaluation's (f costs nq (NM + exp) per point)
+    constexpr int KW = REF ? 1 : (NM <= 9 ? 8 : (NM <= 16 ? 2 : 1));
     static_assert(!(REF && MIXED), "one arithmetic mode");
     __shared__ double sM[kChunk * NMS];
     __shared__ double sH[NM * NM];
     __shared__ double sT[TW][NM * (NM + 1)];
     __shared__ double sW[TW][kChunk];
     __shared__ double sE[TW][kChunk];
-    __shared__ double sA[TW][NM];
-    __shared__ double sF[TW + 1];  // f of each wave's trial; [TW]: f of the sweep
+    __shared__ double sA[TW][KW * NM];
+    __shared__ double sF[TW * KW + 1];  // f at each trial point; [TW * KW]: f of the sweep
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int nWm = uniform_int(*Lc.nm);
     const int nW = nWm + uniform_int(*Lc.nf);
@@ -1377,20 +1416,36 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     for (int p = 0; p < passes; ++p) {
         const bool mx = MIXED && C.mixed(P);
         if (C.phase == kPhaseLS) {
-            // wave w: alpha at the step after w more halvings, formed as advance() forms it
-            double tw = C.tls;
-            for (int k = 0; k < wv; ++k)
-                tw = 0.5 * tw;
-            if (lane < NM)
-                sA[wv][lane] = (wv == 0) ? C.a : C.aOld + tw * C.d;
+            // points k = wv * kw + j at tls / 2^k, formed as advance() forms them
+            const int kw = C.tls < 1.0 ? KW : 1;
+            double tk = C.tls;
+            for (int k = 0; k < wv * kw; ++k)
+                tk = 0.5 * tk;
+#pragma unroll
+            for (int j = 0; j < KW; ++j) {
+                if (j >= kw)
+                    break;
+                if (lane < NM)
+                    sA[wv][j * NM + lane] = (wv == 0 && j == 0) ? C.a : C.aOld + tk * C.d;
+                tk = 0.5 * tk;
+            }
             wave_lds_sync();
-            const double ft = mx ? f_only_mixed<NM>(P, Mg, wg, sA[wv], C.shift, lane)
-                                 : f_only<NM, REF>(P, Mg, wg, sA[wv], sW[wv], sE[wv], lane);
-            if (lane == 0)
-                sF[wv] = ft;
+            double ft[KW];
+            if (mx)
+                f_trials<NM, kFMixed, KW>(P, MT, wg, sA[wv], kw, C.shift, sW[wv], sE[wv], lane,
+                                          ft);
+            else
+                f_trials<NM, REF ? kFRef : kFp64, KW>(P, MT, wg, sA[wv], kw, 0.0, sW[wv],
+                                                      sE[wv], lane, ft);
+            if (lane == 0) {
+#pragma unroll
+                for (int j = 0; j < KW; ++j)
+                    if (j < kw)
+                        sF[wv * kw + j] = ft[j];
+            }
             __syncthreads();
             bool accepted = false;
-            for (int k = 0; k < TW; ++k) {
+            for (int k = 0; k < TW * kw; ++k) {
                 const double fk = sF[k];
                 if (C.ls_accepts(fk)) {
                     accepted = true;
@@ -1477,11 +1532,11 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 }
             }
             if (lane == 0)
-                sF[TW] = fdir;
+                sF[TW * KW] = fdir;
             ++sweeps;
         }
         __syncthreads();
-        done = C.template advance<REF>(P, sH, sF[TW], sT[wv], lane);
+        done = C.template advance<REF>(P, sH, sF[TW * KW], sT[wv], lane);
         __syncthreads();  // sH and sF are rewritten by the next round
         if (done)
             break;
@@ -1904,7 +1959,8 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
                 (void)hipEventRecord(ev[0], s);
 #define C3D_TAIL(R, X, TW)                                                                 \
     hipLaunchKernelGGL((k_tail<NM, R, X, TW>), dim3((unsigned)left), dim3(TW * kWave), 0, s, P, \
-                       ctx->d_M, ctx->d_wpad, S, Lc, Ln, u, alpha, stats, ctx->d_status, K,   \
+                       ctx->d_M, ctx->d_MT, ctx->d_wpad, S, Lc, Ln, u, alpha, stats,          \
+                       ctx->d_status, K,                                                      \
                        cntTail)
             // few cells: four waves per cell settle up to four line-search trials a round
             const bool wide = left <= ctx->tailWideCells;
