@@ -183,8 +183,9 @@ struct Grp<64> {
 template <int J>
 __device__ __forceinline__ double row_bcast(double v)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + J, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + J, 0xf, 0xf, false);
+    // every lane of an active row receives lane J's value, so no old value is needed
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x150 + J, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x150 + J, 0xf, 0xf, false);
     return __hiloint2double(hi, lo);
 }
 
@@ -253,15 +254,15 @@ struct LaneVec {
         double scale = 0.0, ssq = 1.0;
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            double xi = Grp<G>::bc(x, i);
+            const double xi = Grp<G>::bc(x, i);
+            const double a = fabs(xi);
+            const bool up = scale < a;
+            const double r = (up ? scale : a) / (up ? a : scale);  // one division either way
             if (xi != 0.0) {
-                double a = fabs(xi);
-                if (scale < a) {
-                    double r = scale / a;
+                if (up) {
                     ssq = 1.0 + ssq * (r * r);
                     scale = a;
                 } else {
-                    double r = a / scale;
                     ssq = ssq + r * r;
                 }
             }
@@ -1024,7 +1025,8 @@ struct Cell {
         }
     }
 
-    // Mixed mode hands the cell to the FP64 path: the state k_init set up, from the input.
+    // Mixed mode hands the cell to the FP64 path: the state k_init set up, from the input
+    // (rtol as for the batched semantics).
     __device__ __forceinline__ void restart_fp64(const KParams &P, int lane)
     {
         a = aIn;
@@ -1041,7 +1043,7 @@ struct Cell {
         err = 0.0;
         gamma = 0.0;
         r = 0.0;
-        rtol = P.unbatched ? LaneVec<NM, G>::dnrm2(lane < NM ? uOrig : 0.0) * P.tol : P.tol;
+        rtol = P.tol;  // the caller rescales it for the unbatched semantics
         shift = 0.0;
         iter = 0;
         tot = 0;
@@ -1084,6 +1086,27 @@ struct Cell {
         return fNew <= f + 1e-3 * tls * gd;
     }
 
+    // The next trial of the backtracking search at step tls, or its end (linesearch.cpp:46-54):
+    // alpha restored and tprev = 0, which the next iteration turns into a ladder step.
+    __device__ __forceinline__ void ls_next(int lane)
+    {
+        if (tls * maxF > DBL_EPSILON / 2) {
+            a = lane < NM ? aOld + tls * d : 0.0;
+            phase = kPhaseLS;
+        } else {
+            a = aOld;
+            tprev = 0.0;
+            phase = kPhaseH;
+        }
+    }
+
+    // A rejected line-search trial: halve the step (linesearch.cpp:75-78).
+    __device__ __forceinline__ void ls_reject(int lane)
+    {
+        tls = 0.5 * tls;
+        ls_next(lane);
+    }
+
     // One step of the state machine with the sweep result at the current alpha (H row-major,
     // in LDS or global memory; fdir = sum_q w_q e_q): either the Armijo test of a line-search
     // trial, or one Newton iteration of solveTheRest / optScaled.  sT: NM x (NM+1) doubles of
@@ -1100,15 +1123,7 @@ struct Cell {
             if (ls_accepts(fdir)) {
                 tprev = tls;  // accepted; H of this sweep belongs to the new alpha
             } else {
-                tls = 0.5 * tls;
-                if (tls * maxF > DBL_EPSILON / 2) {
-                    a = own ? aOld + tls * d : 0.0;
-                    phase = kPhaseLS;
-                } else {
-                    a = aOld;
-                    tprev = 0.0;
-                    phase = kPhaseH;
-                }
+                ls_reject(lane);
                 return 0;
             }
         }
@@ -1148,43 +1163,44 @@ struct Cell {
             D.build_columns(sT, LD, lane);
             condest = 1.0 / D.pocon(norm1, lane);
         }
+        bool restart = false;  // the iteration starts over; rtol follows the (new) u
         if (mx) {
             if (cholErr != 0 || !(condest <= P.mixedCondMax) || tprev == 0.0 ||
                 iter >= min(maxItNow, P.mixedMaxIter)) {
                 restart_fp64(P, lane);
-                return 0;
+                restart = true;
+            } else {
+                const double lf = log(fdir);  // shift of the sweeps from this iterate on
+                shift = (lf > -700.0 && lf < 700.0) ? lf : 0.0;
             }
-            const double lf = log(fdir);  // shift of the sweeps from this iterate on
-            shift = (lf > -700.0 && lf < 700.0) ? lf : 0.0;
         } else if (tot == 1 && (cholErr != 0 || condest > P.condHMax)) {
             a0 = (lane == 0) ? P.isoAlpha0 : 0.0;  // isotropic restart, optBatch.cpp:208-218
             a = a0;
-            if (P.unbatched)
-                rtol = LaneVec<NM, G>::dnrm2(uu) * P.tol;
             iter = 0;
             igam = 0;
             tprev = 1.0;
             flags |= C3D_FLAG_ISO_RESTART;
             phase = kPhaseH;
-            return 0;
+            restart = true;
+        } else if (iter >= maxItNow || cholErr != 0 || tprev == 0.0 || condest > P.condHMax) {
+            if (reg == kNumReg - 1)  // ladder, optBatch.cpp:222-248
+                return 2;
+            ++reg;
+            r = kReg[reg];
+            uu = (lane == 0) ? 1.0 : (own ? uOrig * (1.0 - r) : 0.0);
+            a = a0;
+            igam = 0;
+            iter = 0;
+            tprev = 1.0;
+            if (P.unbatched && reg == kNumReg - 1)
+                maxIt = 10 * maxIt;
+            phase = kPhaseH;
+            restart = true;
         }
-        if (!mx && (iter >= maxItNow || cholErr != 0 || tprev == 0.0 || condest > P.condHMax)) {
-            if (reg < kNumReg - 1) {  // ladder, optBatch.cpp:222-248
-                ++reg;
-                r = kReg[reg];
-                uu = (lane == 0) ? 1.0 : (own ? uOrig * (1.0 - r) : 0.0);
-                if (P.unbatched)
-                    rtol = LaneVec<NM, G>::dnrm2(uu) * P.tol;
-                a = a0;
-                igam = 0;
-                iter = 0;
-                tprev = 1.0;
-                if (P.unbatched && reg == kNumReg - 1)
-                    maxIt = 10 * maxIt;
-                phase = kPhaseH;
-                return 0;
-            }
-            return 2;
+        if (restart) {
+            if (P.unbatched)  // opt.cpp:195, 240, 270
+                rtol = LaneVec<NM, G>::dnrm2(uu) * P.tol;
+            return 0;
         }
         d = g;
         if (P.unbatched)
@@ -1211,9 +1227,10 @@ struct Cell {
         }
         aOld = a;  // linesearch.cpp:29-54
         maxF = 0.0;
+        const double ratio = fabs(d) / (fabs(aOld) + 1e-100);  // each lane its own component
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            double temp = fabs(Grp<G>::bc(d, i)) / (fabs(Grp<G>::bc(aOld, i)) + 1e-100);
+            const double temp = Grp<G>::bc(ratio, i);
             if (temp > maxF)
                 maxF = temp;
         }
@@ -1222,14 +1239,7 @@ struct Cell {
         for (int i = 0; i < NM; ++i)
             gd = gd + Grp<G>::bc(g, i) * Grp<G>::bc(d, i);
         tls = 1.0;
-        if (tls * maxF > DBL_EPSILON / 2) {
-            a = own ? aOld + tls * d : 0.0;
-            phase = kPhaseLS;
-        } else {
-            a = aOld;
-            tprev = 0.0;
-            phase = kPhaseH;
-        }
+        ls_next(lane);
         return 0;
     }
 };
@@ -1396,7 +1406,11 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     static_assert(!(REF && MIXED), "one arithmetic mode");
     __shared__ double sM[kChunk * NMS];
     __shared__ double sH[NM * NM];
-    __shared__ double sT[TW][NM * (NM + 1)];
+    // the state machine runs on 16-lane rows for NM <= 16 (DPP broadcasts; the four rows of a
+    // wave hold identical copies), else on the whole wave
+    constexpr int G = NM <= 16 ? 16 : kWave;
+    constexpr int RW = kWave / G;
+    __shared__ double sT[TW][RW][NM * (NM + 1)];
     __shared__ double sW[TW][kChunk];
     __shared__ double sE[TW][kChunk];
     __shared__ double sA[TW][KW * NM];
@@ -1408,8 +1422,9 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     if (slot >= nW)
         return;
     const int c = uniform_int(list_cell(Lc, slot, nWm));
-    Cell<NM> C;
-    C.load(S, c, lane);
+    const int li = lane & (G - 1), row = lane / G;
+    Cell<NM, G> C;
+    C.load(S, c, li);
     const int nch = P.nqPad / kChunk;
     int done = 0;
     int sweeps = 0;
@@ -1451,7 +1466,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                     accepted = true;
                     break;
                 }
-                C.template advance<REF>(P, nullptr, fk, sT[wv], lane);  // halve, or give up
+                C.ls_reject(li);  // halve the step, or give up
                 if (C.phase != kPhaseLS)
                     break;
             }
@@ -1536,19 +1551,19 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
             ++sweeps;
         }
         __syncthreads();
-        done = C.template advance<REF>(P, sH, sF[TW * KW], sT[wv], lane);
+        done = C.template advance<REF>(P, sH, sF[TW * KW], sT[wv][row], li);
         __syncthreads();  // sH and sF are rewritten by the next round
         if (done)
             break;
     }
-    if (wv != 0)
+    if (wv != 0 || row != 0)
         return;
     if (lane == 0 && sweepCount)
         atomicAdd(sweepCount, (unsigned long long)sweeps);
     if (done) {
-        C.finish(done, c, u_g, alpha_g, stats, status, lane);
+        C.finish(done, c, u_g, alpha_g, stats, status, li);
     } else {
-        C.store(S, c, lane);
+        C.store(S, c, li);
         if (lane == 0)
             list_append(Ln, C.mixed(P), c);
     }
