@@ -387,27 +387,28 @@ struct Dense {
                 d = d - col(j, lane) * dj;
         }
     }
-    // LAPACK dpotrs('L'): dtrsm(L) then dtrsm(L^T), reference BLAS order (opt.cpp:285)
+    // LAPACK dpotrs('L'): dtrsm(L) then dtrsm(L^T), reference BLAS order (opt.cpp:285).  Only
+    // the unbatched semantics use it, so it is kept compact (rolled loops, factor from LDS).
     __device__ __forceinline__ void potrs_lapack(double &d, int lane)
     {
-#pragma unroll
+#pragma unroll 1
         for (int k = 0; k < NM; ++k) {
-            double dk = Grp<G>::bc(d, k);
+            double dk = Grp<G>::bc_dyn(d, k);
             if (dk != 0.0) {
-                dk = EXACT ? dk / diag(k) : dk * Grp<G>::bc(dinv, k);
+                dk = EXACT ? dk / sT[k * ld + k] : dk * Grp<G>::bc_dyn(dinv, k);
                 if (lane == k)
                     d = dk;
-                else if (lane > k)
-                    d = d - dk * row[k];
+                else if (lane > k && lane < NM)
+                    d = d - dk * sT[lane * ld + k];
             }
         }
-#pragma unroll
+#pragma unroll 1
         for (int i = NM - 1; i >= 0; --i) {
-            double temp = Grp<G>::bc(d, i);
-#pragma unroll
+            double temp = Grp<G>::bc_dyn(d, i);
+#pragma unroll 1
             for (int k = i + 1; k < NM; ++k)
-                temp = temp - sT[k * ld + i] * Grp<G>::bc(d, k);
-            temp = EXACT ? temp / diag(i) : temp * Grp<G>::bc(dinv, i);
+                temp = temp - sT[k * ld + i] * Grp<G>::bc_dyn(d, k);
+            temp = EXACT ? temp / sT[i * ld + i] : temp * Grp<G>::bc_dyn(dinv, i);
             if (lane == i)
                 d = temp;
         }
@@ -1080,9 +1081,10 @@ struct Cell {
     __device__ __forceinline__ bool ls_accepts(double fdir) const
     {
         double fNew = fdir;
+        const double au = a * uu;  // each lane its own product
 #pragma unroll
         for (int i = 0; i < NM; ++i)
-            fNew = fNew - Grp<G>::bc(a, i) * Grp<G>::bc(uu, i);
+            fNew = fNew - Grp<G>::bc(au, i);
         return fNew <= f + 1e-3 * tls * gd;
     }
 
@@ -1143,9 +1145,10 @@ struct Cell {
             f = fdir;
         else
             f = Grp<G>::bc(D.row[0], 0) / (P.m0 * P.m0);
+        const double au = a * uu;
 #pragma unroll
         for (int i = 0; i < NM; ++i)
-            f = f - Grp<G>::bc(a, i) * Grp<G>::bc(uu, i);
+            f = f - Grp<G>::bc(au, i);
         double rs = 0.0;  // utils_norm1
 #pragma unroll
         for (int j = 0; j < NM; ++j)
@@ -1235,9 +1238,10 @@ struct Cell {
                 maxF = temp;
         }
         gd = 0.0;
+        const double gdl = g * d;
 #pragma unroll
         for (int i = 0; i < NM; ++i)
-            gd = gd + Grp<G>::bc(g, i) * Grp<G>::bc(d, i);
+            gd = gd + Grp<G>::bc(gdl, i);
         tls = 1.0;
         ls_next(lane);
         return 0;
