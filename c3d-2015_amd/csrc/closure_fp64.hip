@@ -564,6 +564,42 @@ __global__ __launch_bounds__(kBlock) void k_init(KParams P, const double *__rest
 
 
 // ---------------------------------------------------------------- sweep building blocks
+// M and w chunks of kChunk ordinates staged through double-buffered LDS by a whole block:
+// each thread holds its share of the next chunk in registers while the block consumes the
+// current one, so one barrier per chunk publishes a chunk and the global loads overlap the
+// matrix-core work.
+template <int NM>
+struct ChunkStager {
+    static constexpr int NMS = NM | 1;
+    static constexpr int kPer = (kChunk * NM + kBlock - 1) / kBlock;
+    double pre[kPer];
+    double prew;
+    __device__ __forceinline__ void fetch(const double *__restrict__ Mg,
+                                          const double *__restrict__ wg, int ch, int tid)
+    {
+        const double *src = Mg + (size_t)ch * kChunk * NM;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int e = tid + kBlock * k;
+            pre[k] = (e < kChunk * NM) ? src[e] : 0.0;
+        }
+        prew = tid < kChunk ? wg[(size_t)ch * kChunk + tid] : 0.0;
+    }
+    __device__ __forceinline__ void put(double *sM, double *sW, int tid) const
+    {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int e = tid + kBlock * k;
+            if (e < kChunk * NM) {
+                const int rr = e / NM, cc = e - rr * NM;
+                sM[rr * NMS + cc] = pre[k];
+            }
+        }
+        if (tid < kChunk)
+            sW[tid] = prew;
+    }
+};
+
 // Matrix-core accumulation of one LDS chunk of kChunk ordinates for CW cells: lane l owns
 // ordinate l of the chunk for the exponent, then the MFMA loop consumes 4 ordinates per
 // instruction.  The operation sequence per cell does not depend on CW or on the other cells
@@ -698,11 +734,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep(KParams P, const double *__res
     constexpr int NB = NT * (NT + 1) / 2;
     constexpr int NMS = NM | 1;
     constexpr int CW = kCellsPerWave;
-    __shared__ double sM[kChunk * NMS];
-    __shared__ double sW[kChunk];
+    __shared__ double sM[2][kChunk * NMS];
+    __shared__ double sW[2][kChunk];
     __shared__ double sA[kWavesPerBlock][CW * NM];
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nch = P.nqPad / kChunk;
     const int nW = uniform_int(*nWp);
     const int groups = (nW + CW - 1) / CW;
     const int nBlocksNeeded = (groups + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -730,17 +767,17 @@ __global__ __launch_bounds__(kBlock) void k_sweep(KParams P, const double *__res
         for (int c = 0; c < CW; ++c)
             fpart[c] = 0.0;
         const bool any = cells[0] >= 0;
-        for (int ch = 0; ch < P.nqPad / kChunk; ++ch) {
+        ChunkStager<NM> st;
+        st.fetch(Mg, wg, 0, tid);
+        __syncthreads();  // the previous cells' last chunk is consumed
+        for (int ch = 0; ch < nch; ++ch) {
+            const int b = ch & 1;
+            st.put(sM[b], sW[b], tid);
             __syncthreads();
-            for (int e = tid; e < kChunk * NM; e += kBlock) {
-                const int rr = e / NM, cc = e - rr * NM;
-                sM[rr * NMS + cc] = Mg[(size_t)ch * kChunk * NM + e];
-            }
-            if (tid < kChunk)
-                sW[tid] = wg[(size_t)ch * kChunk + tid];
-            __syncthreads();
+            if (ch + 1 < nch)
+                st.fetch(Mg, wg, ch + 1, tid);
             if (any)
-                chunk_mfma<NM, CW>(sM, sW[lane], sA[wv], acc, fpart, lane);
+                chunk_mfma<NM, CW>(sM[b], sW[b][lane], sA[wv], acc, fpart, lane);
         }
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
@@ -877,8 +914,9 @@ __device__ __forceinline__ void store_H32(const f4 *acc, double es, double *H, i
 }
 
 // Sweep pass of the mixed phase: 4 cells per wave, 16 per block sharing each LDS chunk.
+// Three waves per SIMD (at most 168 registers a lane) measured faster than two.
 template <int NM>
-__global__ __launch_bounds__(kBlock) void k_sweep_mixed(KParams P, const double *__restrict__ Mg,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_sweep_mixed(KParams P, const double *__restrict__ Mg,
                                                          const double *__restrict__ wg, State S,
                                                          const int *__restrict__ W,
                                                          const int *__restrict__ nWp)
@@ -887,10 +925,11 @@ __global__ __launch_bounds__(kBlock) void k_sweep_mixed(KParams P, const double 
     constexpr int NB = NT * (NT + 1) / 2;
     constexpr int NMS = NM | 1;
     constexpr int CW = kCellsPerWave;
-    __shared__ double sM[kChunk * NMS];
-    __shared__ double sW[kChunk];
+    __shared__ double sM[2][kChunk * NMS];
+    __shared__ double sW[2][kChunk];
     __shared__ double sA[kWavesPerBlock][CW * NM];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nch = P.nqPad / kChunk;
     const int nW = uniform_int(*nWp);
     const int groups = (nW + CW - 1) / CW;
     const int nBlocksNeeded = (groups + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -918,17 +957,17 @@ __global__ __launch_bounds__(kBlock) void k_sweep_mixed(KParams P, const double 
         for (int c = 0; c < CW; ++c)
             fpart[c] = 0.0;
         const bool any = cells[0] >= 0;
-        for (int ch = 0; ch < P.nqPad / kChunk; ++ch) {
+        ChunkStager<NM> st;
+        st.fetch(Mg, wg, 0, tid);
+        __syncthreads();  // the previous cells' last chunk is consumed
+        for (int ch = 0; ch < nch; ++ch) {
+            const int b = ch & 1;
+            st.put(sM[b], sW[b], tid);
             __syncthreads();
-            for (int e = tid; e < kChunk * NM; e += kBlock) {
-                const int rr = e / NM, cc = e - rr * NM;
-                sM[rr * NMS + cc] = Mg[(size_t)ch * kChunk * NM + e];
-            }
-            if (tid < kChunk)
-                sW[tid] = wg[(size_t)ch * kChunk + tid];
-            __syncthreads();
+            if (ch + 1 < nch)
+                st.fetch(Mg, wg, ch + 1, tid);
             if (any)
-                chunk_mixed<NM, CW>(sM, sW[lane], sA[wv], shift, acc, fpart, lane);
+                chunk_mixed<NM, CW>(sM[b], sW[b][lane], sA[wv], shift, acc, fpart, lane);
         }
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
@@ -1076,16 +1115,21 @@ struct Cell {
         }
     }
 
-    // Armijo test of the current line-search trial, f(alpha) = fdir - alpha.u
-    // (linesearch.cpp:60-66)
-    __device__ __forceinline__ bool ls_accepts(double fdir) const
+    // Armijo test of a line-search trial at step t and point at (this lane's component),
+    // f(alpha) = fdir - alpha.u (linesearch.cpp:60-66)
+    __device__ __forceinline__ bool accepts_at(double fdir, double at, double t) const
     {
         double fNew = fdir;
-        const double au = a * uu;  // each lane its own product
+        const double au = at * uu;  // each lane its own product
 #pragma unroll
         for (int i = 0; i < NM; ++i)
             fNew = fNew - Grp<G>::bc(au, i);
-        return fNew <= f + 1e-3 * tls * gd;
+        return fNew <= f + 1e-3 * t * gd;
+    }
+    // ... of the current trial
+    __device__ __forceinline__ bool ls_accepts(double fdir) const
+    {
+        return accepts_at(fdir, a, tls);
     }
 
     // The next trial of the backtracking search at step tls, or its end (linesearch.cpp:46-54):
@@ -1328,7 +1372,7 @@ __device__ __forceinline__ void f_trials(const KParams &P, const double *__restr
 #pragma unroll
         for (int j = 0; j < KW; ++j) {
             if (j >= kw)
-                break;
+                continue;
             const double *a = sA + j * NM;
             double x = 0.0;
             if constexpr (MODE == kFMixed) {
@@ -1370,7 +1414,7 @@ __device__ __forceinline__ void f_trials(const KParams &P, const double *__restr
 #pragma unroll
     for (int j = 0; j < KW; ++j) {
         if (j >= kw)
-            break;
+            continue;
         if constexpr (MODE == kFRef)
             out[j] = fpart[j];
         else if constexpr (MODE == kFMixed)
@@ -1418,7 +1462,8 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     __shared__ double sW[TW][kChunk];
     __shared__ double sE[TW][kChunk];
     __shared__ double sA[TW][KW * NM];
-    __shared__ double sF[TW * KW + 1];  // f at each trial point; [TW * KW]: f of the sweep
+    __shared__ int sOk[TW * KW];        // Armijo verdict at each trial point
+    __shared__ double sF[1];            // f of the sweep
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int nWm = uniform_int(*Lc.nm);
     const int nW = nWm + uniform_int(*Lc.nf);
@@ -1435,17 +1480,18 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     for (int p = 0; p < passes; ++p) {
         const bool mx = MIXED && C.mixed(P);
         if (C.phase == kPhaseLS) {
-            // points k = wv * kw + j at tls / 2^k, formed as advance() forms them
+            // points k = wv * kw + j at tls / 2^k, formed as ls_next() forms them
             const int kw = C.tls < 1.0 ? KW : 1;
             double tk = C.tls;
             for (int k = 0; k < wv * kw; ++k)
                 tk = 0.5 * tk;
+            double aj[KW], tj[KW];
 #pragma unroll
             for (int j = 0; j < KW; ++j) {
-                if (j >= kw)
-                    break;
-                if (lane < NM)
-                    sA[wv][j * NM + lane] = (wv == 0 && j == 0) ? C.a : C.aOld + tk * C.d;
+                tj[j] = tk;
+                aj[j] = (wv == 0 && j == 0) ? C.a : (li < NM ? C.aOld + tk * C.d : 0.0);
+                if (j < kw && lane < NM)
+                    sA[wv][j * NM + lane] = aj[j];
                 tk = 0.5 * tk;
             }
             wave_lds_sync();
@@ -1456,17 +1502,19 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
             else
                 f_trials<NM, REF ? kFRef : kFp64, KW>(P, MT, wg, sA[wv], kw, 0.0, sW[wv],
                                                       sE[wv], lane, ft);
-            if (lane == 0) {
+            // each wave judges its own points; the search then walks the verdicts in order
 #pragma unroll
-                for (int j = 0; j < KW; ++j)
-                    if (j < kw)
-                        sF[wv * kw + j] = ft[j];
+            for (int j = 0; j < KW; ++j) {
+                if (j < kw) {
+                    const bool ok = C.accepts_at(ft[j], aj[j], tj[j]);
+                    if (lane == 0)
+                        sOk[wv * kw + j] = ok;
+                }
             }
             __syncthreads();
             bool accepted = false;
             for (int k = 0; k < TW * kw; ++k) {
-                const double fk = sF[k];
-                if (C.ls_accepts(fk)) {
+                if (sOk[k]) {
                     accepted = true;
                     break;
                 }
@@ -1474,7 +1522,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 if (C.phase != kPhaseLS)
                     break;
             }
-            __syncthreads();  // sF and sA are rewritten by the next round
+            __syncthreads();  // sOk and sA are rewritten by the next round
             if (!accepted)
                 continue;
         }
@@ -1551,11 +1599,11 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 }
             }
             if (lane == 0)
-                sF[TW * KW] = fdir;
+                sF[0] = fdir;
             ++sweeps;
         }
         __syncthreads();
-        done = C.template advance<REF>(P, sH, sF[TW * KW], sT[wv][row], li);
+        done = C.template advance<REF>(P, sH, sF[0], sT[wv][row], li);
         __syncthreads();  // sH and sF are rewritten by the next round
         if (done)
             break;
