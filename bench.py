@@ -77,7 +77,7 @@ def main():
     ap.add_argument("--mixed-iter", type=int, default=40)
     ap.add_argument("--train-pairs", type=int, default=1 << 22)
     ap.add_argument("--train-steps", type=int, default=2000, help="MSE steps")
-    ap.add_argument("--dual-steps", type=int, default=3000, help="dual-gap steps")
+    ap.add_argument("--dual-steps", type=int, default=10000, help="dual-gap steps")
     ap.add_argument("--train-batch", type=int, default=8192)
     ap.add_argument("--train-lr", type=float, default=3e-3)
     ap.add_argument("--workload", choices=("closure", "step"), default="closure",

@@ -564,6 +564,27 @@ __global__ __launch_bounds__(kBlock) void k_init(KParams P, const double *__rest
 
 
 // ---------------------------------------------------------------- sweep building blocks
+// FP32 copy of an M chunk for the matrix-core operands, laid out so that lane (rq, gq) of
+// the MFMA finds M[4k + gq][rq] for k = 0..15 in one 16-float row: row (col * 4 + gq),
+// element k, rows kRS floats apart (16-byte aligned, spread over the banks).  Rows of
+// columns nm..16 NT-1 stay zero.
+constexpr int kRS = 20;
+template <int NM>
+struct MixedChunk {
+    static constexpr int NT = (NM + 15) / 16;
+    static constexpr int kFloats = 16 * NT * 4 * kRS;
+    static __device__ __forceinline__ int index(int rr, int cc)
+    {
+        return (cc * 4 + (rr & 3)) * kRS + (rr >> 2);
+    }
+    // zero the padding columns of a buffer (once per kernel, any number of threads)
+    static __device__ __forceinline__ void clear_pad(float *sMfT, int tid, int nthreads)
+    {
+        for (int e = tid; e < (16 * NT - NM) * 4 * kRS; e += nthreads)
+            sMfT[NM * 4 * kRS + e] = 0.0f;
+    }
+};
+
 // M and w chunks of kChunk ordinates staged through double-buffered LDS by a whole block:
 // each thread holds its share of the next chunk in registers while the block consumes the
 // current one, so one barrier per chunk publishes a chunk and the global loads overlap the
@@ -593,6 +614,21 @@ struct ChunkStager {
             if (e < kChunk * NM) {
                 const int rr = e / NM, cc = e - rr * NM;
                 sM[rr * NMS + cc] = pre[k];
+            }
+        }
+        if (tid < kChunk)
+            sW[tid] = prew;
+    }
+    // the same, plus the FP32 operand copy of the chunk for the mixed sweep
+    __device__ __forceinline__ void put_mixed(double *sM, float *sMfT, double *sW, int tid) const
+    {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int e = tid + kBlock * k;
+            if (e < kChunk * NM) {
+                const int rr = e / NM, cc = e - rr * NM;
+                sM[rr * NMS + cc] = pre[k];
+                sMfT[MixedChunk<NM>::index(rr, cc)] = (float)pre[k];
             }
         }
         if (tid < kChunk)
@@ -853,8 +889,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_ref(KParams P, const double *_
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 template <int NM, int CW>
-__device__ __forceinline__ void chunk_mixed(const double *sMc, double wl, const double *sAc,
-                                            const double *shift,
+__device__ __forceinline__ void chunk_mixed(const double *sMc, const float *sMfT, double wl,
+                                            const double *sAc, const double *shift, float *sWE,
                                             f4 (&acc)[CW][(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
                                             double (&fpart)[CW], int lane)
 {
@@ -862,7 +898,8 @@ __device__ __forceinline__ void chunk_mixed(const double *sMc, double wl, const 
     constexpr int NMS = NM | 1;
     const int gq = lane >> 4, rq = lane & 15;
     const double *rowp = &sMc[lane * NMS];
-    float wef[CW];
+    // the lane's ordinate: exponent, FP64 weight for f, FP32 weight to the wave's LDS in the
+    // order the MFMA lanes read them (ordinate 4k + g at g * 16 + k)
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
         double x = 0.0;
@@ -871,24 +908,34 @@ __device__ __forceinline__ void chunk_mixed(const double *sMc, double wl, const 
             x = fma(sAc[c * NM + i], rowp[i], x);
         const double we = wl * exp(x - shift[c]);
         fpart[c] = fpart[c] + we;
-        wef[c] = (float)we;
+        sWE[c * kChunk + (lane & 3) * 16 + (lane >> 2)] = (float)we;
     }
+    wave_lds_sync();
+    const float *m0 = &sMfT[(rq * 4 + gq) * kRS];
+    const float *m1 = &sMfT[((16 + rq) * 4 + gq) * kRS];
 #pragma unroll
-    for (int k = 0; k < kChunk / 4; ++k) {
-        const int ql = 4 * k + gq;
-        const double *mr = &sMc[ql * NMS];
-        const float b0 = (rq < NM) ? (float)mr[rq] : 0.0f;
-        const float b1 = (NT == 2 && 16 + rq < NM) ? (float)mr[16 + rq] : 0.0f;
+    for (int kb = 0; kb < kChunk / 16; ++kb) {
+        const f4 b0v = *reinterpret_cast<const f4 *>(m0 + 4 * kb);
+        f4 b1v = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (NT == 2)
+            b1v = *reinterpret_cast<const f4 *>(m1 + 4 * kb);
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
-            const float wq = __shfl(wef[c], ql);
-            acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b0, b0, acc[c][0], 0, 0, 0);
-            if constexpr (NT == 2) {
-                acc[c][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b1, b0, acc[c][1], 0, 0, 0);
-                acc[c][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b1, b1, acc[c][2], 0, 0, 0);
+            const f4 wv4 = *reinterpret_cast<const f4 *>(&sWE[c * kChunk + gq * 16 + 4 * kb]);
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const float wq = wv4[kk], b0 = b0v[kk], b1 = b1v[kk];
+                acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b0, b0, acc[c][0], 0, 0, 0);
+                if constexpr (NT == 2) {
+                    acc[c][1] =
+                        __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b1, b0, acc[c][1], 0, 0, 0);
+                    acc[c][2] =
+                        __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b1, b1, acc[c][2], 0, 0, 0);
+                }
             }
         }
     }
+    wave_lds_sync();  // sWE is rewritten by the next chunk
 }
 
 // H = e^shift * H32, row-major, lower triangle mirrored
@@ -926,6 +973,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     constexpr int NMS = NM | 1;
     constexpr int CW = kCellsPerWave;
     __shared__ double sM[2][kChunk * NMS];
+    __shared__ __align__(16) float sMfT[2][MixedChunk<NM>::kFloats];
+    __shared__ __align__(16) float sWE[kWavesPerBlock][CW * kChunk];
     __shared__ double sW[2][kChunk];
     __shared__ double sA[kWavesPerBlock][CW * NM];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -933,6 +982,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     const int nW = uniform_int(*nWp);
     const int groups = (nW + CW - 1) / CW;
     const int nBlocksNeeded = (groups + kWavesPerBlock - 1) / kWavesPerBlock;
+    MixedChunk<NM>::clear_pad(sMfT[0], tid, kBlock);
+    MixedChunk<NM>::clear_pad(sMfT[1], tid, kBlock);
     for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
         const int grp = blk * kWavesPerBlock + wv;
         int cells[CW];
@@ -962,12 +1013,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         __syncthreads();  // the previous cells' last chunk is consumed
         for (int ch = 0; ch < nch; ++ch) {
             const int b = ch & 1;
-            st.put(sM[b], sW[b], tid);
+            st.put_mixed(sM[b], sMfT[b], sW[b], tid);
             __syncthreads();
             if (ch + 1 < nch)
                 st.fetch(Mg, wg, ch + 1, tid);
             if (any)
-                chunk_mixed<NM, CW>(sM[b], sW[b][lane], sA[wv], shift, acc, fpart, lane);
+                chunk_mixed<NM, CW>(sM[b], sMfT[b], sW[b][lane], sA[wv], shift, sWE[wv], acc,
+                                    fpart, lane);
         }
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
@@ -1464,6 +1516,8 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     __shared__ double sA[TW][KW * NM];
     __shared__ int sOk[TW * KW];        // Armijo verdict at each trial point
     __shared__ double sF[1];            // f of the sweep
+    __shared__ __align__(16) float sMfT[MIXED ? MixedChunk<NM>::kFloats : 4];
+    __shared__ __align__(16) float sWE[MIXED ? kChunk : 4];
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int nWm = uniform_int(*Lc.nm);
     const int nW = nWm + uniform_int(*Lc.nf);
@@ -1472,6 +1526,10 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
         return;
     const int c = uniform_int(list_cell(Lc, slot, nWm));
     const int li = lane & (G - 1), row = lane / G;
+    if constexpr (MIXED) {
+        MixedChunk<NM>::clear_pad(sMfT, threadIdx.x, TW * kWave);
+        __syncthreads();
+    }
     Cell<NM, G> C;
     C.load(S, c, li);
     const int nch = P.nqPad / kChunk;
@@ -1573,6 +1631,9 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                         const int e = lane + kWave * k;
                         const int rr = e / NM, cc = e - rr * NM;
                         sM[rr * NMS + cc] = pre[k];
+                        if constexpr (MIXED)
+                            if (mx)
+                                sMfT[MixedChunk<NM>::index(rr, cc)] = (float)pre[k];
                     }
                     const double wl = prew;
                     wave_lds_sync();
@@ -1584,7 +1645,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                             pre[k] = Mg[base * NM + lane + kWave * k];
                     }
                     if (mx)
-                        chunk_mixed<NM, 1>(sM, wl, sA[0], shift, acc32, fpart, lane);
+                        chunk_mixed<NM, 1>(sM, sMfT, wl, sA[0], shift, sWE, acc32, fpart, lane);
                     else
                         chunk_mfma<NM, 1>(sM, wl, sA[0], acc, fpart, lane);
                 }
@@ -1838,9 +1899,12 @@ __global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nq
     constexpr int NB = NT * (NT + 1) / 2;
     constexpr int NMS = NM | 1;
     __shared__ double sM[kChunk * NMS];
+    __shared__ __align__(16) float sMfT[MixedChunk<NM>::kFloats];
+    __shared__ __align__(16) float sWE[kWavesPerBlock][kChunk];
     __shared__ double sW[kChunk];
     __shared__ double sA[kWavesPerBlock][NM];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    MixedChunk<NM>::clear_pad(sMfT, tid, kBlock);
     const int64_t cell = (int64_t)blockIdx.x * kWavesPerBlock + wv;
     const bool have = cell < ncells;
     if (lane < NM)
@@ -1855,13 +1919,15 @@ __global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nq
         __syncthreads();
         for (int e = tid; e < kChunk * NM; e += kBlock) {
             int rr = e / NM, cc = e - rr * NM;
-            sM[rr * NMS + cc] = Mg[(size_t)c * kChunk * NM + e];
+            const double m = Mg[(size_t)c * kChunk * NM + e];
+            sM[rr * NMS + cc] = m;
+            sMfT[MixedChunk<NM>::index(rr, cc)] = (float)m;
         }
         if (tid < kChunk)
             sW[tid] = wg[(size_t)c * kChunk + tid];
         __syncthreads();
         if (have)
-            chunk_mixed<NM, 1>(sM, sW[lane], sA[wv], shift, acc, fpart, lane);
+            chunk_mixed<NM, 1>(sM, sMfT, sW[lane], sA[wv], shift, sWE[wv], acc, fpart, lane);
     }
     if (!have)
         return;
