@@ -1704,101 +1704,190 @@ __global__ void k_warm_features(int64_t rows, State S, Lists L, const float *__r
 }
 
 // The gradient-decrease test: for each listed cell, ||grad h||_2 at the caller's alpha and at
-// the network's proposal anet[slot] (one wave per cell, sums in FP32: the test only ranks
-// two candidates, and a proposal whose exponentials overflow loses it); the proposal becomes
-// the starting point (alpha, alpha0 and the line-search anchor) only if its gradient is
-// smaller.  Either way the mixed mode's first shift is log sum_q w e at the chosen point.
+// the network's proposal anet[slot] (sums in FP32: the test only ranks two candidates, and a
+// proposal whose exponentials overflow loses it); the proposal becomes the starting point
+// (alpha, alpha0 and the line-search anchor) only if its gradient is smaller.  Either way the
+// mixed mode's first shift is log sum_q w e at the chosen point.  Four cells per wave, eight
+// candidate points: each lane forms the weights w_q e^{x} of its ordinate for the eight points
+// (VALU), and the gradients sum_q w_q e_q m_q accumulate on the FP32 matrix cores, point v in
+// row v of a 16x16 tile (rows 8..15 zero).
 template <int NM>
 __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double *__restrict__ Mg,
                                                          const double *__restrict__ wg, State S,
                                                          Lists L, const double *__restrict__ anet)
 {
+    constexpr int NT = (NM + 15) / 16;
     constexpr int NMS = NM | 1;
-    __shared__ float sM[kChunk * NMS];
-    __shared__ float sW[kChunk];
-    __shared__ float sA[kWavesPerBlock][2][NM];
+    constexpr int CW = kCellsPerWave;
+    constexpr int NV = 2 * CW;  // point v = 2 c + (0: the caller's alpha, 1: the proposal)
+    constexpr int RS = kRS;
+    constexpr int kPer = (kChunk * NM + kBlock - 1) / kBlock;
+    static_assert(NV <= 16, "points fill the rows of one tile");
+    __shared__ float sMr[2][kChunk * NMS];
+    __shared__ __align__(16) float sMfT[2][MixedChunk<NM>::kFloats];
+    __shared__ float sW[2][kChunk];
+    __shared__ float sA[kWavesPerBlock][NV * NM];
+    __shared__ __align__(16) float sWE[kWavesPerBlock][NV * kChunk];
+    __shared__ float sN[kWavesPerBlock][NV];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int gq = lane >> 4, rq = lane & 15;
+    const int nch = P.nqPad / kChunk;
     const int nWm = uniform_int(*L.nm);
     const int nW = nWm + uniform_int(*L.nf);
-    const int nBlocksNeeded = (nW + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int groups = (nW + CW - 1) / CW;
+    const int nBlocksNeeded = (groups + kWavesPerBlock - 1) / kWavesPerBlock;
+    MixedChunk<NM>::clear_pad(sMfT[0], tid, kBlock);
+    MixedChunk<NM>::clear_pad(sMfT[1], tid, kBlock);
     for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
-        const int slot = blk * kWavesPerBlock + wv;
-        const int cell = slot < nW ? uniform_int(list_cell(L, slot, nWm)) : -1;
-        if (cell >= 0 && lane < NM) {
-            sA[wv][0][lane] = (float)S.vec[(size_t)cell * kVec * NM + kVa * NM + lane];
-            sA[wv][1][lane] = (float)anet[(size_t)slot * NM + lane];
-        }
-        float f1 = 0.0f, f2 = 0.0f, g1[NM], g2[NM];
+        const int grp = blk * kWavesPerBlock + wv;
+        int slots[CW], cells[CW];
 #pragma unroll
-        for (int i = 0; i < NM; ++i) {
-            g1[i] = 0.0f;
-            g2[i] = 0.0f;
+        for (int c = 0; c < CW; ++c) {
+            slots[c] = grp * CW + c;
+            cells[c] = (grp < groups && slots[c] < nW) ? list_cell(L, slots[c], nWm) : -1;
+            if (lane < NM) {
+                const bool ok = cells[c] >= 0;
+                sA[wv][(2 * c) * NM + lane] =
+                    ok ? (float)S.vec[(size_t)cells[c] * kVec * NM + kVa * NM + lane] : 0.0f;
+                sA[wv][(2 * c + 1) * NM + lane] =
+                    ok ? (float)anet[(size_t)slots[c] * NM + lane] : 0.0f;
+            }
         }
-        for (int ch = 0; ch < P.nqPad / kChunk; ++ch) {
-            __syncthreads();
-            for (int e = tid; e < kChunk * NM; e += kBlock) {
-                const int rr = e / NM, cc = e - rr * NM;
-                sM[rr * NMS + cc] = (float)Mg[(size_t)ch * kChunk * NM + e];
+        f4 acc[NT];
+#pragma unroll
+        for (int b = 0; b < NT; ++b)
+            acc[b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        float fpart[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            fpart[v] = 0.0f;
+        const bool any = cells[0] >= 0;
+        // chunk staging: the next chunk in registers, converted to FP32 when stored
+        double pre[kPer], prew;
+        auto fetch = [&](int ch) {
+            const double *src = Mg + (size_t)ch * kChunk * NM;
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int e = tid + kBlock * k;
+                pre[k] = (e < kChunk * NM) ? src[e] : 0.0;
+            }
+            prew = tid < kChunk ? wg[(size_t)ch * kChunk + tid] : 0.0;
+        };
+        fetch(0);
+        __syncthreads();  // the previous cells' last chunk is consumed
+        for (int ch = 0; ch < nch; ++ch) {
+            const int b = ch & 1;
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int e = tid + kBlock * k;
+                if (e < kChunk * NM) {
+                    const int rr = e / NM, cc = e - rr * NM;
+                    sMr[b][rr * NMS + cc] = (float)pre[k];
+                    sMfT[b][MixedChunk<NM>::index(rr, cc)] = (float)pre[k];
+                }
             }
             if (tid < kChunk)
-                sW[tid] = (float)wg[(size_t)ch * kChunk + tid];
+                sW[b][tid] = (float)prew;
             __syncthreads();
-            if (cell >= 0) {
-                const float *rowp = &sM[lane * NMS];
-                float x1 = 0.0f, x2 = 0.0f;
+            if (ch + 1 < nch)
+                fetch(ch + 1);
+            if (!any)
+                continue;
+            // weights of the lane's ordinate at the eight points, to LDS in MFMA order
+            const float *rowp = &sMr[b][lane * NMS];
+            const float wl = sW[b][lane];
 #pragma unroll
-                for (int i = 0; i < NM; ++i) {
-                    x1 = fmaf(sA[wv][0][i], rowp[i], x1);
-                    x2 = fmaf(sA[wv][1][i], rowp[i], x2);
-                }
-                const float e1 = sW[lane] * expf(x1), e2 = sW[lane] * expf(x2);
-                f1 += e1;
-                f2 += e2;
+            for (int v = 0; v < NV; ++v) {
+                float x = 0.0f;
 #pragma unroll
-                for (int i = 0; i < NM; ++i) {
-                    g1[i] = fmaf(e1, rowp[i], g1[i]);
-                    g2[i] = fmaf(e2, rowp[i], g2[i]);
+                for (int i = 0; i < NM; ++i)
+                    x = fmaf(sA[wv][v * NM + i], rowp[i], x);
+                const float we = wl * __expf(x);  // hardware exp2: ranking needs no more
+                fpart[v] += we;
+                sWE[wv][v * kChunk + (lane & 3) * 16 + (lane >> 2)] = we;
+            }
+            wave_lds_sync();
+            const float *m0 = &sMfT[b][(rq * 4 + gq) * RS];
+            const float *a0 = &sWE[wv][(rq < NV ? rq : 0) * kChunk + gq * 16];
+#pragma unroll
+            for (int kb = 0; kb < kChunk / 16; ++kb) {
+                f4 av = *reinterpret_cast<const f4 *>(a0 + 4 * kb);
+                if (rq >= NV)
+                    av = f4{0.0f, 0.0f, 0.0f, 0.0f};
+                const f4 b0v = *reinterpret_cast<const f4 *>(m0 + 4 * kb);
+                f4 b1v = f4{0.0f, 0.0f, 0.0f, 0.0f};
+                if constexpr (NT == 2)
+                    b1v = *reinterpret_cast<const f4 *>(&sMfT[b][((16 + rq) * 4 + gq) * RS + 4 * kb]);
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], b0v[kk], acc[0], 0, 0, 0);
+                    if constexpr (NT == 2)
+                        acc[1] =
+                            __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], b1v[kk], acc[1], 0, 0, 0);
                 }
             }
+            wave_lds_sync();  // sWE is rewritten by the next chunk
         }
-        if (cell < 0)
+        if (!any)
             continue;
-        const double *v = S.vec + (size_t)cell * kVec * NM;
-        float n1 = 0.0f, n2 = 0.0f;
+        // ||g_v - u||^2: lane (rq, gq) holds G[v = 4 gq + r][j = rq (+16)] in acc[.][r]
+        float nv[4];
 #pragma unroll
-        for (int i = 0; i < NM; ++i) {
-            float t1 = g1[i], t2 = g2[i];
+        for (int r = 0; r < 4; ++r) {
+            const int v = 4 * gq + r;
+            const int c = (v < NV ? v : 0) >> 1;
+            float t = 0.0f;
 #pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) {
-                t1 += __shfl_xor(t1, o);
-                t2 += __shfl_xor(t2, o);
+            for (int bt = 0; bt < NT; ++bt) {
+                const int j = 16 * bt + rq;
+                if (j < NM && cells[c] >= 0) {
+                    const float uj = (float)S.vec[(size_t)cells[c] * kVec * NM + kVu * NM + j];
+                    const float d = acc[bt][r] - uj;
+                    t = fmaf(d, d, t);
+                }
             }
-            const float ui = (float)v[kVu * NM + i];
-            n1 = fmaf(t1 - ui, t1 - ui, n1);
-            n2 = fmaf(t2 - ui, t2 - ui, n2);
-        }
-        const bool take = n2 < n1;  // false when the proposal overflowed (NaN or inf)
-        float fs = take ? f2 : f1;
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1)
-            fs += __shfl_xor(fs, o);
-        if (take && lane < NM) {
-            const double an = anet[(size_t)slot * NM + lane];
-            double *vw = S.vec + (size_t)cell * kVec * NM;
-            vw[kVa * NM + lane] = an;
-            vw[kVa0 * NM + lane] = an;
-            vw[kVaOld * NM + lane] = an;
+            for (int o = 8; o >= 1; o >>= 1)
+                t += __shfl_xor(t, o);
+            nv[r] = t;
         }
-        if (lane == 0) {
-            if (take)
-                S.ints[(size_t)cell * kInt + kIflags] |= C3D_FLAG_WARM;
-            if (P.mixed) {
-                const double lf = log((double)fs);
-                S.scal[(size_t)cell * kScal + kSshift] = (lf > -700.0 && lf < 700.0) ? lf : 0.0;
+        if (rq == 0 && gq < NV / 4) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                sN[wv][4 * gq + r] = nv[r];
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            // false when the proposal overflowed (NaN or inf)
+            const bool take = sN[wv][2 * c + 1] < sN[wv][2 * c];
+            float fs = take ? fpart[2 * c + 1] : fpart[2 * c];
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1)
+                fs += __shfl_xor(fs, o);
+            if (cells[c] < 0)
+                continue;
+            double *vw = S.vec + (size_t)cells[c] * kVec * NM;
+            if (take && lane < NM) {
+                const double an = anet[(size_t)slots[c] * NM + lane];
+                vw[kVa * NM + lane] = an;
+                vw[kVa0 * NM + lane] = an;
+                vw[kVaOld * NM + lane] = an;
+            }
+            if (lane == 0) {
+                if (take)
+                    S.ints[(size_t)cells[c] * kInt + kIflags] |= C3D_FLAG_WARM;
+                if (P.mixed) {
+                    const double lf = log((double)fs);
+                    S.scal[(size_t)cells[c] * kScal + kSshift] =
+                        (lf > -700.0 && lf < 700.0) ? lf : 0.0;
+                }
             }
         }
+        wave_lds_sync();  // sN and sA are rewritten for the next cells
     }
 }
+
 
 // Cells still listed after the pass budget: report them and leave their input untouched.
 template <int NM>
