@@ -17,6 +17,7 @@
 // Every expression keeps the reference's evaluation order (no FMA contraction), so the
 // results match the CPU code bit for bit whenever the exponentials agree.
 #include <cfloat>
+#include <cmath>
 
 #include "c3d_detmath.h"
 #include "c3d_internal.h"
@@ -135,20 +136,27 @@ __device__ __forceinline__ double minmod(double x, double y)
 {
     return sgn1(x) * mmax(0.0, mmin(fabs(x), y * sgn1(x)));
 }
-// flux_kinetic.cpp:55-60
+// flux_kinetic.cpp:55-60.  TH1: theta == 1 (the minmod limiter), where th * x == x exactly.
+template <bool TH1>
 __device__ __forceinline__ double slopefit(double l, double c, double r, double th)
 {
-    return minmod(th * (r - c), minmod(0.5 * (r - l), th * (c - l)));
+    if constexpr (TH1)
+        return minmod(r - c, minmod(0.5 * (r - l), c - l));
+    else
+        return minmod(th * (r - c), minmod(0.5 * (r - l), th * (c - l)));
 }
 // one axis of flux_kinetic.cpp:141-191; A at offsets -2..+2 along the axis
+template <bool TH1>
 __device__ __forceinline__ double axis_flux(double am2, double am1, double a0, double ap1,
                                             double ap2, double dir, double wq, double th)
 {
     double f;
     if (dir > 0) {
-        f = a0 + 0.5 * slopefit(am1, a0, ap1, th) - am1 - 0.5 * slopefit(am2, am1, a0, th);
+        f = a0 + 0.5 * slopefit<TH1>(am1, a0, ap1, th) - am1 -
+            0.5 * slopefit<TH1>(am2, am1, a0, th);
     } else {
-        f = ap1 - 0.5 * slopefit(a0, ap1, ap2, th) - a0 + 0.5 * slopefit(am1, a0, ap1, th);
+        f = ap1 - 0.5 * slopefit<TH1>(a0, ap1, ap2, th) - a0 +
+            0.5 * slopefit<TH1>(am1, a0, ap1, th);
     }
     return f * dir * wq;
 }
@@ -158,9 +166,18 @@ struct StencilArgs {
     int nq;
     double dx, dy, dz, theta, dt;
     int fuseEuler;
+    double rdx, rdy, rdz;  // 1/dx, ... (used when every spacing is a power of two)
 };
 
-template <int NM>
+// x / d == x * (1/d) exactly, for every x, iff d is a power of two
+bool exact_reciprocal(double d)
+{
+    int e = 0;
+    return d > 0.0 && std::isfinite(d) && std::frexp(d, &e) == 0.5 && e > -1021 && e < 1022;
+}
+
+// P2: dx, dy, dz powers of two (division by them is multiplication by exact reciprocals)
+template <int NM, bool P2, bool TH1>
 __global__ __launch_bounds__(kTB) void k_stencil(StencilArgs S, const double *__restrict__ A,
                                                   const double *__restrict__ M,
                                                   const double *__restrict__ xi,
@@ -189,13 +206,17 @@ __global__ __launch_bounds__(kTB) void k_stencil(StencilArgs S, const double *__
         const double *Aq = A + (size_t)q * ng;
         const double a0 = Aq[c];
         const double wq = w[q];
-        const double fX = axis_flux(Aq[c - 2 * sx], Aq[c - sx], a0, Aq[c + sx], Aq[c + 2 * sx],
-                                    xi[q], wq, S.theta);
-        const double fY = axis_flux(Aq[c - 2 * sy], Aq[c - sy], a0, Aq[c + sy], Aq[c + 2 * sy],
-                                    eta[q], wq, S.theta);
-        const double fZ = axis_flux(Aq[c - 2], Aq[c - 1], a0, Aq[c + 1], Aq[c + 2], mu[q], wq,
-                                    S.theta);
-        const double s = fX / S.dx + fY / S.dy + fZ / S.dz;
+        const double fX = axis_flux<TH1>(Aq[c - 2 * sx], Aq[c - sx], a0, Aq[c + sx],
+                                         Aq[c + 2 * sx], xi[q], wq, S.theta);
+        const double fY = axis_flux<TH1>(Aq[c - 2 * sy], Aq[c - sy], a0, Aq[c + sy],
+                                         Aq[c + 2 * sy], eta[q], wq, S.theta);
+        const double fZ = axis_flux<TH1>(Aq[c - 2], Aq[c - 1], a0, Aq[c + 1], Aq[c + 2], mu[q],
+                                         wq, S.theta);
+        double s;
+        if constexpr (P2)
+            s = fX * S.rdx + fY * S.rdy + fZ * S.rdz;
+        else
+            s = fX / S.dx + fY / S.dy + fZ / S.dz;
 #pragma unroll
         for (int m = 0; m < NM; ++m)
             fl[m] += M[(size_t)q * NM + m] * s;
@@ -495,7 +516,10 @@ hipError_t launch_stencil(c3d_ctx_s *ctx, int nx, int ny, int nz, const double *
                           double dx, double dy, double dz, double theta, int fuseEuler,
                           int zeroGhosts, hipStream_t s)
 {
-    StencilArgs S{make_grid(nx, ny, nz), ctx->t.nq, dx, dy, dz, theta, dt, fuseEuler};
+    StencilArgs S{make_grid(nx, ny, nz), ctx->t.nq, dx, dy, dz, theta, dt, fuseEuler,
+                  1.0 / dx, 1.0 / dy, 1.0 / dz};
+    const bool p2 = exact_reciprocal(dx) && exact_reciprocal(dy) && exact_reciprocal(dz);
+    const bool th1 = theta == 1.0;
     const int64_t ni = (int64_t)nx * ny * nz;
     dim3 grid(blocks_for(ni, kTB, INT32_MAX));
     if (zeroGhosts && !fuseEuler) {
@@ -503,9 +527,18 @@ hipError_t launch_stencil(c3d_ctx_s *ctx, int nx, int ny, int nz, const double *
         hipLaunchKernelGGL(k_zero_ghost_flux, dim3(blocks_for(ng * ctx->t.nm, 256, 65536)),
                            dim3(256), 0, s, S.G, ctx->t.nm, flux);
     }
+#define C3D_STENCIL_V(NMV, P, T)                                                              \
+    hipLaunchKernelGGL((k_stencil<NMV, P, T>), grid, dim3(kTB), 0, s, S, A, ctx->d_M,          \
+                       ctx->d_xi, ctx->d_eta, ctx->d_mu, ctx->d_w, flux, u, sigmaT, sigmaS)
 #define C3D_STENCIL(NMV)                                                                      \
-    hipLaunchKernelGGL(k_stencil<NMV>, grid, dim3(kTB), 0, s, S, A, ctx->d_M, ctx->d_xi,       \
-                       ctx->d_eta, ctx->d_mu, ctx->d_w, flux, u, sigmaT, sigmaS)
+    if (p2 && th1)                                                                             \
+        C3D_STENCIL_V(NMV, true, true);                                                        \
+    else if (p2)                                                                               \
+        C3D_STENCIL_V(NMV, true, false);                                                       \
+    else if (th1)                                                                              \
+        C3D_STENCIL_V(NMV, false, true);                                                       \
+    else                                                                                       \
+        C3D_STENCIL_V(NMV, false, false)
     switch (ctx->t.nm) {
     case 4: C3D_STENCIL(4); break;
     case 9: C3D_STENCIL(9); break;
@@ -514,6 +547,7 @@ hipError_t launch_stencil(c3d_ctx_s *ctx, int nx, int ny, int nz, const double *
     default: return hipErrorInvalidValue;
     }
 #undef C3D_STENCIL
+#undef C3D_STENCIL_V
     return hipGetLastError();
 }
 

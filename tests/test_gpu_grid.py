@@ -39,10 +39,15 @@ def test_ghost_fill_bit_exact(P, O, dev, T2):
     ctx.close()
 
 
-@pytest.mark.parametrize("isPn,theta", [(0, 2.0), (0, 1.0), (1, 2.0)])
-def test_flux_kinetic(P, O, dev, T2, isPn, theta):
+# dims (9, 8, 7): spacings 2/9, 1/4, 2/7 (divisions); (8, 8, 8): every spacing 1/4, a power of
+# two, where the kernel multiplies by exact reciprocals -- bit-exact either way for P_N
+@pytest.mark.parametrize("isPn,theta,dims", [(0, 2.0, (9, 8, 7)), (0, 1.0, (9, 8, 7)),
+                                             (1, 2.0, (9, 8, 7)), (1, 1.0, (9, 8, 7)),
+                                             (1, 2.0, (8, 8, 8)), (1, 1.0, (8, 8, 8)),
+                                             (0, 1.0, (8, 8, 8))])
+def test_flux_kinetic(P, O, dev, T2, isPn, theta, dims):
     import torch
-    nx, ny, nz = 9, 8, 7
+    nx, ny, nz = dims
     u, _, _ = _state(O, T2, nx, ny, nz)
     alpha = u.copy() if isPn else O.init_alpha(u)
     if not isPn:
