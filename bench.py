@@ -321,6 +321,14 @@ def step_workload(args, P, torch, dev):
     flux_ms = e0.elapsed_time(e1) / reps
     flux_bytes = 8.0 * (ng * nm + 2.0 * ng * nq + ni * nm)
     gbs = flux_bytes / (flux_ms * 1e-3) / 1e9
+    # the same call against the FP64 vector peak: ansatz exponents (2 nm + 1 per cell and
+    # ordinate on every cell) and the limited stencil (~90 + 2 nm per interior cell and ordinate)
+    flux_flops = ng * nq * (2.0 * nm + 1.0) + ni * nq * (90.0 + 2.0 * nm)
+    tfs = flux_flops / (flux_ms * 1e-3) / 1e12
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "pmc_traffic_step.json")
+    if os.path.exists(tp):
+        traffic = json.load(open(tp)).get("flux_call_bytes")
     cpu = None
     if args.cpu_sample > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -354,9 +362,14 @@ def step_workload(args, P, torch, dev):
                    "closure_iters_mean": round(float(st["totalIter"].mean()), 3),
                    "closure_iters_max": int(st["totalIter"].max())},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(gbs / 8000.0, 4), "traffic": None,
+                     "frac": round(gbs / 8000.0, 4),
+                     "traffic": round(traffic) if traffic else None,
+                     "traffic_source": "profiles/step_C2_flux_pmc_traffic.md" if traffic else None,
                      "kernel": "kinetic flux (k_ansatz + k_stencil), ansatz materialised",
-                     "bytes_per_launch": flux_bytes, "ms_per_launch": round(flux_ms, 4)},
+                     "bytes_per_launch": flux_bytes, "ms_per_launch": round(flux_ms, 4),
+                     "fp64_valu": {"achieved": round(tfs, 2), "peak": 78.6, "unit": "TFLOP/s",
+                                   "frac": round(tfs / 78.6, 4),
+                                   "flops_per_launch": flux_flops}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -208,7 +208,7 @@ class Ctx:
     def set_precision(self, mode):
         return check("c3d_set_precision", self.L.c3d_set_precision(self.h, mode))
 
-    def set_mixed(self, cond_max=1e8, max_iter=40, switch_tol=1e-4):
+    def set_mixed(self, cond_max=1e8, max_iter=40, switch_tol=5e-4):
         return check("c3d_set_mixed", self.L.c3d_set_mixed(self.h, cond_max, max_iter,
                                                            switch_tol))
 

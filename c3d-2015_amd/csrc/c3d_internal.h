@@ -79,7 +79,7 @@ struct c3d_ctx_s {
     int precision = C3D_PREC_FP64;
     double mixedCondMax = 1e8;  // C3D_PREC_MIXED hand-over to the FP64 path
     int mixedMaxIter = 40;
-    double mixedSwitch = 1e-4;
+    double mixedSwitch = 5e-4;
     // profiling of the last closure call
     bool profiling = false;
     std::vector<hipEvent_t> evPool;

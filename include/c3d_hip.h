@@ -173,7 +173,7 @@ int c3d_set_precision(c3d_ctx ctx, int mode);
  * factorisation or line search fails, or it has taken max_iter iterations (default 40);
  * it then takes exactly the FP64 path's decisions (regularisation included) and is flagged
  * C3D_FLAG_FP64_FALLBACK.  It continues from its iterate on the FP64 path once its gradient
- * norm falls below switch_tol (default 1e-4; C3D_FLAG_FP64_FINISH).  No counterpart in the
+ * norm falls below switch_tol (default 5e-4; C3D_FLAG_FP64_FINISH).  No counterpart in the
  * reference. */
 int c3d_set_mixed(c3d_ctx ctx, double cond_max, int max_iter, double switch_tol);
 
