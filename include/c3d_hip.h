@@ -169,7 +169,8 @@ int c3d_closure(c3d_ctx ctx, int64_t ncells, double *u, double *alpha,
 int c3d_set_precision(c3d_ctx ctx, int mode);
 
 /* Parameters of C3D_PREC_MIXED.  A cell leaves the FP32-Hessian phase and restarts from its
- * input on the FP64 path when its condition estimate exceeds cond_max (default 1e8), its
+ * input on the FP64 path when (max_i L_ii / min_i L_ii)^2 of its Cholesky factor, a lower
+ * bound on cond_2(H) that stands in for dpocon in this phase, exceeds cond_max (default 1e8), its
  * factorisation or line search fails, or it has taken max_iter iterations (default 40);
  * it then takes exactly the FP64 path's decisions (regularisation included) and is flagged
  * C3D_FLAG_FP64_FALLBACK.  It continues from its iterate on the FP64 path once its gradient
