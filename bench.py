@@ -80,10 +80,13 @@ def main():
     ap.add_argument("--dual-steps", type=int, default=10000, help="dual-gap steps")
     ap.add_argument("--train-batch", type=int, default=8192)
     ap.add_argument("--train-lr", type=float, default=3e-3)
-    ap.add_argument("--workload", choices=("closure", "step"), default="closure",
-                    help="closure: the metric's batch (default); step: M_2 SSP-RK2 time "
-                         "stepping on a 128^3 grid (kinetic flux roofline)")
-    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--workload", choices=("closure", "step", "orders"), default="closure",
+                    help="closure: the metric's batch (default); step: SSP-RK2 time "
+                         "stepping (M_2, Q=8, 128^3 by default; kinetic flux roofline); "
+                         "orders: 2^20 cells, a quarter each of M_1..M_4, mixed+nn vs FP64")
+    ap.add_argument("--grid", type=int, default=128, help="step: cells per axis")
+    ap.add_argument("--order", type=int, default=2, help="step: moment order N")
+    ap.add_argument("--quad", type=int, default=8, help="step: quadrature order Q")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the closure is "
@@ -96,6 +99,8 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "step":
         return step_workload(args, P, torch, dev)
+    if args.workload == "orders":
+        return orders_workload(args, P, torch, dev)
     N, Q = 3, 24
     n = args.cells
     ctx = P.Ctx(N, Q)
@@ -269,14 +274,15 @@ def main():
 
 
 def step_workload(args, P, torch, dev):
-    """M_2 (9 moments), Q = 8 (128 ordinates), periodic n^3 grid on [-1,1]^3, pulse
-    rho = max(exp(-|x|^2/(2 0.05^2)), 1e-4) (1 + 0.01 U[-1,1]), sigma_s = 1, sigma_a = 0,
+    """M_N (default N = 2: 9 moments), Q (default 8: 128 ordinates), periodic n^3 grid on
+    [-1,1]^3, pulse rho = max(exp(-|x|^2/(2 0.05^2)), 1e-4) (1 + 0.01 U[-1,1]), sigma_s = 1,
+    sigma_a = 0,
     minmod slopes (theta = 1), CFL 0.5 (dt of init.cpp:274), SSP-RK2 with the closure on
     every cell (the reference's default batched semantics, tol 1e-8), FP64.  A step is one
     full update() (update.cpp:185-275) on the device; the state keeps evolving across
     steps.  Also timed: the kinetic flux alone (ansatz + limited upwind stencil), against
     its algorithmic HBM bytes with the ansatz materialised."""
-    N, Q, n = 2, 8, args.grid
+    N, Q, n = args.order, args.quad, args.grid
     ctx = P.Ctx(N, Q)
     nm, nq = ctx.nm, ctx.nq
     ng = (n + 4) ** 3
@@ -325,16 +331,16 @@ def step_workload(args, P, torch, dev):
     # ordinate on every cell) and the limited stencil (~90 + 2 nm per interior cell and ordinate)
     flux_flops = ng * nq * (2.0 * nm + 1.0) + ni * nq * (90.0 + 2.0 * nm)
     tfs = flux_flops / (flux_ms * 1e-3) / 1e12
-    traffic = None
+    traffic = None  # PMC-measured bytes per flux call, profiled for M_2, Q = 8, 128^3 only
     tp = os.path.join(ROOT, "profiles", "pmc_traffic_step.json")
-    if os.path.exists(tp):
+    if os.path.exists(tp) and (N, Q, n) == (2, 8, 128):
         traffic = json.load(open(tp)).get("flux_call_bytes")
     cpu = None
     if args.cpu_sample > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
         T = O.Tables(N, Q)
-        m = 32
+        m = 32 if nm * nq <= 2048 else 12
         dm = 2.0 / m
         uc, sTc, sSc = O.init_grid(m, m, m, nm, ic=1, sigma=0.05, floor=1e-4, noise=0.01,
                                    seed=args.seed)
@@ -349,15 +355,16 @@ def step_workload(args, P, torch, dev):
         secs = time.perf_counter() - t0
         cpu = {"value": round(m ** 3 / secs, 1), "unit": "cell-updates/s", "cores": threads,
                "kind": "port", "sample": f"one SSP-RK2 step of the oracle on the same "
-                                         f"problem at 32^3, {secs:.1f} s"}
+                                         f"problem at {m}^3, {secs:.1f} s"}
     out = {
-        "metric": "M_2 SSP-RK2 interior cell-updates/s (time stepping, one GPU)",
+        "metric": f"M_{N} SSP-RK2 interior cell-updates/s (time stepping, one GPU)",
         "value": round(ni / (ms / 1e3), 1), "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"M_2 time stepping, {n}^3 periodic grid, Q=8 (128 ordinates), "
-                               "pulse sigma 0.05 + 1% noise, theta 1, CFL 0.5, closure tol 1e-8",
+        "config": {"workload": f"M_{N} time stepping, {n}^3 periodic grid, Q={Q} ({nq} "
+                               "ordinates), pulse sigma 0.05 + 1% noise, theta 1, CFL 0.5, "
+                               "closure tol 1e-8",
                    "grid": n, "moment_order": N, "quad_order": Q, "dt": dt, "status": rc,
                    "closure_iters_mean": round(float(st["totalIter"].mean()), 3),
                    "closure_iters_max": int(st["totalIter"].max())},
@@ -371,6 +378,95 @@ def step_workload(args, P, torch, dev):
                                    "frac": round(tfs / 78.6, 4),
                                    "flops_per_launch": flux_flops}},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def orders_workload(args, P, torch, dev):
+    """2^20 independent cells, a quarter each of M_1, M_2, M_3, M_4 (Q = 24), alpha* as in the
+    metric's batch.  Each order has its own context and warm-start network (trained on a
+    quarter of --train-pairs solver pairs of that order); one step solves the four quarters
+    back to back on one stream, mixed+nn, then the same with the plain FP64 path."""
+    Q, n = 24, args.cells
+    opts = P.closure_opts(maxIter=100, tol=1e-8, tolGamma=1.1, condHMax=1e10)
+    parts = []
+    for k, N in enumerate((1, 2, 3, 4)):
+        ctx = P.Ctx(N, Q)
+        ctx.set_mixed(args.mixed_cond, args.mixed_iter)
+        nm = ctx.nm
+        nc = n // 4
+        u0 = torch.empty((nc, nm), dtype=torch.float64, device=dev)
+        a0 = torch.empty_like(u0)
+        ctx.make_closure_batch(nc, args.seed + 16 * k, 0.5, 8.0, u0, a0)
+        m = args.train_pairs // 4
+        ut = torch.empty((m, nm), dtype=torch.float64, device=dev)
+        at = torch.empty_like(ut)
+        ctx.set_precision(P.PREC_FP64)
+        ctx.make_closure_batch(m, args.seed + 16 * k + 1, 0.5, 8.0, ut, at)
+        ctx.closure(ut, at, opts)
+        ctx.sync()
+        ctx.mlp_train(ut, at, mse_steps=args.train_steps, dual_steps=args.dual_steps,
+                      batch=args.train_batch, lr=args.train_lr, seed=args.seed)
+        del ut, at
+        parts.append({"N": N, "ctx": ctx, "u0": u0, "a0": a0, "u": torch.empty_like(u0),
+                      "a": torch.empty_like(a0), "st": P.alloc_stats(nc, dev)})
+    torch.cuda.empty_cache()
+    log("trained four warm starts")
+
+    def run(path, steps, warmup):
+        for p in parts:
+            p["ctx"].set_precision(P.PREC_MIXED if path == "mixed+nn" else P.PREC_FP64)
+            p["ctx"].set_warmstart(path == "mixed+nn")
+
+        def step():
+            for p in parts:
+                p["u"].copy_(p["u0"])
+                p["a"].copy_(p["a0"])
+                p["ctx"].closure(p["u"], p["a"], opts, stats=p["st"])
+
+        for _ in range(warmup):
+            step()
+        for p in parts:
+            p["ctx"].sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        rc = 0
+        for p in parts:
+            rc = rc or p["ctx"].sync()
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        per = []
+        for p in parts:
+            st = P.stats_to_numpy(p["st"])
+            per.append({"N": p["N"], "iters_mean": round(float(st["totalIter"].mean()), 3),
+                        "fallback_frac": round(float(((st["flags"] & P.FLAG_FP64_FALLBACK) != 0)
+                                                     .mean()), 5),
+                        "u": p["u"].cpu().numpy(), "reg": st["regIndex"].copy()})
+        log(f"{path}: {ms:.1f} ms/step")
+        return ms, rc, per
+
+    ms_m, rc_m, per_m = run("mixed+nn", args.steps, args.warmup)
+    ms_f, rc_f, per_f = run("fp64", args.steps, args.warmup)
+    orders = []
+    for pm, pf in zip(per_m, per_f):
+        du = np.abs(pm["u"] - pf["u"]).max(axis=1) / np.abs(pf["u"][:, 0])
+        orders.append({"N": pm["N"], "iters_mixed_nn": pm["iters_mean"],
+                       "iters_fp64": pf["iters_mean"], "fallback_frac": pm["fallback_frac"],
+                       "moments_within_1e-6": round(float((du <= 1e-6).mean()), 6),
+                       "rung_agreement": round(float((pm["reg"] == pf["reg"]).mean()), 6)})
+    out = {
+        "metric": "M_N cell-updates/s at ||grad h||<=1e-8, orders 1-4 mixed, mixed+NN, one GPU",
+        "value": round(n / (ms_m / 1e3), 1), "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_m, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"closure-only batch: {n} cells, a quarter each of M_1..M_4, "
+                               "Q=24, alpha*~N(0,s^2), s~U[0.5,8]",
+                   "status": rc_m, "fp64_ms_per_step": round(ms_f, 3), "fp64_status": rc_f,
+                   "speedup_vs_fp64": round(ms_f / ms_m, 3), "orders": orders},
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
 
