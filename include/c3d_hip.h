@@ -198,8 +198,8 @@ int c3d_set_profiling(c3d_ctx ctx, int enable);
 int c3d_get_closure_profile(c3d_ctx ctx, c3d_closure_profile *out);
 
 /* Upper bound on the sweep passes one c3d_closure call may run (default 100000).  A pass
- * evaluates every unconverged cell once (a Newton iteration or one line-search trial; up to
- * four trials in the fused kernel's four-wave form, see c3d_set_tail_wide); the
+ * evaluates every unconverged cell once (a Newton iteration or one line-search trial; a
+ * round of the fused kernel's four-wave form settles up to 32, see c3d_set_tail_wide); the
  * reference has no such bound (its worst case is 1400 iterations x ~1100 backtracking
  * steps per cell).  No counterpart in the reference. */
 int c3d_set_pass_budget(c3d_ctx ctx, int max_passes);
@@ -211,11 +211,12 @@ int c3d_set_pass_budget(c3d_ctx ctx, int max_passes);
  * disables the fused kernel.  Defaults 8192 and 32.  No counterpart in the reference. */
 int c3d_set_tail(c3d_ctx ctx, int max_cells, int passes_per_launch);
 
-/* Once at most max_cells cells remain, the fused kernel runs four wavefronts per cell: each
- * evaluates f at one of the next four steps of the backtracking line search (t, t/2, t/4,
- * t/8), so a round settles up to four trials of the sequential search, with the same
- * accept/reject decisions and results.  0 keeps one wavefront per cell.  Default 512.  No
- * counterpart in the reference. */
+/* Once at most max_cells cells remain, the fused kernel runs four wavefronts per cell that
+ * evaluate f at the next steps of the backtracking line search (t, t/2, t/4, ...): one step
+ * per wavefront in a search's first round, then 8 (nm <= 9), 2 (nm = 16) or 1 (nm = 25) each,
+ * so a round settles up to 4-32 trials of the sequential search, with the same accept/reject
+ * decisions and results.  0 keeps one wavefront per cell.  Default 512.  No counterpart in
+ * the reference. */
 int c3d_set_tail_wide(c3d_ctx ctx, int max_cells);
 
 /* H(alpha) for ncells cells (solveHMn semantics), device pointers, hs [ncells][nm][nm].
