@@ -87,6 +87,10 @@ def main():
     ap.add_argument("--grid", type=int, default=128, help="step: cells per axis")
     ap.add_argument("--order", type=int, default=2, help="step: moment order N")
     ap.add_argument("--quad", type=int, default=8, help="step: quadrature order Q")
+    ap.add_argument("--step-precision", choices=("fp64", "mixed"), default="fp64",
+                    help="step: mixed runs the mixed-precision closure's trajectory and the "
+                         "reference-order FP64 one beside the FP64 one from the same state "
+                         "and reports both differences from it after every step")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the closure is "
@@ -281,7 +285,14 @@ def step_workload(args, P, torch, dev):
     every cell (the reference's default batched semantics, tol 1e-8), FP64.  A step is one
     full update() (update.cpp:185-275) on the device; the state keeps evolving across
     steps.  Also timed: the kinetic flux alone (ansatz + limited upwind stencil), against
-    its algorithmic HBM bytes with the ansatz materialised."""
+    its algorithmic HBM bytes with the ansatz materialised.
+
+    --step-precision mixed: the same problem advanced from one initial state with the FP64
+    closure, with the mixed-precision one and with the reference-order FP64 one; `value` is
+    the mixed trajectory's rate, and after every step the interior cells' difference from
+    the FP64 trajectory is reported for the other two: max over cells of
+    max_k |dalpha_k| / max_k |alpha_k|, of max_k |du_k| / u_0, of the density's relative
+    difference, and the share of cells whose moments agree to 1e-6 u_0."""
     N, Q, n = args.order, args.quad, args.grid
     ctx = P.Ctx(N, Q)
     nm, nq = ctx.nm, ctx.nq
@@ -300,19 +311,62 @@ def step_workload(args, P, torch, dev):
     stats = P.alloc_stats(2 * ng, dev)
     stream = torch.cuda.current_stream()
 
-    def step():
-        ctx.update(n, n, n, u, a, sT, sS, d, d, d, theta, dt, 1, 0, opts, stats=stats)
+    def step(uu, aa):
+        ctx.update(n, n, n, uu, aa, sT, sS, d, d, d, theta, dt, 1, 0, opts, stats=stats)
 
-    for _ in range(args.warmup):
-        step()
-    rc = ctx.sync()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    rc = rc or ctx.sync()
-    torch.cuda.synchronize()
-    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    mixed = args.step_precision == "mixed"
+    errors, fp64_ms = [], None
+    if not mixed:
+        for _ in range(args.warmup):
+            step(u, a)
+        rc = ctx.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(u, a)
+        rc = rc or ctx.sync()
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    else:
+        # three trajectories from the same state: the FP64 closure's, the mixed one's and,
+        # as the yardstick of the FP64 closure's own sensitivity to rounding, the
+        # reference-order FP64 one (other summation order, other exp); each step is timed
+        # on its own (the other trajectories' steps are not in it)
+        uF, aF = u.clone(), a.clone()
+        uR, aR = u.clone(), a.clone()
+        rc, times = 0, {"fp64": [], "mixed": [], "reforder": []}
+        inner = lambda x: x.view(n + 4, n + 4, n + 4, nm)[2:-2, 2:-2, 2:-2].reshape(-1, nm)
+
+        def diff(uu, aa):
+            ui, uf, ai, af = inner(uu), inner(uF), inner(aa), inner(aF)
+            du = (ui - uf).abs().amax(dim=1) / uf[:, 0].abs()
+            da = (ai - af).abs().amax(dim=1) / af.abs().amax(dim=1)
+            drho = (ui[:, 0] - uf[:, 0]).abs() / uf[:, 0].abs()
+            return {"alpha_max_rel": float(da.max()), "moments_max_rel_u0": float(du.max()),
+                    "rho_max_rel": float(drho.max()),
+                    "cells_moments_within_1e-6": round(float((du <= 1e-6).double().mean()), 6)}
+
+        for k in range(args.warmup + args.steps):
+            for name, (uu, aa), prec in (("fp64", (uF, aF), P.PREC_FP64),
+                                         ("mixed", (u, a), P.PREC_MIXED),
+                                         ("reforder", (uR, aR), P.PREC_FP64_REFORDER)):
+                ctx.set_precision(prec)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                step(uu, aa)
+                torch.cuda.synchronize()
+                if k >= args.warmup:
+                    times[name].append(time.perf_counter() - t0)
+                rc = rc or ctx.sync()
+            e = {"step": k + 1, "mixed": diff(u, a), "fp64_reference_order": diff(uR, aR)}
+            errors.append(e)
+            log(f"step {k + 1}: " + " | ".join(
+                f"{key}: alpha {v['alpha_max_rel']:.2e} moments {v['moments_max_rel_u0']:.2e} "
+                f"rho {v['rho_max_rel']:.2e} within 1e-6 {v['cells_moments_within_1e-6']}"
+                for key, v in e.items() if key != "step"))
+        ctx.set_precision(P.PREC_FP64)
+        ms = 1e3 * sum(times["mixed"]) / args.steps
+        fp64_ms = 1e3 * sum(times["fp64"]) / args.steps
     st = P.stats_to_numpy(stats)
     # the kinetic flux alone, HIP events on the stream it is launched on
     flux = torch.zeros_like(u)
@@ -356,18 +410,26 @@ def step_workload(args, P, torch, dev):
         cpu = {"value": round(m ** 3 / secs, 1), "unit": "cell-updates/s", "cores": threads,
                "kind": "port", "sample": f"one SSP-RK2 step of the oracle on the same "
                                          f"problem at {m}^3, {secs:.1f} s"}
+    prec = ("mixed-precision closure (FP32 matrix-core Hessian, FP64 finish)" if mixed
+            else "FP64 closure")
     out = {
         "metric": f"M_{N} SSP-RK2 interior cell-updates/s (time stepping, one GPU)",
         "value": round(ni / (ms / 1e3), 1), "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic",
         "config": {"workload": f"M_{N} time stepping, {n}^3 periodic grid, Q={Q} ({nq} "
                                "ordinates), pulse sigma 0.05 + 1% noise, theta 1, CFL 0.5, "
-                               "closure tol 1e-8",
+                               f"closure tol 1e-8, {prec}",
                    "grid": n, "moment_order": N, "quad_order": Q, "dt": dt, "status": rc,
                    "closure_iters_mean": round(float(st["totalIter"].mean()), 3),
                    "closure_iters_max": int(st["totalIter"].max())},
+        "fp64_trajectory": None if not mixed else {
+            "ms_per_step": round(fp64_ms, 3), "value": round(ni / (fp64_ms / 1e3), 1),
+            "speedup_of_mixed": round(fp64_ms / ms, 3),
+            "reference_order_ms_per_step": round(1e3 * sum(times["reforder"]) / args.steps, 3),
+            "difference_from_fp64_after_each_step": errors},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(gbs / 8000.0, 4),
                      "traffic": round(traffic) if traffic else None,
