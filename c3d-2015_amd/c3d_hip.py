@@ -220,10 +220,10 @@ class Ctx:
         if wide_cells is not None:
             check("c3d_set_tail_wide", self.L.c3d_set_tail_wide(self.h, wide_cells))
 
-    def hessian(self, alpha, hs, stream=None):
+    def hessian(self, alpha, hs, stream=None, gaunt=False, sparse=False):
         n = alpha.numel() // self.nm
-        return check("c3d_hessian", self.L.c3d_hessian(self.h, n, _p(alpha), _p(hs), 0, 0,
-                                                       _stream(stream)))
+        return check("c3d_hessian", self.L.c3d_hessian(self.h, n, _p(alpha), _p(hs), int(gaunt),
+                                                       int(sparse), _stream(stream)))
 
     def sync(self, stream=None):
         return self.L.c3d_sync(self.h, _stream(stream))

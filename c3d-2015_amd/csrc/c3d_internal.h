@@ -93,6 +93,7 @@ struct c3d_ctx_s {
     int tailCells = 8192;
     int tailPasses = 32;
     int tailWideCells = 512;  // four waves per cell (speculative line-search trials) below this
+    int gauntDirect[2] = {-1, -1};  // Gaunt form (dense, CSR) equals the direct form: 1, 0, -1 unknown
     std::mutex mtx;
 };
 

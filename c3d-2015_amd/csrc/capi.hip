@@ -95,6 +95,65 @@ int finish_ctx(c3d_ctx_s *ctx)
 
 bool valid_order(int N) { return N >= 1 && N <= 4; }
 
+// The Gaunt form of the Hessian (fobj.cpp:59-95) is H_ij = sum_k beta_ijk G_k with
+// G_k = sum_q w_q e_q m2_qk.  Where beta reproduces the basis products at every ordinate,
+// sum_k beta_ijk m2_qk = m_qi m_qj, that H is sum_q w_q e_q m_qi m_qj, the direct form's,
+// up to the identity's residual.  Products of real spherical harmonics of degree <= N are
+// combinations of those of degree <= 2N, so the identity holds once the quadrature resolves
+// degree 2N, and the device then evaluates the direct form.  Checked on the host once per
+// context and form (dense beta, or its CSR without the entries |beta| <= 1e-13); the caller
+// holds ctx->mtx.
+bool gaunt_matches_direct(c3d_ctx_s *ctx, bool sparse)
+{
+    int &flag = ctx->gauntDirect[sparse ? 1 : 0];
+    if (flag >= 0)
+        return flag == 1;
+    const HostTables &t = ctx->t;
+    const int nm = t.nm, nm2 = t.nm2, nq = t.nq;
+    bool ok = nm2 > 0 && t.M2.size() == (size_t)nq * nm2 &&
+              (sparse ? t.betaRowPtr.size() == (size_t)nm * nm + 1
+                      : t.beta.size() == (size_t)nm * nm * nm2);
+    double worst = 0.0, scale = 0.0;
+    for (int q = 0; ok && q < nq; ++q) {
+        const double *m = &t.M[(size_t)q * nm], *m2 = &t.M2[(size_t)q * nm2];
+        for (int i = 0; i < nm; ++i)
+            for (int j = 0; j < nm; ++j) {
+                const int ij = i * nm + j;
+                double s = 0.0;
+                if (sparse) {
+                    for (int e = t.betaRowPtr[ij]; e < t.betaRowPtr[ij + 1]; ++e) {
+                        const int k = t.betaColInd[e];
+                        if (k < 0 || k >= nm2) {
+                            ok = false;
+                            break;
+                        }
+                        s += t.betaVal[e] * m2[k];
+                    }
+                } else {
+                    for (int k = 0; k < nm2; ++k)
+                        s += t.beta[(size_t)ij * nm2 + k] * m2[k];
+                }
+                const double prod = m[i] * m[j];
+                worst = std::fmax(worst, std::fabs(s - prod));
+                scale = std::fmax(scale, std::fabs(prod));
+            }
+    }
+    ok = ok && std::isfinite(worst) && worst <= 1e-10 * scale;
+    flag = ok ? 1 : 0;
+    return ok;
+}
+
+// A closure or Hessian request in the Gaunt form: served by the direct form where the two
+// agree (above); not in the reference-order mode, whose contract is the reference's bits.
+int gaunt_status(c3d_ctx_s *ctx, int useGaunt, int useGauntSparse)
+{
+    if (!useGaunt)
+        return C3D_OK;
+    if (ctx->precision == C3D_PREC_FP64_REFORDER)
+        return C3D_ERR_UNSUPPORTED;
+    return gaunt_matches_direct(ctx, useGauntSparse != 0) ? C3D_OK : C3D_ERR_UNSUPPORTED;
+}
+
 int hip_status(hipError_t e) { return e == hipSuccess ? C3D_OK : C3D_ERR_HIP; }
 
 }  // namespace
@@ -254,10 +313,12 @@ int c3d_closure(c3d_ctx ctx, int64_t ncells, double *u, double *alpha,
 {
     if (!ctx || !opts || ncells < 0 || (ncells > 0 && (!u || !alpha)))
         return C3D_ERR_ARG;
-    if (opts->useGaunt)
-        return C3D_ERR_UNSUPPORTED;
     if (opts->maxIter < 1)
         return C3D_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    const int g = gaunt_status(ctx, opts->useGaunt, opts->useGauntSparse);
+    if (g != C3D_OK)
+        return g;
     return hip_status(launch_closure_fp64(ctx, ncells, u, alpha, *opts, stats,
                                           (hipStream_t)stream));
 }
@@ -477,11 +538,12 @@ int c3d_set_pass_budget(c3d_ctx ctx, int max_passes)
 int c3d_hessian(c3d_ctx ctx, int64_t ncells, const double *alpha, double *hs, int useGaunt,
                 int useGauntSparse, void *stream)
 {
-    (void)useGauntSparse;
     if (!ctx || ncells < 0 || (ncells > 0 && (!alpha || !hs)))
         return C3D_ERR_ARG;
-    if (useGaunt)
-        return C3D_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    const int g = gaunt_status(ctx, useGaunt, useGauntSparse);
+    if (g != C3D_OK)
+        return g;
     return hip_status(launch_hessian_fp64(ctx, ncells, alpha, hs, (hipStream_t)stream));
 }
 
@@ -574,9 +636,14 @@ int c3d_update(c3d_ctx ctx, int nx, int ny, int nz, double *u, double *alpha,
 {
     if (!ctx || nx < 2 || ny < 2 || nz < 2 || !u || !sigmaT || !sigmaS)
         return C3D_ERR_ARG;
-    if (solver == 1 && (!alpha || !opts || opts->useGaunt || opts->maxIter < 1))
-        return (opts && opts->useGaunt) ? C3D_ERR_UNSUPPORTED : C3D_ERR_ARG;
+    if (solver == 1 && (!alpha || !opts || opts->maxIter < 1))
+        return C3D_ERR_ARG;
     std::lock_guard<std::mutex> lk(ctx->mtx);
+    if (solver == 1) {
+        const int g = gaunt_status(ctx, opts->useGaunt, opts->useGauntSparse);
+        if (g != C3D_OK)
+            return g;
+    }
     const int nm = ctx->t.nm;
     const size_t ng = (size_t)(nx + 4) * (ny + 4) * (nz + 4);
     hipStream_t s = (hipStream_t)stream;

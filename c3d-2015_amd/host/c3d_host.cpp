@@ -99,7 +99,7 @@ struct HostState {
     std::mutex mtx;
     c3d_ctx ctx = nullptr;
     int nm = 0, nq = 0;
-    const double *w = nullptr, *m = nullptr, *xi = nullptr;
+    const double *w = nullptr, *m = nullptr, *xi = nullptr, *beta = nullptr;
     int precision = C3D_PREC_FP64;
     DevArray u, alpha, flux, stats, sig;
 };
@@ -121,12 +121,17 @@ int quad_of(int nq)
     return 2 * q * q == nq ? q : -1;
 }
 
-// (Re)build the context when the tables change; caller holds the mutex.
+// (Re)build the context when the tables change; caller holds the mutex.  With gaunt, the
+// context also takes the reference's Gaunt tables (g_beta and its CSR, which initMn fills
+// when MN_USE_GAUNT_COEF is true: init.cpp:287-337), against which the library checks that
+// the Gaunt form equals the direct one before it evaluates the latter.
 c3d_ctx context(int nm, int nq, const double *w, const double *xi, const double *eta,
-                const double *mu, const double *M, const double *M2)
+                const double *mu, const double *M, const double *M2, bool gaunt)
 {
     HostState &S = state();
-    if (S.ctx && S.nm == nm && S.nq == nq && S.w == w && S.m == M && S.xi == xi)
+    const double *beta = gaunt ? g_beta : nullptr;
+    if (S.ctx && S.nm == nm && S.nq == nq && S.w == w && S.m == M && S.xi == xi &&
+        S.beta == beta)
         return S.ctx;
     if (S.ctx)
         check("c3d_destroy", c3d_destroy(S.ctx));
@@ -140,21 +145,25 @@ c3d_ctx context(int nm, int nq, const double *w, const double *xi, const double 
         zero.assign(nq, 0.0);
         xi = eta = mu = zero.data();
     }
+    const bool csr = beta && g_betaVal && g_betaColInd && g_betaRowPtr;
     check("c3d_create_from_tables",
-          c3d_create_from_tables(&S.ctx, N, Q, w, xi, eta, mu, M, M2, nullptr, 0, nullptr,
-                                 nullptr, nullptr));
+          c3d_create_from_tables(&S.ctx, N, Q, w, xi, eta, mu, M, M2, beta,
+                                 csr ? g_betaRowPtr[nm * nm] : 0, csr ? g_betaVal : nullptr,
+                                 csr ? g_betaColInd : nullptr, csr ? g_betaRowPtr : nullptr));
     check("c3d_set_precision", c3d_set_precision(S.ctx, S.precision));
     S.nm = nm;
     S.nq = nq;
     S.w = w;
     S.m = M;
     S.xi = xiKey;
+    S.beta = beta;
     return S.ctx;
 }
 
 c3d_ctx grid_context()
 {
-    return context(g_numMoments, g_numQuadPoints, g_w, g_xi, g_eta, g_mu, g_m, g_m2);
+    return context(g_numMoments, g_numQuadPoints, g_w, g_xi, g_eta, g_mu, g_m, g_m2,
+                   g_mnUseGaunt);
 }
 
 size_t grid_cells()
@@ -205,7 +214,7 @@ void opt(int nm, int nm2, int nq, double *u, double *alpha, OPTIONS options, dou
     (void)nm2;
     HostState &S = state();
     std::lock_guard<std::mutex> lk(S.mtx);
-    c3d_ctx ctx = context(nm, nq, w, nullptr, nullptr, nullptr, p, p2);
+    c3d_ctx ctx = context(nm, nq, w, nullptr, nullptr, nullptr, p, p2, options.useGaunt);
     c3d_closure_opts o{options.maxIter, options.useGaunt ? 1 : 0, options.useGauntSparse ? 1 : 0,
                        1, options.tolRel, options.tolGamma, options.condHMax};
     double *du = (double *)S.u.get(nm * sizeof(double));
@@ -300,7 +309,7 @@ void Optimization::opt(int nm, int nm2, int nq)
     }
     HostState &S = state();
     std::lock_guard<std::mutex> lk(S.mtx);
-    c3d_ctx ctx = context(nm, nq, g_w, g_xi, g_eta, g_mu, g_m, g_m2);
+    c3d_ctx ctx = context(nm, nq, g_w, g_xi, g_eta, g_mu, g_m, g_m2, g_mnUseGaunt);
     c3d_closure_opts o{g_mnMaxIter, g_mnUseGaunt ? 1 : 0, g_mnUseGauntSparse ? 1 : 0, 0, g_mnTol,
                        g_mnTolGamma, g_mnCondHMax};
     double *du = (double *)S.u.get(u.size() * sizeof(double));

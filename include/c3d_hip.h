@@ -67,7 +67,13 @@ typedef struct c3d_ctx_s *c3d_ctx;
  *                  tolGamma from the deck, the custom dpotrs (utils.cpp:71-86).
  *   unbatched = 1: opt()/optScaled semantics (opt.cpp:139-355): tol relative to
  *                  ||u/u0||_2 (OPTIONS.tolRel), tolGamma from the options (update() passes
- *                  1.1, update.cpp:57; c3d_update does the same), LAPACK dpotrs order. */
+ *                  1.1, update.cpp:57; c3d_update does the same), LAPACK dpotrs order.
+ *   useGaunt = 1 (dense beta, or its CSR with useGauntSparse = 1; fobj.cpp:59-95) asks for H
+ *                  as sum_k beta_ijk G_k.  The library checks once per context that beta
+ *                  reproduces the basis products m_i m_j at every ordinate (to 1e-10 of their
+ *                  largest value), where that H equals the direct form's, and then evaluates
+ *                  the direct form; otherwise, and in C3D_PREC_FP64_REFORDER (whose contract
+ *                  is the reference's bits), it returns C3D_ERR_UNSUPPORTED. */
 typedef struct {
     int32_t maxIter;        /* MN_MAX_ITER */
     int32_t useGaunt;       /* MN_USE_GAUNT_COEF */
@@ -220,7 +226,8 @@ int c3d_set_tail(c3d_ctx ctx, int max_cells, int passes_per_launch);
 int c3d_set_tail_wide(c3d_ctx ctx, int max_cells);
 
 /* H(alpha) for ncells cells (solveHMn semantics), device pointers, hs [ncells][nm][nm].
- * Under C3D_PREC_MIXED the sums run on the FP32 matrix cores (relative error ~1e-6). */
+ * Under C3D_PREC_MIXED the sums run on the FP32 matrix cores (relative error ~1e-6).  The
+ * Gaunt form is served as c3d_closure_opts describes. */
 int c3d_hessian(c3d_ctx ctx, int64_t ncells, const double *alpha, double *hs, int useGaunt,
                 int useGauntSparse, void *stream);
 

@@ -136,6 +136,58 @@ def test_hessian_matches_oracle(P, O, dev, N, Q):
     assert rel < 1e-14, rel
 
 
+@pytest.mark.parametrize("sparse", [False, True])
+@pytest.mark.parametrize("N,Q", [(2, 8), (3, 12), (4, 10)])
+def test_hessian_gaunt_form_matches_oracle(P, O, dev, N, Q, sparse):
+    """The Gaunt form (fobj.cpp:59-95, dense beta or its CSR) on the device is the direct
+    form, which equals the oracle's sum_k beta_ijk G_k to the beta fit's residual."""
+    import torch
+    T = _tables(O, N, Q)
+    rng = np.random.default_rng(N * 10 + Q)
+    n = 29
+    a = rng.normal(0, 0.7, (n, T.nm))
+    h_ref = np.zeros((n, T.nm, T.nm))
+    gx = np.zeros(T.nm2)
+    for c in range(n):
+        O.lib().orc_solveH(T.nm, T.nm2, T.nq, np.ascontiguousarray(a[c]), O.C.byref(T.c), 1,
+                           int(sparse), h_ref[c], gx)
+    ctx = P.Ctx(tables=T)
+    ad = torch.from_numpy(a).to(dev)
+    hd = torch.zeros((n, T.nm, T.nm), dtype=torch.float64, device=dev)
+    ctx.hessian(ad, hd, gaunt=True, sparse=sparse)
+    torch.cuda.synchronize()
+    rel = np.abs(hd.cpu().numpy() - h_ref).max() / np.abs(h_ref).max()
+    assert rel < 1e-12, rel
+    # the reference-order mode promises the reference's bits, which the direct form does not
+    # give for the Gaunt form
+    ctx.set_precision(P.PREC_FP64_REFORDER)
+    with pytest.raises(P.C3DError) as e:
+        ctx.hessian(ad, hd, gaunt=True, sparse=sparse)
+    assert e.value.rc == P.C3D_ERR_UNSUPPORTED
+    ctx.close()
+
+
+@pytest.mark.parametrize("sparse", [False, True])
+@pytest.mark.parametrize("N,Q,n", [(3, 24, 2000), (2, 8, 1500)])
+def test_closure_gaunt_form_matches_oracle(P, O, dev, N, Q, n, sparse):
+    """MN_USE_GAUNT_COEF true (the reference deck's default) against the oracle in the Gaunt
+    form: the tolerance-ball bar of the direct form.  The share of identical outcome tuples
+    is measured against the oracle's own direct-vs-Gaunt agreement on the same cells (91.6%
+    at M_2 with 128 ordinates, where near-singular cells take other ladder rungs when H
+    changes at rounding level; 99.6% at M_3 with 1152): the device may fall short of it by
+    at most 2 points."""
+    T = _tables(O, N, Q)
+    a_star, u, a0 = O.closure_batch_input(T, n, 0xC3D2015 + 7 * N)
+    _, s_dir = O.closure(T, u.copy(), a0.copy(), O.opts())
+    _, s_gau = O.closure(T, u.copy(), a0.copy(), O.opts(useGaunt=True, useGauntSparse=sparse))
+    floor = np.mean(np.all([s_dir[k] == s_gau[k] for k in KEYS], axis=0))
+    so, sg, same, rc_o, rc_g = _compare(O, T, u, a0, O.opts(useGaunt=True, useGauntSparse=sparse),
+                                        P.closure_opts(useGaunt=True, useGauntSparse=sparse),
+                                        P, dev, min_match=min(0.98, floor - 0.02))
+    # a near-singular cell can exhaust the ladder in the Gaunt form (oracle and device alike)
+    assert rc_o in (0, P.C3D_ERR_LADDER) and rc_g in (0, P.C3D_ERR_LADDER)
+
+
 @pytest.mark.parametrize("tail", ["bulk", "switch"])
 @pytest.mark.parametrize("N,Q,n", [(3, 24, 3000), (2, 8, 2000), (1, 6, 1000), (4, 12, 600)])
 def test_closure_batched_matches_oracle(P, O, dev, N, Q, n, tail):
