@@ -151,6 +151,8 @@ extern double g_mnIterMean, g_mnIterGammaMean;
 extern int g_mnNumDualSolves;
 extern OPTIMIZATION_STATS *g_optStats;
 extern bool g_outputOptStats;
+extern double *g_beta, *g_betaVal; /* Gaunt tables (src/opt/opt.h:56-59), read when */
+extern int *g_betaColInd, *g_betaRowPtr; /* g_mnUseGaunt is true                    */
 
 /* ---------------------------------------------------------------- C helpers (tests, FFI) */
 extern "C" {

@@ -1,0 +1,158 @@
+"""GPU check of the c3d_run program (the reference's main.cpp + init() + output.cpp over the
+host drop-in): a deck is run end to end in the reference-order mode and every output file
+is compared with the oracle driven through main.cpp's time loop on the same tables and the
+same initial state -- bit for bit for the moments, and field by field for the headers, the
+statistics tail and the timing / statistics files (main.cpp:31-112, init.cpp:344-576,
+output.cpp:19-222)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "c3d-2015_amd", "c3d_run")
+
+DECK = """SOLVER {solver}
+TFINAL {tfinal!r}
+OUTDT {outdt!r}
+OUTPUT_DATA true
+OUTPUT_OPT_STATS true
+OUTPUT_TIMINGS true
+NX {n[0]}
+NY {n[1]}
+NZ {n[2]}
+MPIX 1
+MPIY 1
+MPIZ 1
+AX -1.0
+AY -1.0
+AZ -1.0
+BX 1.0
+BY 1.0
+BZ 1.0
+MOMENT_ORDER 2
+QUAD_ORDER 8
+CFL_FACTOR 0.5
+CROSS_SECTION 1.0
+INIT_FLOOR 1e-4
+GAUSS_SIGMA 0.3
+INIT_COND {ic}
+THETA {theta}
+OMP_THREADS 4
+PN_USE_KINETIC_FLUX 0
+MN_TOL 1.0e-8
+MN_TOL_GAMMA 1.1
+MN_COND_H_MAX 1.0e10
+MN_MAX_ITER 100
+MN_BATCHED true
+MN_USE_GAUNT_COEF false
+MN_USE_GAUNT_SPARSE false
+CUDA_KINETIC true
+CUDA_BATCHED true
+CUDA_BATCH_SIZE 50
+"""
+
+
+def _library_tables(O, P, N, Q):
+    """Oracle tables holding the library's own values (the program builds its tables with
+    c3d_tables_host), updated in place so the oracle's table struct still points at them."""
+    T = O.Tables(N, Q)
+    t = P.tables_host(N, Q)
+    for k in ("w", "xi", "eta", "mu", "M", "M2", "beta"):
+        np.copyto(getattr(T, k), t[k])
+    np.copyto(T.ops, t["pnOps"])
+    return T
+
+
+def _oracle_run(O, T, solver_mn, n, d, theta, cfl, tfinal, outdt, ic):
+    """main.cpp:50-76 with the oracle's update (init.cpp's initial state and alpha)."""
+    nx, ny, nz = n
+    ick = 0 if ic == "gaussian" else 2
+    u, sT, sS = O.init_grid(nx, ny, nz, T.nm, a=(-1.0, -1.0, -1.0), d=d, ic=ick, sigma=0.3,
+                            floor=1e-4, cross=1.0)
+    alpha = O.init_alpha(u)
+    ng = u.shape[0]
+    if solver_mn:
+        max_dt = cfl * (2.0 / (2.0 + theta)) / (1.0 / d[0] + 1.0 / d[1] + 1.0 / d[2])
+    else:
+        max_dt = cfl * 0.5 * min(min(d[0], d[1]), d[2])
+    outs = {}
+
+    def snap(t):
+        outs["%.6f" % t] = u.reshape(nx + 4, ny + 4, nz + 4, -1)[2:-2, 2:-2, 2:-2].copy()
+
+    steps = 0
+    t = 0.0
+    snap(t)
+    tout = min(t + outdt, tfinal)
+    while tout <= tfinal:
+        dt = max_dt
+        while t < tout - 1e-14 * tout:
+            t += dt
+            if t > tout:
+                dt = tout - (t - dt)
+                t = tout
+            with O.exp_mode(1):
+                O.lib().orc_update(nx, ny, nz, T.nm, T.nm2, T.nq, 1 if solver_mn else 0, 0, u,
+                                   alpha, np.zeros_like(u), np.zeros_like(u),
+                                   np.zeros(ng * T.nq), sT, sS, O.C.byref(T.c), T.ops,
+                                   O.C.byref(O.opts()), d[0], d[1], d[2], theta, dt, None, 0)
+            steps += 1
+        snap(t)
+        if tout > tfinal - 1e-14 * tfinal:
+            break
+        tout = min(tout + outdt, tfinal)
+    return outs, steps
+
+
+@pytest.mark.parametrize("solver,ic,theta", [("mn", "gaussian", 2.0), ("mn", "smooth", 1.0),
+                                             ("pn", "gaussian", 2.0)])
+def test_c3d_run_matches_oracle(P, O, dev, tmp_path, solver, ic, theta):
+    assert os.path.exists(EXE), "build c3d-2015_amd/c3d_run first"
+    n = (6, 5, 7)
+    d = tuple(2.0 / k for k in n)
+    mn = solver == "mn"
+    cfl = 0.5
+    max_dt = (cfl * (2.0 / (2.0 + theta)) / sum(1.0 / x for x in d) if mn
+              else cfl * 0.5 * min(d))
+    tfinal, outdt = 3.5 * max_dt, 2.0 * max_dt  # outputs at 2 dt and 3.5 dt (a short step)
+    (tmp_path / "input.deck").write_text(DECK.format(solver=solver, tfinal=tfinal, outdt=outdt,
+                                                     n=n, ic=ic, theta=theta))
+    print(f"c3d_run {solver} {ic} theta {theta}", flush=True)
+    r = subprocess.run([EXE, "--precision", "reforder"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Init done." in r.stdout and "Time taken:" in r.stdout
+    T = _library_tables(O, P, 2, 8)
+    ref, steps = _oracle_run(O, T, mn, n, d, theta, cfl, tfinal, outdt, ic)
+    out = tmp_path / "output"
+    assert len(ref) == 3
+    for tkey, uref in ref.items():
+        master = (out / f"out_{tkey}.master").read_text().split("\n")
+        assert master[:3] == [str(k) for k in n]
+        assert [float(x) for x in master[3:9]] == [-1.0] * 3 + [1.0] * 3
+        assert master[9:13] == [str(T.nm), "1", "1", "1"] and master[13] == f"out_{tkey}_0.pn"
+        raw = (out / f"out_{tkey}_0.pn").read_bytes()
+        head = np.frombuffer(raw[:24], dtype=np.int64)
+        assert list(head) == list(n)
+        nval = n[0] * n[1] * n[2] * T.nm
+        data = np.frombuffer(raw[24:24 + 8 * nval], dtype=np.float64).reshape(uref.shape)
+        assert np.array_equal(data, uref), tkey
+        tail = raw[24 + 8 * nval:]
+        assert len(tail) == 8 * (5 + 5)  # 2 int64, 2 double, 1 int64, 5 int64 rungs
+        hist = np.frombuffer(tail[40:], dtype=np.int64)
+    ng = (n[0] + 4) * (n[1] + 4) * (n[2] + 4)
+    if mn:  # every cell solved twice per step, each once in the rung histogram
+        assert hist.sum() == 2 * ng * steps
+        assert np.frombuffer(tail[32:40], dtype=np.int64)[0] == 2 * ng * steps
+    else:
+        assert hist.sum() == 0
+    tm = (out / "times.master").read_text().split("\n")
+    assert tm[:5] == [str(steps), "1", "1", "1", "times_0.dat"]
+    assert (out / "times_0.dat").stat().st_size == 8 * 8 * steps
+    st = (out / "stats_0.dat").read_bytes()
+    assert list(np.frombuffer(st[:16], dtype=np.int64)) == [steps, n[0]]
+    assert len(st) == 16 + 2 * steps * n[0] * 40
