@@ -179,10 +179,12 @@ void closure_errors(int rc)
 {
     if (rc == C3D_ERR_U0) {
         std::printf("u[0] < 0\n");
+        std::fflush(stdout);
         std::abort();
     }
     if (rc == C3D_ERR_LADDER) {
         std::printf("Max regularization achieved.\n");
+        std::fflush(stdout);
         std::abort();
     }
     check("c3d_closure", rc);

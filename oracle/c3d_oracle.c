@@ -1225,6 +1225,7 @@ double orc_normal(uint64_t seed, uint64_t idx, uint64_t stream)
  *   icKind 1: pulse, rho = max(exp(-|x|^2/(2 sigma^2)), floor) * (1 + noise * U[-1,1]),
  *             u0 = 2 sqrt(pi) rho   (the time-stepping workloads)
  *   icKind 2: smooth (init.cpp:497-502)
+ *   icKind 3: funky (init.cpp:42-66, 503-504) on the box [a, a + n d]
  * Cell centres follow init.cpp:483-485 (ghosts at their own, unwrapped centres). */
 void orc_init_grid(int nx, int ny, int nz, int nm, double ax, double ay, double az,
                    double dx, double dy, double dz, int icKind, double sigma, double floorv,
@@ -1254,6 +1255,23 @@ void orc_init_grid(int nx, int ny, int nz, int nm, double ax, double ay, double 
                     if (noise != 0.0)
                         rho = rho * (1.0 + noise * (2.0 * orc_uniform(seed, c, 7) - 1.0));
                     u[c * nm] = 2.0 * sqrt(M_PI) * rho;
+                } else if (icKind == 3) {
+                    /* init.cpp:42-66 with the box [a, a + n d]: 125 Gaussians centred on the
+                       odd tenths of the box, plus the smooth state */
+                    double wx = nx * dx, wy = ny * dy, wz = nz * dz, sum = 0.0;
+                    double s2 = sigma * sigma;
+                    double f1 = 1.0 / (sigma * sqrt(2.0 * M_PI));
+                    double f3 = f1 * f1 * f1;
+                    for (int gi = 1; gi < 10; gi += 2)
+                        for (int gj = 1; gj < 10; gj += 2)
+                            for (int gk = 1; gk < 10; gk += 2) {
+                                double px = x - (ax + gi * wx / 10.0);
+                                double py = y - (ay + gj * wy / 10.0);
+                                double pz = z - (az + gk * wz / 10.0);
+                                sum += f3 * exp(-(px * px + py * py + pz * pz) / (2.0 * s2));
+                            }
+                    u[c * nm] = (2.0 + cos(2 * M_PI * x) * cos(2 * M_PI * y) *
+                                           cos(2 * M_PI * z) + sum) * 2.0 * sqrt(M_PI);
                 } else {
                     u[c * nm] = (2.0 + cos(2 * M_PI * x) * cos(2 * M_PI * y) *
                                            cos(2 * M_PI * z)) * 2.0 * sqrt(M_PI);

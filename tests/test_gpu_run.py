@@ -38,7 +38,7 @@ QUAD_ORDER 8
 CFL_FACTOR 0.5
 CROSS_SECTION 1.0
 INIT_FLOOR 1e-4
-GAUSS_SIGMA 0.3
+GAUSS_SIGMA {sigma!r}
 INIT_COND {ic}
 THETA {theta}
 OMP_THREADS 4
@@ -67,11 +67,11 @@ def _library_tables(O, P, N, Q):
     return T
 
 
-def _oracle_run(O, T, solver_mn, n, d, theta, cfl, tfinal, outdt, ic):
+def _oracle_run(O, T, solver_mn, n, d, theta, cfl, tfinal, outdt, ic, sigma):
     """main.cpp:50-76 with the oracle's update (init.cpp's initial state and alpha)."""
     nx, ny, nz = n
-    ick = 0 if ic == "gaussian" else 2
-    u, sT, sS = O.init_grid(nx, ny, nz, T.nm, a=(-1.0, -1.0, -1.0), d=d, ic=ick, sigma=0.3,
+    ick = {"gaussian": 0, "smooth": 2, "funky": 3}[ic]
+    u, sT, sS = O.init_grid(nx, ny, nz, T.nm, a=(-1.0, -1.0, -1.0), d=d, ic=ick, sigma=sigma,
                             floor=1e-4, cross=1.0)
     alpha = O.init_alpha(u)
     ng = u.shape[0]
@@ -108,11 +108,16 @@ def _oracle_run(O, T, solver_mn, n, d, theta, cfl, tfinal, outdt, ic):
     return outs, steps
 
 
-@pytest.mark.parametrize("solver,ic,theta", [("mn", "gaussian", 2.0), ("mn", "smooth", 1.0),
-                                             ("pn", "gaussian", 2.0)])
-def test_c3d_run_matches_oracle(P, O, dev, tmp_path, solver, ic, theta):
+# funky: a grid whose spacings are powers of two, so that n d is the deck's box width
+# exactly (the oracle places its Gaussians on [a, a + n d]), and sigma 0.5: at 0.3 a cell
+# exhausts the regularisation ladder in the first step, where the reference (and the host
+# drop-in) abort
+@pytest.mark.parametrize("solver,ic,theta,n,sigma", [("mn", "gaussian", 2.0, (6, 5, 7), 0.3),
+                                                     ("mn", "smooth", 1.0, (6, 5, 7), 0.3),
+                                                     ("pn", "gaussian", 2.0, (6, 5, 7), 0.3),
+                                                     ("mn", "funky", 2.0, (4, 8, 4), 0.5)])
+def test_c3d_run_matches_oracle(P, O, dev, tmp_path, solver, ic, theta, n, sigma):
     assert os.path.exists(EXE), "build c3d-2015_amd/c3d_run first"
-    n = (6, 5, 7)
     d = tuple(2.0 / k for k in n)
     mn = solver == "mn"
     cfl = 0.5
@@ -120,14 +125,14 @@ def test_c3d_run_matches_oracle(P, O, dev, tmp_path, solver, ic, theta):
               else cfl * 0.5 * min(d))
     tfinal, outdt = 3.5 * max_dt, 2.0 * max_dt  # outputs at 2 dt and 3.5 dt (a short step)
     (tmp_path / "input.deck").write_text(DECK.format(solver=solver, tfinal=tfinal, outdt=outdt,
-                                                     n=n, ic=ic, theta=theta))
+                                                     n=n, ic=ic, theta=theta, sigma=sigma))
     print(f"c3d_run {solver} {ic} theta {theta}", flush=True)
     r = subprocess.run([EXE, "--precision", "reforder"], cwd=tmp_path, capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "Init done." in r.stdout and "Time taken:" in r.stdout
     T = _library_tables(O, P, 2, 8)
-    ref, steps = _oracle_run(O, T, mn, n, d, theta, cfl, tfinal, outdt, ic)
+    ref, steps = _oracle_run(O, T, mn, n, d, theta, cfl, tfinal, outdt, ic, sigma)
     out = tmp_path / "output"
     assert len(ref) == 3
     for tkey, uref in ref.items():

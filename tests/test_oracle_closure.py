@@ -213,3 +213,33 @@ def test_flux_conservation_limiter_and_fromm(O):
     # and equals A_x^+ slope + A_x^- slope = A_x slope / dx
     Ax = T.ops[0] + T.ops[1]
     assert np.allclose(im, (Ax @ slope) / 0.25, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("ic", [0, 2, 3])
+def test_initial_states_closed_form(O, ic):
+    """The oracle's initial states (init.cpp:27-66, 479-506) against a direct numpy
+    evaluation at every cell centre, ghosts included: the normalised Gaussian with its floor,
+    the smooth state with u1..u3 = 0.3 u0, and the "funky" sum of 125 Gaussians on the odd
+    tenths of the box plus the smooth state."""
+    n, d, a, sig, floor = (4, 8, 4), (0.5, 0.25, 0.5), (-1.0, -1.0, -1.0), 0.3, 1e-4
+    u, sT, sS = O.init_grid(*n, 9, a=a, d=d, ic=ic, sigma=sig, floor=floor, cross=1.25)
+    assert (sT == 1.25).all() and (sS == 1.25).all()
+    U = u.reshape(n[0] + 4, n[1] + 4, n[2] + 4, -1)
+    c = [a[k] + (np.arange(n[k] + 4) - 2) * d[k] + d[k] / 2 for k in range(3)]
+    x, y, z = np.meshgrid(*c, indexing="ij")
+    g = lambda px, py, pz: (np.exp(-(px * px + py * py + pz * pz) / (2 * sig * sig))
+                            / (sig * np.sqrt(2 * np.pi)) ** 3)
+    smooth = 2 + np.cos(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.cos(2 * np.pi * z)
+    if ic == 0:
+        ref = np.maximum(floor, g(x, y, z) * 2 * np.sqrt(np.pi))
+    elif ic == 2:
+        ref = smooth * 2 * np.sqrt(np.pi)
+    else:
+        w = [n[k] * d[k] for k in range(3)]
+        s = sum(g(x - (a[0] + i * w[0] / 10), y - (a[1] + j * w[1] / 10), z - (a[2] + k * w[2] / 10))
+                for i in range(1, 10, 2) for j in range(1, 10, 2) for k in range(1, 10, 2))
+        ref = (smooth + s) * 2 * np.sqrt(np.pi)
+    assert np.abs(U[..., 0] - ref).max() <= 2e-15 * np.abs(ref).max()
+    rest = U[..., 1:4] if ic != 2 else U[..., 1:4] - 0.3 * U[..., :1]
+    assert np.abs(rest).max() <= (1e-15 * np.abs(ref).max() if ic == 2 else 0.0)
+    assert (U[..., 4:] == 0).all()
