@@ -27,6 +27,7 @@ import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -87,6 +88,9 @@ def main():
     ap.add_argument("--grid", type=int, default=128, help="step: cells per axis")
     ap.add_argument("--order", type=int, default=2, help="step: moment order N")
     ap.add_argument("--quad", type=int, default=8, help="step: quadrature order Q")
+    ap.add_argument("--orders-streams", type=int, choices=(1, 4), default=4,
+                    help="orders: 4 solves the four orders concurrently (one stream and one "
+                         "host thread per order's context), 1 back to back on one stream")
     ap.add_argument("--step-precision", choices=("fp64", "mixed"), default="fp64",
                     help="step: mixed runs the mixed-precision closure's trajectory and the "
                          "reference-order FP64 one beside the FP64 one from the same state "
@@ -447,8 +451,12 @@ def step_workload(args, P, torch, dev):
 def orders_workload(args, P, torch, dev):
     """2^20 independent cells, a quarter each of M_1, M_2, M_3, M_4 (Q = 24), alpha* as in the
     metric's batch.  Each order has its own context and warm-start network (trained on a
-    quarter of --train-pairs solver pairs of that order); one step solves the four quarters
-    back to back on one stream, mixed+nn, then the same with the plain FP64 path."""
+    quarter of --train-pairs solver pairs of that order); one step solves the four quarters,
+    mixed+nn, then the same with the plain FP64 path.  With --orders-streams 4 (default) the
+    four solves run concurrently, each on its own stream driven by its own host thread (a
+    closure call waits on its stream between passes, so one host thread would serialise
+    them), so one order's latency-bound tail overlaps the others' bulk passes; with 1 they
+    run back to back on one stream."""
     Q, n = 24, args.cells
     opts = P.closure_opts(maxIter=100, tol=1e-8, tolGamma=1.1, condHMax=1e10)
     parts = []
@@ -475,28 +483,41 @@ def orders_workload(args, P, torch, dev):
     torch.cuda.empty_cache()
     log("trained four warm starts")
 
+    concurrent = args.orders_streams > 1
+    for p in parts:
+        p["stream"] = torch.cuda.Stream() if concurrent else torch.cuda.current_stream()
+    pool = ThreadPoolExecutor(max_workers=len(parts)) if concurrent else None
+
+    def solve(p):
+        with torch.cuda.stream(p["stream"]):
+            p["u"].copy_(p["u0"])
+            p["a"].copy_(p["a0"])
+            p["ctx"].closure(p["u"], p["a"], opts, stats=p["st"], stream=p["stream"])
+
     def run(path, steps, warmup):
         for p in parts:
             p["ctx"].set_precision(P.PREC_MIXED if path == "mixed+nn" else P.PREC_FP64)
             p["ctx"].set_warmstart(path == "mixed+nn")
 
         def step():
-            for p in parts:
-                p["u"].copy_(p["u0"])
-                p["a"].copy_(p["a0"])
-                p["ctx"].closure(p["u"], p["a"], opts, stats=p["st"])
+            if pool is None:
+                for p in parts:
+                    solve(p)
+            else:
+                for f in [pool.submit(solve, p) for p in parts]:
+                    f.result()
 
         for _ in range(warmup):
             step()
         for p in parts:
-            p["ctx"].sync()
+            p["ctx"].sync(p["stream"])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             step()
         rc = 0
         for p in parts:
-            rc = rc or p["ctx"].sync()
+            rc = rc or p["ctx"].sync(p["stream"])
         torch.cuda.synchronize()
         ms = 1e3 * (time.perf_counter() - t0) / steps
         per = []
@@ -526,6 +547,8 @@ def orders_workload(args, P, torch, dev):
         "data": "synthetic",
         "config": {"workload": f"closure-only batch: {n} cells, a quarter each of M_1..M_4, "
                                "Q=24, alpha*~N(0,s^2), s~U[0.5,8]",
+                   "orders_run": "concurrently, one stream and host thread per order"
+                                 if concurrent else "back to back on one stream",
                    "status": rc_m, "fp64_ms_per_step": round(ms_f, 3), "fp64_status": rc_f,
                    "speedup_vs_fp64": round(ms_f / ms_m, 3), "orders": orders},
         "roofline": None, "cpu_baseline": None,
