@@ -42,12 +42,12 @@ GAUSS_SIGMA {sigma!r}
 INIT_COND {ic}
 THETA {theta}
 OMP_THREADS 4
-PN_USE_KINETIC_FLUX 0
+PN_USE_KINETIC_FLUX {pnk}
 MN_TOL 1.0e-8
 MN_TOL_GAMMA 1.1
 MN_COND_H_MAX 1.0e10
 MN_MAX_ITER 100
-MN_BATCHED true
+MN_BATCHED {batched}
 MN_USE_GAUNT_COEF false
 MN_USE_GAUNT_SPARSE false
 CUDA_KINETIC true
@@ -67,7 +67,7 @@ def _library_tables(O, P, N, Q):
     return T
 
 
-def _oracle_run(O, T, solver_mn, n, d, theta, cfl, tfinal, outdt, ic, sigma):
+def _oracle_run(O, T, solver_mn, n, d, theta, cfl, tfinal, outdt, ic, sigma, pnk, batched):
     """main.cpp:50-76 with the oracle's update (init.cpp's initial state and alpha)."""
     nx, ny, nz = n
     ick = {"gaussian": 0, "smooth": 2, "funky": 3}[ic]
@@ -96,10 +96,11 @@ def _oracle_run(O, T, solver_mn, n, d, theta, cfl, tfinal, outdt, ic, sigma):
                 dt = tout - (t - dt)
                 t = tout
             with O.exp_mode(1):
-                O.lib().orc_update(nx, ny, nz, T.nm, T.nm2, T.nq, 1 if solver_mn else 0, 0, u,
+                O.lib().orc_update(nx, ny, nz, T.nm, T.nm2, T.nq, 1 if solver_mn else 0, pnk, u,
                                    alpha, np.zeros_like(u), np.zeros_like(u),
                                    np.zeros(ng * T.nq), sT, sS, O.C.byref(T.c), T.ops,
-                                   O.C.byref(O.opts()), d[0], d[1], d[2], theta, dt, None, 0)
+                                   O.C.byref(O.opts(unbatched=not batched)), d[0], d[1], d[2],
+                                   theta, dt, None, 0)
             steps += 1
         snap(t)
         if tout > tfinal - 1e-14 * tfinal:
@@ -112,11 +113,15 @@ def _oracle_run(O, T, solver_mn, n, d, theta, cfl, tfinal, outdt, ic, sigma):
 # exactly (the oracle places its Gaussians on [a, a + n d]), and sigma 0.5: at 0.3 a cell
 # exhausts the regularisation ladder in the first step, where the reference (and the host
 # drop-in) abort
-@pytest.mark.parametrize("solver,ic,theta,n,sigma", [("mn", "gaussian", 2.0, (6, 5, 7), 0.3),
-                                                     ("mn", "smooth", 1.0, (6, 5, 7), 0.3),
-                                                     ("pn", "gaussian", 2.0, (6, 5, 7), 0.3),
-                                                     ("mn", "funky", 2.0, (4, 8, 4), 0.5)])
-def test_c3d_run_matches_oracle(P, O, dev, tmp_path, solver, ic, theta, n, sigma):
+# (also: the unbatched closure, MN_BATCHED false, and P_N with the kinetic flux)
+@pytest.mark.parametrize("solver,ic,theta,n,sigma,pnk,batched", [
+    ("mn", "gaussian", 2.0, (6, 5, 7), 0.3, 0, True),
+    ("mn", "smooth", 1.0, (6, 5, 7), 0.3, 0, True),
+    ("pn", "gaussian", 2.0, (6, 5, 7), 0.3, 0, True),
+    ("mn", "funky", 2.0, (4, 8, 4), 0.5, 0, True),
+    ("mn", "gaussian", 2.0, (6, 5, 7), 0.3, 0, False),
+    ("pn", "smooth", 1.0, (6, 5, 7), 0.3, 1, True)])
+def test_c3d_run_matches_oracle(P, O, dev, tmp_path, solver, ic, theta, n, sigma, pnk, batched):
     assert os.path.exists(EXE), "build c3d-2015_amd/c3d_run first"
     d = tuple(2.0 / k for k in n)
     mn = solver == "mn"
@@ -125,14 +130,16 @@ def test_c3d_run_matches_oracle(P, O, dev, tmp_path, solver, ic, theta, n, sigma
               else cfl * 0.5 * min(d))
     tfinal, outdt = 3.5 * max_dt, 2.0 * max_dt  # outputs at 2 dt and 3.5 dt (a short step)
     (tmp_path / "input.deck").write_text(DECK.format(solver=solver, tfinal=tfinal, outdt=outdt,
-                                                     n=n, ic=ic, theta=theta, sigma=sigma))
+                                                     n=n, ic=ic, theta=theta, sigma=sigma,
+                                                     pnk=pnk, batched=str(batched).lower()))
     print(f"c3d_run {solver} {ic} theta {theta}", flush=True)
     r = subprocess.run([EXE, "--precision", "reforder"], cwd=tmp_path, capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "Init done." in r.stdout and "Time taken:" in r.stdout
     T = _library_tables(O, P, 2, 8)
-    ref, steps = _oracle_run(O, T, mn, n, d, theta, cfl, tfinal, outdt, ic, sigma)
+    ref, steps = _oracle_run(O, T, mn, n, d, theta, cfl, tfinal, outdt, ic, sigma, pnk,
+                             batched)
     out = tmp_path / "output"
     assert len(ref) == 3
     for tkey, uref in ref.items():
