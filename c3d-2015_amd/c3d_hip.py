@@ -92,6 +92,7 @@ SIGNATURES = [
     ("c3d_update", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _d, _d, _d,
                              _d, _d, C.c_int, C.c_int, C.POINTER(ClosureOpts), _vp, _vp]),
     ("c3d_make_closure_batch", C.c_int, [_vp, _i64, C.c_uint64, _d, _d, _vp, _vp, _vp, _vp]),
+    ("c3d_stats_summary", C.c_int, [_vp, _i64, _vp, _vp, _vp]),
     ("c3d_init_grid", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_int, _d, _d, _d,
                                 C.c_uint64, _d, _vp, _vp, _vp, _vp, _vp]),
     ("c3d_mlp_train", C.c_int, [_vp, _i64, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float,
@@ -141,6 +142,11 @@ def _stream(s):
     if isinstance(s, int):
         return s
     return s.cuda_stream
+
+
+class StatsSum(C.Structure):
+    _fields_ = [("cells", C.c_int64), ("sumIter", C.c_int64), ("sumIterGamma", C.c_int64),
+                ("maxIter", C.c_int64), ("maxIterGamma", C.c_int64), ("rung", C.c_int64 * 5)]
 
 
 class C3DError(RuntimeError):
@@ -227,6 +233,16 @@ class Ctx:
 
     def sync(self, stream=None):
         return self.L.c3d_sync(self.h, _stream(stream))
+
+    def stats_summary(self, stats, ncells=None, stream=None):
+        """Sums of per-cell outcomes on the device (c3d_stats_summary) as a dict."""
+        n = ncells if ncells is not None else stats.numel() * stats.element_size() // STATS_BYTES
+        out = StatsSum()
+        check("c3d_stats_summary", self.L.c3d_stats_summary(self.h, n, _p(stats), C.byref(out),
+                                                             _stream(stream)))
+        d = {k: getattr(out, k) for k, _ in StatsSum._fields_ if k != "rung"}
+        d["rung"] = list(out.rung)
+        return d
 
     def ghost_fill(self, nx, ny, nz, u, stream=None):
         return check("c3d_ghost_fill", self.L.c3d_ghost_fill(self.h, nx, ny, nz, _p(u),

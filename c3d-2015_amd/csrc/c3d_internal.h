@@ -72,6 +72,7 @@ struct c3d_ctx_s {
     // scratch owned by the context, grown on demand
     c3d::DevBuf ansatz, uOld, flux;
     c3d::DevBuf closureState;  // per-cell Newton state, Hessians, worklists
+    c3d::DevBuf statsPart;     // c3d_stats_summary's per-block partial sums
     c3d::Mlp mlp;
     bool warm = false;         // closure starts from the network's proposal where it is better
     c3d::DevBuf warmBuf;       // features and proposals of the closure's cells
@@ -149,5 +150,7 @@ hipError_t launch_init_grid(int nx, int ny, int nz, int nm, const double *a, con
 hipError_t launch_make_batch(c3d_ctx_s *ctx, int64_t n, uint64_t seed, double smin,
                              double smax, double *u, double *alpha0, double *alphaStar,
                              hipStream_t s);
+hipError_t launch_stats_partials(int64_t n, const c3d_cell_stats *st, long long *part,
+                                 int blocks, hipStream_t s);
 
 }  // namespace c3d

@@ -1,8 +1,10 @@
 // capi.hip -- the extern "C" surface of libc3d_hip.so (declared in include/c3d_hip.h).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "c3d_internal.h"
 
@@ -697,6 +699,43 @@ int c3d_make_closure_batch(c3d_ctx ctx, int64_t ncells, uint64_t seed, double sm
         return C3D_ERR_ARG;
     return hip_status(launch_make_batch(ctx, ncells, seed, smin, smax, u, alpha0, alphaStar,
                                         (hipStream_t)stream));
+}
+
+int c3d_stats_summary(c3d_ctx ctx, int64_t ncells, const c3d_cell_stats *stats,
+                      c3d_stats_sum *out, void *stream)
+{
+    if (!ctx || !out || ncells < 0 || (ncells > 0 && !stats))
+        return C3D_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    if (ncells == 0)
+        return C3D_OK;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    constexpr int kFields = 10;
+    const int64_t want = (ncells + 255) / 256;
+    const int blocks = (int)std::min<int64_t>(want, (int64_t)std::max(ctx->numCU, 1) * 4);
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = grow(ctx->statsPart, (size_t)blocks * kFields * sizeof(long long));
+    std::vector<long long> part((size_t)blocks * kFields);
+    if (e == hipSuccess)
+        e = launch_stats_partials(ncells, stats, (long long *)ctx->statsPart.p, blocks, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(part.data(), ctx->statsPart.p, part.size() * sizeof(long long),
+                           hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return C3D_ERR_HIP;
+    for (int b = 0; b < blocks; ++b) {
+        const long long *v = &part[(size_t)b * kFields];
+        out->cells += v[0];
+        out->sumIter += v[1];
+        out->sumIterGamma += v[2];
+        out->maxIter = std::max<int64_t>(out->maxIter, v[3]);
+        out->maxIterGamma = std::max<int64_t>(out->maxIterGamma, v[4]);
+        for (int r = 0; r < 5; ++r)
+            out->rung[r] += v[5 + r];
+    }
+    return C3D_OK;
 }
 
 /* ------------------------------------------------------------------------------

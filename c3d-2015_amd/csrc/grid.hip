@@ -614,4 +614,55 @@ hipError_t launch_make_batch(c3d_ctx_s *ctx, int64_t n, uint64_t seed, double sm
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ statistics summary
+// Per-block partial sums of the closure's per-cell outcomes, the quantities update.cpp:145-174
+// folds: sums and maxima of totalIter and iterGamma, and the count of cells on each ladder
+// rung (r within 1e-10 of the rung's value, opt.h:37-38).  Row b of `part` holds block b's
+// [n, sumIter, sumGamma, maxIter, maxGamma, rung0..rung4]; the host adds the rows.  No
+// atomics: a grid-stride loop, then a tree in LDS.
+constexpr int kSumFields = 10;
+__global__ __launch_bounds__(kTB) void k_stats_partials(int64_t n,
+                                                        const c3d_cell_stats *__restrict__ st,
+                                                        long long *part)
+{
+    __shared__ long long sh[kSumFields][kTB];
+    long long v[kSumFields] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const double reg[5] = {0.0, 1e-3, 1e-2, 1e-1, 5e-1};
+    for (int64_t c = (int64_t)blockIdx.x * kTB + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * kTB) {
+        const c3d_cell_stats s = st[c];
+        v[0] += 1;
+        v[1] += s.totalIter;
+        v[2] += s.iterGamma;
+        v[3] = v[3] > s.totalIter ? v[3] : s.totalIter;
+        v[4] = v[4] > s.iterGamma ? v[4] : s.iterGamma;
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+            v[5 + r] += fabs(s.r - reg[r]) < 1e-10 ? 1 : 0;
+    }
+#pragma unroll
+    for (int f = 0; f < kSumFields; ++f)
+        sh[f][threadIdx.x] = v[f];
+    __syncthreads();
+    for (int w = kTB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int f = 0; f < kSumFields; ++f) {
+                const long long a = sh[f][threadIdx.x], b = sh[f][threadIdx.x + w];
+                sh[f][threadIdx.x] = (f == 3 || f == 4) ? (a > b ? a : b) : a + b;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kSumFields)
+        part[(size_t)blockIdx.x * kSumFields + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+hipError_t launch_stats_partials(int64_t n, const c3d_cell_stats *st, long long *part,
+                                 int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stats_partials, dim3(blocks), dim3(kTB), 0, s, n, st, part);
+    return hipGetLastError();
+}
+
 }  // namespace c3d

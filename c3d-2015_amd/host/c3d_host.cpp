@@ -102,6 +102,13 @@ struct HostState {
     const double *w = nullptr, *m = nullptr, *xi = nullptr, *beta = nullptr;
     int precision = C3D_PREC_FP64;
     DevArray u, alpha, flux, stats, sig;
+    // resident mode of update(dt): the grid it advances stays in these between calls
+    bool resident = false;
+    bool devValid = false;  // ru/ralpha/rsig hold the state of the host arrays below
+    bool devAhead = false;  // ... and are newer than them
+    const double *hostU = nullptr, *hostA = nullptr;
+    size_t gridBytes = 0;
+    DevArray ru, ralpha, rsig, rstats;
 };
 HostState &state()
 {
@@ -188,6 +195,26 @@ void closure_errors(int rc)
         std::abort();
     }
     check("c3d_closure", rc);
+}
+
+// The same fold over a whole stage from its device-side sums (resident mode): maxima and rung
+// counts as the per-cell fold gives them; the two running means take the stage's sums in one
+// step, (N mean + sum) / (N + n), which agrees with the per-cell recurrence up to rounding.
+void fold_summary(const c3d_stats_sum &t)
+{
+    if (t.cells == 0)
+        return;
+    if (t.maxIter > g_mnMaxIterHist)
+        g_mnMaxIterHist = (int)t.maxIter;
+    if (t.maxIterGamma > g_mnMaxGammaIter)
+        g_mnMaxGammaIter = (int)t.maxIterGamma;
+    if (g_mnHistReg)
+        for (int r = 0; r < 5; ++r)
+            g_mnHistReg[r] += (int)t.rung[r];
+    const double n0 = g_mnNumDualSolves, n1 = n0 + (double)t.cells;
+    g_mnIterMean = (n0 * g_mnIterMean + (double)t.sumIter) / n1;
+    g_mnIterGammaMean = (n0 * g_mnIterGammaMean + (double)t.sumIterGamma) / n1;
+    g_mnNumDualSolves += (int)t.cells;
 }
 
 // update.cpp:145-174: fold one cell's outcome into the run statistics
@@ -383,8 +410,24 @@ void communicateBoundaries()
     d2h(g_u, du, b);
 }
 
+// g_optStats row of one stage (update.cpp:147-156): the cells of the line j = gY[3]/2,
+// k = gZ[3]/2 across the interior in x.
+void record_line(const c3d_cell_stats &s, int i, int stage)
+{
+    const int idx = (i - g_gX[1]) + g_nx * (2 * g_numUpdates + stage);
+    g_optStats[idx].iter = s.totalIter;
+    g_optStats[idx].iterGamma = s.iterGamma;
+    g_optStats[idx].r = s.r;
+    // the batched path stores iterGamma as gamma (update.cpp:153)
+    g_optStats[idx].gamma = g_mnBatched ? s.iterGamma : s.gamma;
+    g_optStats[idx].normG = s.err;
+}
+
 // update.cpp:185-275 on the device: uOld copy, two stages of [ghost fill, closure over all
-// cells, flux, Euler step], RK2 average; statistics folded as update.cpp:145-174 does.
+// cells, flux, Euler step], RK2 average; statistics folded as update.cpp:145-174 does.  In
+// resident mode (c3d_host_set_resident) the grid stays on the device between calls: it is
+// uploaded on the first call, the host arrays are refreshed by c3d_host_sync_to_host, and
+// the statistics are folded from device-side sums plus the one g_optStats line.
 void update(double dt)
 {
     HostState &S = state();
@@ -394,45 +437,76 @@ void update(double dt)
     const size_t ng = grid_cells();
     const size_t b = ng * g_numMoments * sizeof(double);
     const bool mn = g_solver == SOLVER_MN;
-    double *du = (double *)S.u.get(b);
-    double *da = (double *)S.alpha.get(b);
-    double *dsig = (double *)S.sig.get(2 * ng * sizeof(double));
-    c3d_cell_stats *ds = (c3d_cell_stats *)S.stats.get(2 * ng * sizeof(c3d_cell_stats));
-    h2d(du, g_u, b);
-    if (mn)
-        h2d(da, g_mnAlpha, b);
-    h2d(dsig, g_sigmaT, ng * sizeof(double));
-    h2d(dsig + ng, g_sigmaS, ng * sizeof(double));
+    const bool res = S.resident;
+    if (res && (!S.devValid || S.hostU != g_u || S.hostA != (mn ? g_mnAlpha : nullptr) ||
+                S.gridBytes != b)) {
+        S.devValid = false;  // another grid (or the first call): upload it
+        S.devAhead = false;
+    }
+    double *du = (double *)(res ? S.ru : S.u).get(b);
+    double *da = (double *)(res ? S.ralpha : S.alpha).get(b);
+    double *dsig = (double *)(res ? S.rsig : S.sig).get(2 * ng * sizeof(double));
+    c3d_cell_stats *ds =
+        (c3d_cell_stats *)(res ? S.rstats : S.stats).get(2 * ng * sizeof(c3d_cell_stats));
+    if (!res || !S.devValid) {
+        h2d(du, g_u, b);
+        if (mn)
+            h2d(da, g_mnAlpha, b);
+        h2d(dsig, g_sigmaT, ng * sizeof(double));
+        h2d(dsig + ng, g_sigmaS, ng * sizeof(double));
+        if (res) {
+            S.devValid = true;
+            S.hostU = g_u;
+            S.hostA = mn ? g_mnAlpha : nullptr;
+            S.gridBytes = b;
+        }
+    }
     c3d_closure_opts o{g_mnMaxIter, g_mnUseGaunt ? 1 : 0, g_mnUseGauntSparse ? 1 : 0,
                        g_mnBatched ? 0 : 1, g_mnTol, g_mnTolGamma, g_mnCondHMax};
     check("c3d_update", c3d_update(ctx, g_nx, g_ny, g_nz, du, mn ? da : nullptr, dsig, dsig + ng,
                                    g_dx, g_dy, g_dz, g_theta, dt, mn ? 1 : 0, g_pnUseKineticFlux,
                                    &o, mn ? ds : nullptr, nullptr));
     closure_errors(c3d_sync(ctx, nullptr));
-    d2h(g_u, du, b);
-    if (mn) {
-        d2h(g_mnAlpha, da, b);
-        std::vector<c3d_cell_stats> st(2 * ng);
-        d2h(st.data(), ds, st.size() * sizeof(c3d_cell_stats));
-        for (int stage = 0; stage < 2; ++stage)
-            for (int i = g_gX[0]; i <= g_gX[3]; i++)
-                for (int j = g_gY[0]; j <= g_gY[3]; j++)
-                    for (int k = g_gZ[0]; k <= g_gZ[3]; k++) {
-                        const size_t c = (size_t)k + (g_gZ[3] - g_gZ[0] + 1) *
-                                                         ((size_t)j + (g_gY[3] - g_gY[0] + 1) * i);
-                        const c3d_cell_stats &s = st[stage * ng + c];
-                        if (g_outputOptStats && g_optStats && j == g_gY[3] / 2 &&
-                            k == g_gZ[3] / 2 && i >= g_gX[1] && i <= g_gX[2]) {
-                            const int idx = (i - g_gX[1]) + g_nx * (2 * g_numUpdates + stage);
-                            g_optStats[idx].iter = s.totalIter;
-                            g_optStats[idx].iterGamma = s.iterGamma;
-                            g_optStats[idx].r = s.r;
-                            // the batched path stores iterGamma as gamma (update.cpp:153)
-                            g_optStats[idx].gamma = g_mnBatched ? s.iterGamma : s.gamma;
-                            g_optStats[idx].normG = s.err;
+    const int NY = g_gY[3] - g_gY[0] + 1, NZ = g_gZ[3] - g_gZ[0] + 1;
+    const bool line = g_outputOptStats && g_optStats;
+    if (res) {
+        S.devAhead = true;
+        if (mn)
+            for (int stage = 0; stage < 2; ++stage) {
+                c3d_stats_sum t;
+                check("c3d_stats_summary",
+                      c3d_stats_summary(ctx, (int64_t)ng, ds + stage * ng, &t, nullptr));
+                if (line) {
+                    std::vector<c3d_cell_stats> row(g_nx);
+                    const size_t c0 = (size_t)(g_gZ[3] / 2) +
+                                      (size_t)NZ * ((size_t)(g_gY[3] / 2) + (size_t)NY * g_gX[1]);
+                    hcheck("hipMemcpy2D",
+                           hipMemcpy2D(row.data(), sizeof(c3d_cell_stats), ds + stage * ng + c0,
+                                       (size_t)NY * NZ * sizeof(c3d_cell_stats),
+                                       sizeof(c3d_cell_stats), g_nx, hipMemcpyDeviceToHost));
+                    for (int i = g_gX[1]; i <= g_gX[2]; ++i)
+                        record_line(row[i - g_gX[1]], i, stage);
+                }
+                fold_summary(t);
+            }
+    } else {
+        d2h(g_u, du, b);
+        if (mn) {
+            d2h(g_mnAlpha, da, b);
+            std::vector<c3d_cell_stats> st(2 * ng);
+            d2h(st.data(), ds, st.size() * sizeof(c3d_cell_stats));
+            for (int stage = 0; stage < 2; ++stage)
+                for (int i = g_gX[0]; i <= g_gX[3]; i++)
+                    for (int j = g_gY[0]; j <= g_gY[3]; j++)
+                        for (int k = g_gZ[0]; k <= g_gZ[3]; k++) {
+                            const size_t c = (size_t)k + NZ * ((size_t)j + (size_t)NY * i);
+                            const c3d_cell_stats &s = st[stage * ng + c];
+                            if (line && j == g_gY[3] / 2 && k == g_gZ[3] / 2 && i >= g_gX[1] &&
+                                i <= g_gX[2])
+                                record_line(s, i, stage);
+                            fold_stats(s);
                         }
-                        fold_stats(s);
-                    }
+        }
     }
     // the whole step runs on the device: its time is booked to stage 0's flux slot
     if (g_updateTimes) {
@@ -585,6 +659,37 @@ extern "C" int c3d_host_optimize_cells(int64_t n, int nm, int nm2, int nq, doubl
         ++done;
     }
     return done == n ? C3D_OK : C3D_ERR_ARG;
+}
+
+// Copy a resident grid back into the host arrays it was uploaded from; caller holds the
+// mutex.
+static void resident_to_host(HostState &S)
+{
+    if (!S.resident || !S.devValid || !S.devAhead)
+        return;
+    d2h((void *)S.hostU, S.ru.p, S.gridBytes);
+    if (S.hostA)
+        d2h((void *)S.hostA, S.ralpha.p, S.gridBytes);
+    S.devAhead = false;
+}
+
+extern "C" int c3d_host_set_resident(int on)
+{
+    HostState &S = state();
+    std::lock_guard<std::mutex> lk(S.mtx);
+    resident_to_host(S);
+    S.resident = on != 0;
+    S.devValid = false;  // the next update() uploads the host arrays
+    S.devAhead = false;
+    return 0;
+}
+
+extern "C" int c3d_host_sync_to_host(void)
+{
+    HostState &S = state();
+    std::lock_guard<std::mutex> lk(S.mtx);
+    resident_to_host(S);
+    return 0;
 }
 
 extern "C" int c3d_host_set_precision(int mode)

@@ -2,7 +2,7 @@
 // main.cpp and the output files of output.cpp, over the host-level drop-in (update(dt) of
 // libc3d_host.so, which runs each SSP-RK2 step on the device).
 //
-//   c3d_run [--precision fp64|mixed|reforder] [deck]        (deck: input.deck)
+//   c3d_run [--precision fp64|mixed|reforder] [--staged] [deck]   (deck: input.deck)
 //
 // Follows src/main.cpp:31-112 (loop, prints), src/init.cpp:344-576 with
 // setupLocalMpiInfo's single-domain branch (89-125), initPn (194-238) and initMn
@@ -17,7 +17,11 @@
 //   * the output/ directory is created when missing (the reference aborts);
 //   * TFINAL <= 0 or OUTDT <= 0 is an input error (the reference's loop never ends);
 //   * --precision selects the closure's arithmetic (default: FP64 on the matrix cores;
-//     reforder: the reference's summation order, bit-identical to the oracle).
+//     reforder: the reference's summation order, bit-identical to the oracle);
+//   * the grid stays on the device between steps (the host drop-in's resident mode) and is
+//     copied back before each output; the run statistics are then folded per stage from
+//     device-side sums, so the two means in the .pn tail can differ from a per-cell fold in
+//     the last bits.  --staged copies the grid through host memory every step instead.
 #include <sys/stat.h>
 
 #include <chrono>
@@ -294,6 +298,7 @@ void output_data(const Run &R, double time)
 {
     if (!R.outputData)
         return;
+    c3d_host_sync_to_host();
     char name[256];
     std::snprintf(name, sizeof(name), "out_%.6f.master", time);
     FILE *f = open_out(name, "w");
@@ -357,6 +362,7 @@ int main(int argc, char **argv)
 {
     const char *deckPath = "input.deck";
     int precision = C3D_PREC_FP64;
+    bool staged = false;
     for (int a = 1; a < argc; ++a) {
         if (std::strcmp(argv[a], "--precision") == 0 && a + 1 < argc) {
             const char *p = argv[++a];
@@ -368,8 +374,10 @@ int main(int argc, char **argv)
                 precision = C3D_PREC_FP64_REFORDER;
             else
                 fail("unknown precision", p);
+        } else if (std::strcmp(argv[a], "--staged") == 0) {
+            staged = true;
         } else if (argv[a][0] == '-') {
-            std::printf("usage: c3d_run [--precision fp64|mixed|reforder] [deck]\n");
+            std::printf("usage: c3d_run [--precision fp64|mixed|reforder] [--staged] [deck]\n");
             return 2;
         } else {
             deckPath = argv[a];
@@ -388,6 +396,7 @@ int main(int argc, char **argv)
     if (stat("output", &sb) != 0 && mkdir("output", 0755) != 0)
         fail("could not create", "output/");
     c3d_host_set_precision(precision);
+    c3d_host_set_resident(staged ? 0 : 1);
     std::printf("Init done.\n");
 
     // main.cpp:50-76
@@ -427,5 +436,6 @@ int main(int argc, char **argv)
     }
     output_timings(R);
     output_opt_stats();
+    c3d_host_set_resident(0);
     return 0;
 }

@@ -95,6 +95,15 @@ typedef struct {
     int32_t flags;      /* C3D_FLAG_* */
 } c3d_cell_stats;
 
+/* Sums over a range of c3d_cell_stats (c3d_stats_summary): what update.cpp:145-174 folds
+ * into its run statistics. */
+typedef struct {
+    int64_t cells;
+    int64_t sumIter, sumIterGamma; /* totalIter, iterGamma summed */
+    int64_t maxIter, maxIterGamma; /* their maxima (0 for no cells) */
+    int64_t rung[5];               /* cells with r within 1e-10 of 0, 1e-3, 1e-2, 1e-1, 0.5 */
+} c3d_stats_sum;
+
 /* ======================================================================
  * 1. Drop-in replacements for src/cuda/cuda_headers.h (host pointers, synchronous)
  * ====================================================================== */
@@ -346,6 +355,13 @@ int c3d_init_grid(c3d_ctx ctx, int nx, int ny, int nz, const double *a, const do
                   double sigma, double floor_value, double noise, uint64_t seed,
                   double cross_section, double *u, double *alpha, double *sigmaT,
                   double *sigmaS, void *stream);
+
+/* Sums of ncells per-cell outcomes (device, as c3d_closure/c3d_update write them) into
+ * *out (host); blocks on `stream`.  No counterpart in the reference: it replaces the
+ * per-cell host fold of update.cpp:145-174 when the grid stays on the device (the host
+ * drop-in's resident mode), so the statistics need no per-cell copy. */
+int c3d_stats_summary(c3d_ctx ctx, int64_t ncells, const c3d_cell_stats *stats,
+                      c3d_stats_sum *out, void *stream);
 
 /* Synthetic closure batch on the device: per cell s ~ U[smin,smax], alpha*_k ~ N(0,s^2),
  * u = sum_q w_q m_q exp(alpha*.m_q), alpha0 from init.cpp:265.  SplitMix64 counter RNG

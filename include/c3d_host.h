@@ -161,6 +161,16 @@ extern "C" {
 int c3d_host_deck_value(const char *path, const char *key, char *value);
 /* The precision mode (C3D_PREC_*) of the contexts behind the host entry points. */
 int c3d_host_set_precision(int mode);
+/* Resident mode of update(dt) (on = 1): the grid (g_u, g_mnAlpha, g_sigmaT/S) is uploaded
+ * on the next call and stays on the device across calls; g_u and g_mnAlpha are refreshed
+ * only by c3d_host_sync_to_host() (call it before reading them, e.g. before output), and
+ * host-side changes to them are not seen until c3d_host_set_resident is called again.  The
+ * run statistics are folded from device-side sums (c3d_stats_summary): counts and maxima as
+ * in the per-cell fold, the two means up to rounding.  on = 0 syncs and returns to staging
+ * the grid through host memory on every call (the default).  No counterpart in the
+ * reference, whose update() works on host arrays. */
+int c3d_host_set_resident(int on);
+int c3d_host_sync_to_host(void);
 /* The batched closure as update.cpp:105-178 drives it, for n cells (host arrays [n][nm],
  * tables from the globals g_w, g_m, ...): one Optimization per cell on c_stack1,
  * Optimization::opt, results collected from c_stack3 into u, alpha and (if not NULL)

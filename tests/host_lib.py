@@ -24,7 +24,8 @@ CXX_SYMBOLS = {
     "InputDeckReader::getValue(bool*)": "_ZN15InputDeckReader8getValueEPKcPb",
     "InputDeckReader::print": "_ZN15InputDeckReader5printEv",
 }
-C_SYMBOLS = ("c3d_host_deck_value", "c3d_host_set_precision", "c3d_host_optimize_cells")
+C_SYMBOLS = ("c3d_host_deck_value", "c3d_host_set_precision", "c3d_host_optimize_cells",
+             "c3d_host_set_resident", "c3d_host_sync_to_host")
 
 
 class OPTIONS(C.Structure):
@@ -57,6 +58,10 @@ def load():
         L.c3d_host_optimize_cells.argtypes = [C.c_int64, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                               C.c_void_p, C.c_void_p, C.c_void_p]
         L.c3d_host_optimize_cells.restype = C.c_int
+        L.c3d_host_set_resident.argtypes = [C.c_int]
+        L.c3d_host_set_resident.restype = C.c_int
+        L.c3d_host_sync_to_host.argtypes = []
+        L.c3d_host_sync_to_host.restype = C.c_int
         fn = getattr(L, CXX_SYMBOLS["opt"])
         fn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, OPTIONS, C.c_void_p,
                        C.c_void_p, C.c_void_p, C.POINTER(OUTS)]

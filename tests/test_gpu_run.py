@@ -113,15 +113,18 @@ def _oracle_run(O, T, solver_mn, n, d, theta, cfl, tfinal, outdt, ic, sigma, pnk
 # exactly (the oracle places its Gaussians on [a, a + n d]), and sigma 0.5: at 0.3 a cell
 # exhausts the regularisation ladder in the first step, where the reference (and the host
 # drop-in) abort
-# (also: the unbatched closure, MN_BATCHED false, and P_N with the kinetic flux)
-@pytest.mark.parametrize("solver,ic,theta,n,sigma,pnk,batched", [
-    ("mn", "gaussian", 2.0, (6, 5, 7), 0.3, 0, True),
-    ("mn", "smooth", 1.0, (6, 5, 7), 0.3, 0, True),
-    ("pn", "gaussian", 2.0, (6, 5, 7), 0.3, 0, True),
-    ("mn", "funky", 2.0, (4, 8, 4), 0.5, 0, True),
-    ("mn", "gaussian", 2.0, (6, 5, 7), 0.3, 0, False),
-    ("pn", "smooth", 1.0, (6, 5, 7), 0.3, 1, True)])
-def test_c3d_run_matches_oracle(P, O, dev, tmp_path, solver, ic, theta, n, sigma, pnk, batched):
+# (also: the unbatched closure, MN_BATCHED false, P_N with the kinetic flux, and the grid
+# staged through host memory every step instead of kept on the device)
+@pytest.mark.parametrize("solver,ic,theta,n,sigma,pnk,batched,staged", [
+    ("mn", "gaussian", 2.0, (6, 5, 7), 0.3, 0, True, False),
+    ("mn", "smooth", 1.0, (6, 5, 7), 0.3, 0, True, False),
+    ("pn", "gaussian", 2.0, (6, 5, 7), 0.3, 0, True, False),
+    ("mn", "funky", 2.0, (4, 8, 4), 0.5, 0, True, False),
+    ("mn", "gaussian", 2.0, (6, 5, 7), 0.3, 0, False, False),
+    ("pn", "smooth", 1.0, (6, 5, 7), 0.3, 1, True, False),
+    ("mn", "gaussian", 2.0, (6, 5, 7), 0.3, 0, True, True)])
+def test_c3d_run_matches_oracle(P, O, dev, tmp_path, solver, ic, theta, n, sigma, pnk, batched,
+                                staged):
     assert os.path.exists(EXE), "build c3d-2015_amd/c3d_run first"
     d = tuple(2.0 / k for k in n)
     mn = solver == "mn"
@@ -133,8 +136,8 @@ def test_c3d_run_matches_oracle(P, O, dev, tmp_path, solver, ic, theta, n, sigma
                                                      n=n, ic=ic, theta=theta, sigma=sigma,
                                                      pnk=pnk, batched=str(batched).lower()))
     print(f"c3d_run {solver} {ic} theta {theta}", flush=True)
-    r = subprocess.run([EXE, "--precision", "reforder"], cwd=tmp_path, capture_output=True,
-                       text=True, timeout=120)
+    cmd = [EXE, "--precision", "reforder"] + (["--staged"] if staged else [])
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "Init done." in r.stdout and "Time taken:" in r.stdout
     T = _library_tables(O, P, 2, 8)
