@@ -273,15 +273,19 @@ struct LaneVec {
 
 // Wave-resident factor: lane i < NM holds row i (L after Cholesky; L(i,i) in row[i]) and
 // the wave's LDS copy of the factor (sT[i ld + j] = L(i,j), j <= i) serves the transposed
-// solves.  EXACT divides by the diagonal as
-// LAPACK/BLAS do; otherwise the triangular solves multiply by reciprocals of the diagonal
-// computed once per factor (dinv, lane i: 1/L(i,i)), which shortens their critical path.
+// solves.  EXACT divides by the diagonal as LAPACK/BLAS do, one row at a time.  Otherwise
+// the triangular solves run by 4 x 4 blocks: the inverse T of each diagonal block is formed
+// once per factor (lane i keeps row i%4 and column i%4 of its block's T), and a block step
+// is one small product with T plus the update of the later rows, so a solve's dependent
+// chain is NM/4 block steps instead of NM scalar ones (the same solution, other rounding).
 template <int NM, bool EXACT = true, int G = 64>
 struct Dense {
+    static constexpr int NB = (NM + 3) / 4;  // 4 x 4 blocks of the blocked solves
     double row[NM];
     const double *sT = nullptr;  // LDS copy of L, row-major with leading dimension ld
     int ld = 0;
     double dinv;
+    double tr[4], tc[4];  // row and column lane%4 of the lane's diagonal-block inverse
 
     // L(j, lane) for j > lane (the column of L below the diagonal, read from LDS)
     __device__ __forceinline__ double col(int j, int lane) const
@@ -338,10 +342,122 @@ struct Dense {
                     sTw[lane * ldw + j] = row[j];
         }
         wave_lds_sync();
+        if constexpr (!EXACT)
+            block_inverse(lane);
+    }
+
+    // T = inverse of the lane's 4 x 4 diagonal block of L (identity rows beyond NM), by
+    // forward substitution of L_bb T = I; the diagonal reciprocals are dinv's.
+    __device__ __forceinline__ void block_inverse(int lane)
+    {
+        const int b0 = (lane >> 2) * 4, r = lane & 3;
+        double L[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j <= i; ++j)
+                L[i][j] = (b0 + i < NM) ? sT[(b0 + i) * ld + b0 + j] : (i == j ? 1.0 : 0.0);
+        double T[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            T[i][i] = 1.0 / L[i][i];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int i = c + 1; i < 4; ++i) {
+                double s = 0.0;
+#pragma unroll
+                for (int k = c; k < i; ++k)
+                    s = s + L[i][k] * T[k][c];
+                T[i][c] = -s * T[i][i];
+            }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double vr = 0.0, vc = 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (i == r) {
+                    vr = (c <= i) ? T[i][c] : 0.0;
+                    vc = (c >= i) ? T[c][i] : 0.0;
+                }
+            }
+            tr[c] = vr;
+            tc[c] = vc;
+        }
+    }
+
+    // x <- L^-1 x by blocks (non-EXACT)
+    __device__ __forceinline__ void trsv_ln_blocked(double &x, int lane)
+    {
+        const int myb = lane >> 2, r = lane & 3;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            double xb[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                xb[c] = (4 * b + c < NM) ? Grp<G>::bc(x, 4 * b + c) : 0.0;
+            if (myb == b && lane < NM) {
+                double y = 0.0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (c <= r)
+                        y = y + tr[c] * xb[c];
+                x = y;
+            }
+            if (b + 1 < NB) {
+                double yb[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    yb[c] = (4 * b + c < NM) ? Grp<G>::bc(x, 4 * b + c) : 0.0;
+                if (lane >= 4 * b + 4 && lane < NM) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        if (4 * b + c < NM)
+                            x = x - row[4 * b + c] * yb[c];
+                }
+            }
+        }
+    }
+
+    // x <- L^-T x by blocks (non-EXACT)
+    __device__ __forceinline__ void trsv_lt_blocked(double &x, int lane)
+    {
+        const int myb = lane >> 2, r = lane & 3;
+#pragma unroll
+        for (int b = NB - 1; b >= 0; --b) {
+            double xb[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                xb[c] = (4 * b + c < NM) ? Grp<G>::bc(x, 4 * b + c) : 0.0;
+            if (myb == b && lane < NM) {
+                double y = 0.0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (c >= r)
+                        y = y + tc[c] * xb[c];
+                x = y;
+            }
+            if (b > 0) {
+                double yb[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    yb[c] = (4 * b + c < NM) ? Grp<G>::bc(x, 4 * b + c) : 0.0;
+                if (lane < 4 * b) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        if (4 * b + c < NM)
+                            x = x - col(4 * b + c, lane) * yb[c];
+                }
+            }
+        }
     }
 
     __device__ __forceinline__ void trsv_ln(double &x, int lane)  // dtrsv('L','N','N')
     {
+        if constexpr (!EXACT) {
+            trsv_ln_blocked(x, lane);
+            return;
+        }
 #pragma unroll
         for (int j = 0; j < NM; ++j) {
             double xj = Grp<G>::bc(x, j);
@@ -359,6 +475,10 @@ struct Dense {
     }
     __device__ __forceinline__ void trsv_lt(double &x, int lane)  // dtrsv('L','T','N')
     {
+        if constexpr (!EXACT) {
+            trsv_lt_blocked(x, lane);
+            return;
+        }
 #pragma unroll
         for (int j = NM - 1; j >= 0; --j) {
             if (lane == j)
@@ -370,6 +490,11 @@ struct Dense {
     }
     __device__ __forceinline__ void potrs_custom(double &d, int lane)  // utils.cpp:71-86
     {
+        if constexpr (!EXACT) {
+            trsv_ln_blocked(d, lane);
+            trsv_lt_blocked(d, lane);
+            return;
+        }
 #pragma unroll
         for (int j = 0; j < NM; ++j) {
             if (lane == j)
