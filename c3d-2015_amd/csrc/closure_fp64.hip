@@ -225,10 +225,54 @@ struct Grp<16> {
     }
 };
 
-template <int NM, int G = 64>
+// Pairwise (tree) reductions of NM values: depth ceil(log2 NM) instead of NM.
+template <int NM>
+__device__ __forceinline__ double tree_sum(double (&t)[NM])
+{
+#pragma unroll
+    for (int w = 1; w < NM; w *= 2)
+#pragma unroll
+        for (int i = 0; i + w < NM; i += 2 * w)
+            t[i] = t[i] + t[i + w];
+    return t[0];
+}
+// the same over the first n entries (n fixed after unrolling)
+template <int NM>
+__device__ __forceinline__ double tree_sum_n(double (&t)[NM], int n)
+{
+#pragma unroll
+    for (int w = 1; w < NM; w *= 2)
+#pragma unroll
+        for (int i = 0; i + w < NM; i += 2 * w)
+            if (i + w < n)
+                t[i] = t[i] + t[i + w];
+    return n > 0 ? t[0] : 0.0;
+}
+template <int NM>
+__device__ __forceinline__ double tree_max(double (&t)[NM])
+{
+#pragma unroll
+    for (int w = 1; w < NM; w *= 2)
+#pragma unroll
+        for (int i = 0; i + w < NM; i += 2 * w)
+            t[i] = t[i + w] > t[i] ? t[i + w] : t[i];
+    return t[0];
+}
+
+// BLAS level-1 reductions over a lane vector (lane i < NM holds element i).  EXACT keeps the
+// reference BLAS's sequential order; otherwise the NM broadcasts are independent and the
+// reduction is a tree (other rounding; idamax keeps the first index of the maximum).
+template <int NM, int G = 64, bool EXACT = true>
 struct LaneVec {
     static __device__ __forceinline__ double dasum(double x)
     {
+        if constexpr (!EXACT) {
+            double t[NM];
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                t[i] = fabs(Grp<G>::bc(x, i));
+            return tree_sum(t);
+        }
         double s = 0.0;
 #pragma unroll
         for (int i = 0; i < NM; ++i)
@@ -237,6 +281,24 @@ struct LaneVec {
     }
     static __device__ __forceinline__ int idamax(double x)
     {
+        if constexpr (!EXACT) {
+            double t[NM];
+            int k[NM];
+#pragma unroll
+            for (int i = 0; i < NM; ++i) {
+                t[i] = fabs(Grp<G>::bc(x, i));
+                k[i] = i;
+            }
+#pragma unroll
+            for (int w = 1; w < NM; w *= 2)
+#pragma unroll
+                for (int i = 0; i + w < NM; i += 2 * w)
+                    if (t[i + w] > t[i]) {
+                        t[i] = t[i + w];
+                        k[i] = k[i + w];
+                    }
+            return k[0];
+        }
         int im = 0;
         double m = fabs(Grp<G>::bc(x, 0));
 #pragma unroll
@@ -251,6 +313,24 @@ struct LaneVec {
     }
     static __device__ __forceinline__ double dnrm2(double x)  // reference BLAS scaling
     {
+        if constexpr (!EXACT) {  // max-scaled sum of squares, one reciprocal
+            double v[NM], t[NM];
+#pragma unroll
+            for (int i = 0; i < NM; ++i) {
+                v[i] = fabs(Grp<G>::bc(x, i));
+                t[i] = v[i];
+            }
+            const double scale = tree_max(t);
+            if (!(scale > 0.0) || !isfinite(scale))
+                return scale;  // 0, inf or NaN
+            const double rs = 1.0 / scale;
+#pragma unroll
+            for (int i = 0; i < NM; ++i) {
+                const double r = v[i] * rs;
+                t[i] = r * r;
+            }
+            return scale * sqrt(tree_sum(t));
+        }
         double scale = 0.0, ssq = 1.0;
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
@@ -298,6 +378,8 @@ struct Dense {
     // dpotf2 'L': ajj = a(j,j) - ddot; column by dgemv then dscal(1/ajj)
     __device__ __forceinline__ int cholesky(int lane)
     {
+        if constexpr (!EXACT)
+            return cholesky_short(lane);
 #pragma unroll
         for (int j = 0; j < NM; ++j) {
             double s = 0.0;
@@ -319,6 +401,42 @@ struct Dense {
                 row[j] = ajj;
             else if (lane > j)
                 row[j] = rr * y;
+        }
+        return 0;
+    }
+
+    // The same factor with a shorter dependent chain per column: the dot products as trees,
+    // and sqrt(ajj), 1/sqrt(ajj) from the hardware reciprocal square root refined by two
+    // Newton steps (rsq' = rsq (3 - a rsq^2) / 2) instead of a correctly rounded sqrt and a
+    // division (the factor differs from dpotf2's in the last bits).
+    __device__ __forceinline__ int cholesky_short(int lane)
+    {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+            double t[NM];
+#pragma unroll
+            for (int k = 0; k < j; ++k)
+                t[k] = row[k] * row[k];
+            const double s = tree_sum_n(t, j);
+            const double ajj = Grp<G>::bc(row[j] - s, j);
+            if (!(ajj > 0.0))
+                return j + 1;
+#pragma unroll
+            for (int k = 0; k < j; ++k)
+                t[k] = Grp<G>::bc(row[k], j) * row[k];
+            const double y = row[j] - tree_sum_n(t, j);
+            double q;
+            if (ajj > 1e-280 && ajj < 1e280) {  // far from the exponent range's ends
+                q = __builtin_amdgcn_rsq(ajj);
+                q = q * (1.5 - (0.5 * ajj) * (q * q));
+                q = q * (1.5 - (0.5 * ajj) * (q * q));
+            } else {
+                q = 1.0 / sqrt(ajj);
+            }
+            if (lane == j)
+                row[j] = ajj * q;
+            else if (lane > j)
+                row[j] = q * y;
         }
         return 0;
     }
@@ -573,7 +691,7 @@ struct Dense {
             trsv_lt(x, lane);
             bool l120 = false;
             if (isave1 == 1) {
-                est = LaneVec<NM, G>::dasum(x);
+                est = LaneVec<NM, G, EXACT>::dasum(x);
                 if (own) {
                     x = (x >= 0.0) ? 1.0 : -1.0;
                     isgn = (x >= 0.0) ? 1 : -1;
@@ -581,14 +699,14 @@ struct Dense {
                 isave1 = 2;
                 continue;
             } else if (isave1 == 2) {
-                isave2 = LaneVec<NM, G>::idamax(x);
+                isave2 = LaneVec<NM, G, EXACT>::idamax(x);
                 isave3 = 2;
                 x = (lane == isave2) ? 1.0 : 0.0;
                 isave1 = 3;
                 continue;
             } else if (isave1 == 3) {
                 double estold = est;
-                est = LaneVec<NM, G>::dasum(x);
+                est = LaneVec<NM, G, EXACT>::dasum(x);
                 bool same = !own || (((x >= 0.0) ? 1 : -1) == isgn);
                 if (Grp<G>::all(same) || est <= estold) {
                     l120 = true;
@@ -602,7 +720,7 @@ struct Dense {
                 }
             } else if (isave1 == 4) {
                 int jlast = isave2;
-                isave2 = LaneVec<NM, G>::idamax(x);
+                isave2 = LaneVec<NM, G, EXACT>::idamax(x);
                 if (Grp<G>::bc_dyn(x, jlast) != fabs(Grp<G>::bc_dyn(x, isave2)) && isave3 < 5) {
                     ++isave3;
                     x = (lane == isave2) ? 1.0 : 0.0;
@@ -611,7 +729,7 @@ struct Dense {
                 }
                 l120 = true;
             } else {
-                double temp = 2.0 * (LaneVec<NM, G>::dasum(x) / (double)(3 * NM));
+                double temp = 2.0 * (LaneVec<NM, G, EXACT>::dasum(x) / (double)(3 * NM));
                 if (temp > est)
                     est = temp;
                 break;
@@ -1388,16 +1506,28 @@ struct Cell {
 #pragma unroll
         for (int i = 0; i < NM; ++i)
             f = f - Grp<G>::bc(au, i);
-        double rs = 0.0;  // utils_norm1
+        double norm1 = 0.0;  // utils_norm1: the largest absolute row sum
+        if constexpr (EXACT) {
+            double rs = 0.0;
 #pragma unroll
-        for (int j = 0; j < NM; ++j)
-            rs = rs + fabs(D.row[j]);
-        double norm1 = 0.0;
+            for (int j = 0; j < NM; ++j)
+                rs = rs + fabs(D.row[j]);
 #pragma unroll
-        for (int i = 0; i < NM; ++i) {
-            double vv = Grp<G>::bc(rs, i);
-            if (vv > norm1)
-                norm1 = vv;
+            for (int i = 0; i < NM; ++i) {
+                double vv = Grp<G>::bc(rs, i);
+                if (vv > norm1)
+                    norm1 = vv;
+            }
+        } else {
+            double t[NM];
+#pragma unroll
+            for (int j = 0; j < NM; ++j)
+                t[j] = fabs(D.row[j]);
+            const double rs = tree_sum(t);
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                t[i] = Grp<G>::bc(rs, i);
+            norm1 = fmax(tree_max(t), 0.0);
         }
         const int cholErr = D.cholesky(lane);
         double condest = 0.0;
@@ -1452,7 +1582,7 @@ struct Cell {
         else
             D.potrs_custom(d, lane);
         d = own ? -d : 0.0;
-        err = LaneVec<NM, G>::dnrm2(g);
+        err = LaneVec<NM, G, EXACT>::dnrm2(g);
         double da1 = 0.0;
 #pragma unroll
         for (int i = 0; i < NM; ++i)
