@@ -1429,20 +1429,30 @@ struct Cell {
     }
 
     // Armijo test of a line-search trial at step t and point at (this lane's component),
-    // f(alpha) = fdir - alpha.u (linesearch.cpp:60-66)
+    // f(alpha) = fdir - alpha.u (linesearch.cpp:60-66); alpha.u in order (EXACT) or as a tree
+    template <bool EXACT>
     __device__ __forceinline__ bool accepts_at(double fdir, double at, double t) const
     {
         double fNew = fdir;
         const double au = at * uu;  // each lane its own product
+        if constexpr (EXACT) {
 #pragma unroll
-        for (int i = 0; i < NM; ++i)
-            fNew = fNew - Grp<G>::bc(au, i);
+            for (int i = 0; i < NM; ++i)
+                fNew = fNew - Grp<G>::bc(au, i);
+        } else {
+            double v[NM];
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                v[i] = Grp<G>::bc(au, i);
+            fNew = fNew - tree_sum(v);
+        }
         return fNew <= f + 1e-3 * t * gd;
     }
     // ... of the current trial
+    template <bool EXACT>
     __device__ __forceinline__ bool ls_accepts(double fdir) const
     {
-        return accepts_at(fdir, a, tls);
+        return accepts_at<EXACT>(fdir, a, tls);
     }
 
     // The next trial of the backtracking search at step tls, or its end (linesearch.cpp:46-54):
@@ -1479,7 +1489,7 @@ struct Cell {
         const bool own = lane < NM;
         const double tolGamma = P.tolGamma;  // opt.cpp:149 / optBatch.cpp:268
         if (phase == kPhaseLS) {
-            if (ls_accepts(fdir)) {
+            if (ls_accepts<EXACT>(fdir)) {
                 tprev = tls;  // accepted; H of this sweep belongs to the new alpha
             } else {
                 ls_reject(lane);
@@ -1503,9 +1513,17 @@ struct Cell {
         else
             f = Grp<G>::bc(D.row[0], 0) / (P.m0 * P.m0);
         const double au = a * uu;
+        if constexpr (EXACT) {
 #pragma unroll
-        for (int i = 0; i < NM; ++i)
-            f = f - Grp<G>::bc(au, i);
+            for (int i = 0; i < NM; ++i)
+                f = f - Grp<G>::bc(au, i);
+        } else {
+            double t[NM];
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                t[i] = Grp<G>::bc(au, i);
+            f = f - tree_sum(t);
+        }
         double norm1 = 0.0;  // utils_norm1: the largest absolute row sum
         if constexpr (EXACT) {
             double rs = 0.0;
@@ -1583,11 +1601,8 @@ struct Cell {
             D.potrs_custom(d, lane);
         d = own ? -d : 0.0;
         err = LaneVec<NM, G, EXACT>::dnrm2(g);
-        double da1 = 0.0;
-#pragma unroll
-        for (int i = 0; i < NM; ++i)
-            da1 = da1 + fabs(Grp<G>::bc(d, i));
-        gamma = det_exp(2.0 * da1 * P.minf);
+        const double da1 = LaneVec<NM, G, EXACT>::dasum(d);
+        gamma = EXACT ? det_exp(2.0 * da1 * P.minf) : exp(2.0 * da1 * P.minf);
         if (mx) {
             // the stopping test belongs to the FP64 path: hand the cell over once its
             // FP32-accurate gradient is small, after taking this Newton step
@@ -1602,17 +1617,29 @@ struct Cell {
         aOld = a;  // linesearch.cpp:29-54
         maxF = 0.0;
         const double ratio = fabs(d) / (fabs(aOld) + 1e-100);  // each lane its own component
-#pragma unroll
-        for (int i = 0; i < NM; ++i) {
-            const double temp = Grp<G>::bc(ratio, i);
-            if (temp > maxF)
-                maxF = temp;
-        }
-        gd = 0.0;
         const double gdl = g * d;
+        if constexpr (EXACT) {
 #pragma unroll
-        for (int i = 0; i < NM; ++i)
-            gd = gd + Grp<G>::bc(gdl, i);
+            for (int i = 0; i < NM; ++i) {
+                const double temp = Grp<G>::bc(ratio, i);
+                if (temp > maxF)
+                    maxF = temp;
+            }
+            gd = 0.0;
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                gd = gd + Grp<G>::bc(gdl, i);
+        } else {
+            double t[NM];
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                t[i] = Grp<G>::bc(ratio, i);
+            maxF = fmax(tree_max(t), 0.0);
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                t[i] = Grp<G>::bc(gdl, i);
+            gd = tree_sum(t);
+        }
         tls = 1.0;
         ls_next(lane);
         return 0;
@@ -1839,7 +1866,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
 #pragma unroll
             for (int j = 0; j < KW; ++j) {
                 if (j < kw) {
-                    const bool ok = C.accepts_at(ft[j], aj[j], tj[j]);
+                    const bool ok = C.template accepts_at<REF>(ft[j], aj[j], tj[j]);
                     if (lane == 0)
                         sOk[wv * kw + j] = ok;
                 }
