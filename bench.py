@@ -77,10 +77,10 @@ def main():
     ap.add_argument("--mixed-cond", type=float, default=1e8)
     ap.add_argument("--mixed-iter", type=int, default=40)
     ap.add_argument("--train-pairs", type=int, default=1 << 22)
-    ap.add_argument("--train-steps", type=int, default=2000, help="MSE steps")
-    ap.add_argument("--dual-steps", type=int, default=10000, help="dual-gap steps")
-    ap.add_argument("--train-batch", type=int, default=8192)
-    ap.add_argument("--train-lr", type=float, default=3e-3)
+    ap.add_argument("--train-steps", type=int, default=4000, help="MSE steps")
+    ap.add_argument("--dual-steps", type=int, default=40000, help="dual-gap steps")
+    ap.add_argument("--train-batch", type=int, default=16384)
+    ap.add_argument("--train-lr", type=float, default=2e-3)
     ap.add_argument("--workload", choices=("closure", "step", "orders"), default="closure",
                     help="closure: the metric's batch (default); step: SSP-RK2 time "
                          "stepping (M_2, Q=8, 128^3 by default; kinetic flux roofline); "
