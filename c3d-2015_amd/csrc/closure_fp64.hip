@@ -11,7 +11,7 @@
 //   k_sweep  evaluates, for every listed cell at its current alpha, e_q = exp(alpha.m_q),
 //            sum_q w_q e_q and H = M^T diag(w e) M.  A wavefront takes 4 cells; the block's
 //            16 cells share every M chunk staged in LDS, each lane computes one ordinate's
-//            exponent for the 4 cells in the reference's sequential order, and the four
+//            exponent for the 4 cells (fused multiply-adds in moment order), and the four
 //            Hessians accumulate on the matrix cores (V_MFMA_F64_16X16X4_F64, 4 ordinates
 //            per instruction, A = (w e m_q)^T, B = m_q; nm = 25 uses the three 16x16 tiles
 //            of the lower block triangle).  Operand lanes: A[i][k], B[k][j] sit in lane
@@ -897,10 +897,14 @@ struct ChunkStager {
     }
 };
 
+// The chunk ordinate whose exponent and weight lane l computes in the FP64 sweep.
+__device__ __forceinline__ int mfma_ordinate(int lane) { return lane; }
+
 // Matrix-core accumulation of one LDS chunk of kChunk ordinates for CW cells: lane l owns
-// ordinate l of the chunk for the exponent, then the MFMA loop consumes 4 ordinates per
-// instruction.  The operation sequence per cell does not depend on CW or on the other cells
-// of the wave, so a cell's H is the same bits whichever kernel evaluates it.
+// ordinate mfma_ordinate(l) of the chunk for the exponent (wl is that ordinate's weight),
+// then the MFMA loop consumes 4 ordinates per instruction.  The operation sequence per cell
+// does not depend on CW or on the other cells of the wave, so a cell's H is the same bits
+// whichever kernel evaluates it.
 template <int NM, int CW>
 __device__ __forceinline__ void chunk_mfma(const double *sMc, double wl, const double *sAc,
                                            d4 (&acc)[CW][(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
@@ -909,14 +913,14 @@ __device__ __forceinline__ void chunk_mfma(const double *sMc, double wl, const d
     constexpr int NT = (NM + 15) / 16;
     constexpr int NMS = NM | 1;
     const int gq = lane >> 4, rq = lane & 15;
-    const double *rowp = &sMc[lane * NMS];
+    const double *rowp = &sMc[mfma_ordinate(lane) * NMS];
     double we[CW];
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
         double x = 0.0;
 #pragma unroll
         for (int i = 0; i < NM; ++i)
-            x = x + sAc[c * NM + i] * rowp[i];
+            x = fma(sAc[c * NM + i], rowp[i], x);
         we[c] = wl * exp(x);
         fpart[c] = fpart[c] + we[c];
     }
@@ -1074,7 +1078,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(KParams P, const double *__res
             if (ch + 1 < nch)
                 st.fetch(Mg, wg, ch + 1, tid);
             if (any)
-                chunk_mfma<NM, CW>(sM[b], sW[b][lane], sA[wv], acc, fpart, lane);
+                chunk_mfma<NM, CW>(sM[b], sW[b][mfma_ordinate(lane)], sA[wv], acc, fpart, lane);
         }
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
@@ -1711,8 +1715,10 @@ __device__ __forceinline__ void f_trials(const KParams &P, const double *__restr
     const int nch = P.nqPad / kChunk;
     const size_t ld = (size_t)P.nqPad;
     double mA[NM], mB[NM], wA = 0.0, wB = 0.0;
+    // kFp64 evaluates ordinate mfma_ordinate(lane) of each chunk, as chunk_mfma does
+    const int qo = MODE == kFp64 ? mfma_ordinate(lane) : lane;
     auto load = [&](double(&m)[NM], double &w, int ch) {
-        const size_t q = (size_t)ch * kChunk + lane;
+        const size_t q = (size_t)ch * kChunk + qo;
 #pragma unroll
         for (int i = 0; i < NM; ++i)
             m[i] = MT[i * ld + q];
@@ -1735,9 +1741,15 @@ __device__ __forceinline__ void f_trials(const KParams &P, const double *__restr
                     x = fma(a[i], m[i], x);
                 fpart[j] = fpart[j] + w * exp(x - shift);
             } else {
+                if constexpr (MODE == kFRef) {
 #pragma unroll
-                for (int i = 0; i < NM; ++i)
-                    x = x + a[i] * m[i];
+                    for (int i = 0; i < NM; ++i)
+                        x = x + a[i] * m[i];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < NM; ++i)
+                        x = fma(a[i], m[i], x);
+                }
                 if constexpr (MODE == kFRef) {
                     wave_lds_sync();
                     sE[lane] = det_exp(x);
@@ -1922,7 +1934,8 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 double fpart[1] = {0.0};
                 const double shift[1] = {C.shift};
                 double pre[NM];
-                double prew = wg[lane];
+                const int qo = mx ? lane : mfma_ordinate(lane);  // the lane's weight
+                double prew = wg[qo];
 #pragma unroll
                 for (int k = 0; k < NM; ++k)
                     pre[k] = Mg[lane + kWave * k];
@@ -1941,7 +1954,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                     wave_lds_sync();
                     if (ch + 1 < nch) {
                         const size_t base = (size_t)(ch + 1) * kChunk;
-                        prew = wg[base + lane];
+                        prew = wg[base + qo];
 #pragma unroll
                         for (int k = 0; k < NM; ++k)
                             pre[k] = Mg[base * NM + lane + kWave * k];
