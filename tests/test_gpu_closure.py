@@ -352,3 +352,29 @@ def test_closure_pass_budget_reports_and_leaves_cells_untouched(P, O, dev, tail)
     assert np.array_equal(ug[left], u[left]) and np.array_equal(ag[left], a0[left])
     # cells that finished inside the budget are the unbudgeted run's cells, bit for bit
     assert np.array_equal(ag[~left], afull[~left]) and np.array_equal(ug[~left], ufull[~left])
+
+
+def test_stats_summary_matches_per_cell_stats(P, O, dev):
+    """c3d_stats_summary (the device-side fold of the run statistics) against the same sums
+    over the per-cell records on the host, on a batch with ladder rungs."""
+    T = _tables(O, 2, 8)
+    a_star, u, a0 = O.closure_batch_input(T, 3000, 5, 2.0, 6.0)
+    import torch
+    ctx = P.Ctx(tables=T)
+    ud, ad = torch.from_numpy(u.copy()).to(dev), torch.from_numpy(a0.copy()).to(dev)
+    st = P.alloc_stats(u.shape[0], dev)
+    ctx.closure(ud, ad, P.closure_opts(maxIter=3), stats=st)
+    ctx.sync()
+    s = P.stats_to_numpy(st)
+    got = ctx.stats_summary(st)
+    reg = [0.0, 1e-3, 1e-2, 1e-1, 0.5]
+    assert got["cells"] == len(s)
+    assert got["sumIter"] == int(s["totalIter"].sum())
+    assert got["sumIterGamma"] == int(s["iterGamma"].sum())
+    assert got["maxIter"] == int(s["totalIter"].max())
+    assert got["maxIterGamma"] == int(s["iterGamma"].max())
+    assert got["rung"] == [int((np.abs(s["r"] - r) < 1e-10).sum()) for r in reg]
+    assert sum(got["rung"][1:]) > 0  # maxIter = 3 forces rungs
+    half = ctx.stats_summary(st, ncells=1000)
+    assert half["cells"] == 1000 and half["sumIter"] == int(s["totalIter"][:1000].sum())
+    ctx.close()
