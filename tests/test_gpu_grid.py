@@ -143,6 +143,36 @@ def test_update_steps_match_oracle(P, O, dev, T2, solver, pnk):
     ctx.close()
 
 
+def test_c0_configuration_ten_steps_bit_exact(P, O, dev, T2):
+    """The CPU-runnable configuration end to end: M_2, Q = 8 (128 ordinates), 20^3 periodic
+    cells on [-1,1]^3, sigma_s = 1, sigma_a = 0, rho = max(exp(-|x|^2/(2 0.1^2)), 1e-4),
+    minmod slopes (theta = 1), CFL 0.5, ten SSP-RK2 steps; reference-order mode against the
+    oracle with the deterministic exp/log, bit for bit, with the closure status of every
+    step."""
+    n = 20
+    d = 2.0 / n
+    u, sT, sS = O.init_grid(n, n, n, T2.nm, ic=1, sigma=0.1, floor=1e-4, cross=1.0)
+    alpha = O.init_alpha(u)
+    dt = 0.5 * (2 / 3) / (3 / d)
+    ur, ar = u.copy(), alpha.copy()
+    ng = u.shape[0]
+    scratch = [np.zeros_like(u), np.zeros_like(u), np.zeros(ng * T2.nq)]
+    ctx = P.Ctx(tables=T2)
+    ctx.set_precision(P.PREC_FP64_REFORDER)
+    ud, ad = _dev_arr(u, dev), _dev_arr(alpha, dev)
+    sTd, sSd = _dev_arr(sT, dev), _dev_arr(sS, dev)
+    with O.exp_mode(1):
+        for step in range(10):
+            rc = O.lib().orc_update(n, n, n, T2.nm, T2.nm2, T2.nq, 1, 0, ur, ar, scratch[0],
+                                    scratch[1], scratch[2], sT, sS, O.C.byref(T2.c), T2.ops,
+                                    O.C.byref(O.opts()), d, d, d, 1.0, dt, None, 0)
+            ctx.update(n, n, n, ud, ad, sTd, sSd, d, d, d, 1.0, dt, 1, 0, P.closure_opts())
+            assert ctx.sync() == (P.C3D_ERR_LADDER if rc else 0), step
+    assert np.array_equal(ud.cpu().numpy(), ur)
+    assert np.array_equal(ad.cpu().numpy(), ar)
+    ctx.close()
+
+
 def test_update_fp64_matrix_core_mode(P, O, dev, T2):
     """The default (matrix-core) mode over three M_N steps: the density agrees with the
     oracle to 1e-5 and all moments to 2e-3 of max|u| -- a closure decision that differs
