@@ -24,6 +24,7 @@ device); the CPU oracle timed on a bounded sample of the same inputs.
 import argparse
 import importlib.util
 import json
+import math
 import os
 import sys
 import time
@@ -289,7 +290,9 @@ def step_workload(args, P, torch, dev):
     every cell (the reference's default batched semantics, tol 1e-8), FP64.  A step is one
     full update() (update.cpp:185-275) on the device; the state keeps evolving across
     steps.  Also timed: the kinetic flux alone (ansatz + limited upwind stencil), against
-    its algorithmic HBM bytes with the ansatz materialised.
+    its algorithmic HBM bytes with the ansatz materialised; and P_N of the same order (the
+    moment flux, linear closure) from the same initial state to the same time with its own
+    dt, with its density compared to the M_N one.
 
     --step-precision mixed: the same problem advanced from one initial state with the FP64
     closure, with the mixed-precision one and with the reference-order FP64 one; `value` is
@@ -309,6 +312,7 @@ def step_workload(args, P, torch, dev):
     sS = torch.empty_like(sT)
     ctx.init_grid(n, n, n, u, sT, sS, alpha=a, ic=1, sigma=0.05, floor=1e-4, noise=0.01,
                   seed=args.seed, cross=1.0)
+    u_init = u.clone()
     theta = 1.0
     dt = 0.5 * (2.0 / (2.0 + theta)) / (3.0 / d)
     opts = P.closure_opts(maxIter=100, tol=1e-8, tolGamma=1.1, condHMax=1e10)
@@ -372,6 +376,34 @@ def step_workload(args, P, torch, dev):
         ms = 1e3 * sum(times["mixed"]) / args.steps
         fp64_ms = 1e3 * sum(times["fp64"]) / args.steps
     st = P.stats_to_numpy(stats)
+    # P_N of the same order on the same initial state (flux_moment.cpp, linear closure), with
+    # its own dt (init.cpp:237) and main.cpp's clipped last step, up to the time the M_N run
+    # reached: its cost per step and its density against the M_N one at equal t
+    pn = None
+    if not mixed:
+        t_end = (args.warmup + args.steps) * dt
+        dt_pn = 0.5 * 0.5 * d
+        up = u_init.clone()
+        t_pn, k_pn = 0.0, 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        while t_pn < t_end - 1e-14 * t_end:
+            h = min(dt_pn, t_end - t_pn)
+            ctx.update(n, n, n, up, None, sT, sS, d, d, d, theta, h, 0, 0, opts)
+            t_pn += h
+            k_pn += 1
+        torch.cuda.synchronize()
+        pn_ms = 1e3 * (time.perf_counter() - t0) / k_pn
+        rm = u.view(n + 4, n + 4, n + 4, nm)[2:-2, 2:-2, 2:-2, 0]
+        rp = up.view(n + 4, n + 4, n + 4, nm)[2:-2, 2:-2, 2:-2, 0]
+        pn = {"ms_per_step": round(pn_ms, 3), "value": round(ni / (pn_ms / 1e3), 1),
+              "steps": k_pn, "t": t_end, "dt": dt_pn,
+              "rho_rel_l2_diff_vs_mn": float(torch.linalg.vector_norm(rp - rm)
+                                             / torch.linalg.vector_norm(rm)),
+              "rho_max_rel_diff_vs_mn": float(((rp - rm).abs() / rm.abs()).max()),
+              "min_rho_pn": float(rp.min() / (2.0 * math.pi ** 0.5)),
+              "min_rho_mn": float(rm.min() / (2.0 * math.pi ** 0.5))}
+        del up
     # the kinetic flux alone, HIP events on the stream it is launched on
     flux = torch.zeros_like(u)
     reps = 5
@@ -429,6 +461,7 @@ def step_workload(args, P, torch, dev):
                    "grid": n, "moment_order": N, "quad_order": Q, "dt": dt, "status": rc,
                    "closure_iters_mean": round(float(st["totalIter"].mean()), 3),
                    "closure_iters_max": int(st["totalIter"].max())},
+        "pn_same_grid": pn,
         "fp64_trajectory": None if not mixed else {
             "ms_per_step": round(fp64_ms, 3), "value": round(ni / (fp64_ms / 1e3), 1),
             "speedup_of_mixed": round(fp64_ms / ms, 3),
