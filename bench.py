@@ -254,7 +254,9 @@ def main():
         "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(main_r["ms_per_step"], 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        # the mixed phase sums the Hessian on the FP32 matrix cores; the rest is FP64
+        "dtype": "f32+f64" if args.path.startswith("mixed") else "f64",
         "data": "synthetic",
         "config": {"workload": "closure-only batch: 1048576 independent M_3 dual problems, "
                                "24x48=1152 ordinates, alpha*~N(0,s^2), s~U[0.5,8], "
@@ -576,7 +578,7 @@ def orders_workload(args, P, torch, dev):
         "metric": "M_N cell-updates/s at ||grad h||<=1e-8, orders 1-4 mixed, mixed+NN, one GPU",
         "value": round(n / (ms_m / 1e3), 1), "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_m, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"closure-only batch: {n} cells, a quarter each of M_1..M_4, "
                                "Q=24, alpha*~N(0,s^2), s~U[0.5,8]",
