@@ -2022,10 +2022,10 @@ __global__ void k_warm_features(int64_t rows, State S, Lists L, const float *__r
 // the network's proposal anet[slot] (sums in FP32: the test only ranks two candidates, and a
 // proposal whose exponentials overflow loses it); the proposal becomes the starting point
 // (alpha, alpha0 and the line-search anchor) only if its gradient is smaller.  Either way the
-// mixed mode's first shift is log sum_q w e at the chosen point.  Four cells per wave, eight
-// candidate points: each lane forms the weights w_q e^{x} of its ordinate for the eight points
-// (VALU), and the gradients sum_q w_q e_q m_q accumulate on the FP32 matrix cores, point v in
-// row v of a 16x16 tile (rows 8..15 zero).
+// mixed mode's first shift is log sum_q w e at the chosen point.  Eight cells per wave,
+// sixteen candidate points: each lane forms the weights w_q e^{x} of its ordinate for the
+// sixteen points (VALU), and the gradients sum_q w_q e_q m_q accumulate on the FP32 matrix
+// cores, point v in row v of a 16x16 tile.
 template <int NM>
 __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double *__restrict__ Mg,
                                                          const double *__restrict__ wg, State S,
@@ -2033,7 +2033,7 @@ __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double 
 {
     constexpr int NT = (NM + 15) / 16;
     constexpr int NMS = NM | 1;
-    constexpr int CW = kCellsPerWave;
+    constexpr int CW = 8;  // 16 candidate points fill the rows of one 16x16 tile
     constexpr int NV = 2 * CW;  // point v = 2 c + (0: the caller's alpha, 1: the proposal)
     constexpr int RS = kRS;
     constexpr int kPer = (kChunk * NM + kBlock - 1) / kBlock;
