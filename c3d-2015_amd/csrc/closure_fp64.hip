@@ -1606,7 +1606,9 @@ struct Cell {
         d = own ? -d : 0.0;
         err = LaneVec<NM, G, EXACT>::dnrm2(g);
         const double da1 = LaneVec<NM, G, EXACT>::dasum(d);
-        gamma = EXACT ? det_exp(2.0 * da1 * P.minf) : exp(2.0 * da1 * P.minf);
+        // the deterministic exp in every mode: the device library's exp, inlined here, pushed
+        // the step kernel past 256 registers (24 spilled)
+        gamma = det_exp(2.0 * da1 * P.minf);
         if (mx) {
             // the stopping test belongs to the FP64 path: hand the cell over once its
             // FP32-accurate gradient is small, after taking this Newton step
