@@ -1226,9 +1226,10 @@ __device__ __forceinline__ void store_H32(const f4 *acc, double es, double *H, i
 }
 
 // Sweep pass of the mixed phase: 4 cells per wave, 16 per block sharing each LDS chunk.
-// Three waves per SIMD (at most 168 registers a lane) measured faster than two.
+// Three waves per SIMD (at most 168 registers a lane) measured faster than two for nm <= 16;
+// nm = 25 needs 241 registers and takes two (three spilled 86).
 template <int NM>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_sweep_mixed(KParams P, const double *__restrict__ Mg,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NM <= 16 ? 3 : 2))) void k_sweep_mixed(KParams P, const double *__restrict__ Mg,
                                                          const double *__restrict__ wg, State S,
                                                          const int *__restrict__ W,
                                                          const int *__restrict__ nWp)
