@@ -98,9 +98,9 @@ def main():
                          "and reports both differences from it after every step")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the closure is "
-              "embarrassingly parallel over cells; see DESIGN.md for the sharding design)",
-              file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (closure batches split "
+              "over cells with no exchange; a time-stepped grid would split into slabs with a "
+              "two-cell halo exchanged each stage)", file=sys.stderr)
         sys.exit(2)
 
     import torch

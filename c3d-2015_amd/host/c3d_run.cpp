@@ -163,7 +163,7 @@ void read_deck(Run &R, InputDeckReader &r)
     if (!(R.tFinal > 0.0) || !(R.outDt > 0.0))
         fail("TFINAL and OUTDT must be positive", nullptr);
     if (R.momentOrder < 1 || R.momentOrder > 4)
-        fail("MOMENT_ORDER must be 1..4 on this build", nullptr);
+        fail("MOMENT_ORDER must be 1..4 (the orders the closure kernels are built for)", nullptr);
     if (R.quadOrder < 1)
         fail("QUAD_ORDER must be positive", nullptr);
     for (int d = 0; d < 3; ++d)
