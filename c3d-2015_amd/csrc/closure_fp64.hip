@@ -2115,14 +2115,21 @@ __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double 
             const float *rowp = &sMr[b][lane * NMS];
             const float wl = sW[b][lane];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                float x = 0.0f;
+            for (int h = 0; h < 4; ++h) {
+                // four groups of four points, the scheduler kept from interleaving them: one
+                // group's operands are live at a time (152 registers for nm = 16, three waves
+                // per SIMD; 192 and two waves unsplit)
 #pragma unroll
-                for (int i = 0; i < NM; ++i)
-                    x = fmaf(sA[wv][v * NM + i], rowp[i], x);
-                const float we = wl * __expf(x);  // hardware exp2: ranking needs no more
-                fpart[v] += we;
-                sWE[wv][v * kChunk + (lane & 3) * 16 + (lane >> 2)] = we;
+                for (int v = h * NV / 4; v < (h + 1) * NV / 4; ++v) {
+                    float x = 0.0f;
+#pragma unroll
+                    for (int i = 0; i < NM; ++i)
+                        x = fmaf(sA[wv][v * NM + i], rowp[i], x);
+                    const float we = wl * __expf(x);  // hardware exp2: ranking needs no more
+                    fpart[v] += we;
+                    sWE[wv][v * kChunk + (lane & 3) * 16 + (lane >> 2)] = we;
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
             wave_lds_sync();
             const float *m0 = &sMfT[b][(rq * 4 + gq) * RS];
