@@ -65,6 +65,7 @@ SIGNATURES = [
     ("solveFluxKinetic_hip", None, [C.c_int] * 5 + [_d] * 3 + [_vp] * 7 + [C.c_int]),
     ("solveHInit_hip", None, [C.c_int] * 4 + [_vp] * 7 + [C.c_int]),
     ("solveH_hip", None, [C.c_int] * 4 + [_vp, _vp, C.c_int, C.c_int]),
+    ("c3d_seam_status", C.c_int, []),
     ("c3d_create", C.c_int, [C.POINTER(_vp), C.c_int, C.c_int]),
     ("c3d_create_from_tables", C.c_int, [C.POINTER(_vp), C.c_int, C.c_int] + [_vp] * 7 +
      [C.c_int, _vp, _vp, _vp]),

@@ -740,12 +740,19 @@ int c3d_stats_summary(c3d_ctx ctx, int64_t ncells, const c3d_cell_stats *stats,
 
 /* ------------------------------------------------------------------------------
  * Drop-ins for src/cuda/cuda_headers.h.  Like the reference, these keep their device
- * state in file statics created on first use and block on every call.
+ * state in file statics created on first use and block on every call.  Their staging
+ * buffers (alphas/Hessians, the alpha and flux grids) are kept and grown on demand, so a
+ * steady stream of calls of one size allocates nothing after the first.  They return void,
+ * as the reference's do; failures are printed (as cuda_solveH.cu:13-20 prints CUDA errors)
+ * and the last call's status is kept for c3d_seam_status().
  * ------------------------------------------------------------------------------ */
 static std::mutex g_dropin_mtx;
 static c3d_ctx g_h_ctx = nullptr;  // solveHInit_hip / solveH_hip
 static c3d_ctx g_f_ctx = nullptr;  // solveFluxKinetic_hip
 static int g_f_nm = 0, g_f_nq = 0;
+static DevBuf g_h_alpha, g_h_hess;  // solveH_hip staging
+static DevBuf g_f_alpha, g_f_flux;  // solveFluxKinetic_hip staging
+static int g_seam_status = C3D_OK;
 
 static int order_from_nm(int nm)
 {
@@ -758,16 +765,32 @@ static int quad_from_nq(int nq)
     return (2 * Q * Q == nq) ? Q : -1;
 }
 
+static void seam_fail(const char *who, int rc, hipError_t e)
+{
+    g_seam_status = rc;
+    if (e != hipSuccess)
+        std::fprintf(stderr, "%s: %s\n", who, hipGetErrorString(e));
+    else
+        std::fprintf(stderr, "%s: error %d\n", who, rc);
+}
+
+int c3d_seam_status(void)
+{
+    std::lock_guard<std::mutex> lk(g_dropin_mtx);
+    return g_seam_status;
+}
+
 void solveHInit_hip(int nm, int nm2, int nq, int batchSize, double *w, double *p, double *p2,
                     double *beta, double *betaVal, int *betaColInd, int *betaRowPtr,
                     int useGaunt)
 {
     (void)nm2;
-    (void)batchSize;
     std::lock_guard<std::mutex> lk(g_dropin_mtx);
+    g_seam_status = C3D_OK;
     int N = order_from_nm(nm), Q = quad_from_nq(nq);
-    if (N < 1 || Q < 1) {
-        std::fprintf(stderr, "solveHInit_hip: unsupported nm=%d nq=%d\n", nm, nq);
+    if (N < 1 || N > 4 || Q < 1 || !w || !p) {
+        seam_fail("solveHInit_hip: unsupported nm/nq or missing tables", C3D_ERR_ARG,
+                  hipSuccess);
         return;
     }
     if (g_h_ctx)
@@ -777,34 +800,54 @@ void solveHInit_hip(int nm, int nm2, int nq, int batchSize, double *w, double *p
     int nnz = (useGaunt && betaRowPtr) ? betaRowPtr[nm * nm] : 0;
     int rc = c3d_create_from_tables(&g_h_ctx, N, Q, w, w, w, w, p, p2, useGaunt ? beta : nullptr,
                                     nnz, betaVal, betaColInd, betaRowPtr);
-    if (rc != C3D_OK)
-        std::fprintf(stderr, "solveHInit_hip: error %d\n", rc);
+    if (rc != C3D_OK) {
+        seam_fail("solveHInit_hip", rc, hipSuccess);
+        return;
+    }
+    // staging for the batches the caller announces (optBatch.cpp:364-365 passes its size)
+    hipError_t e = hipSuccess;
+    if (batchSize > 0) {
+        e = grow(g_h_alpha, (size_t)batchSize * nm * sizeof(double));
+        if (e == hipSuccess)
+            e = grow(g_h_hess, (size_t)batchSize * nm * nm * sizeof(double));
+    }
+    if (e != hipSuccess)
+        seam_fail("solveHInit_hip", C3D_ERR_HIP, e);
 }
 
 void solveH_hip(int nm, int nm2, int nq, int batchSize, double *alphas, double *hs,
                 int useGaunt, int useGauntSparse)
 {
     (void)nm2;
-    (void)nq;
     std::lock_guard<std::mutex> lk(g_dropin_mtx);
-    if (!g_h_ctx || g_h_ctx->t.nm != nm || batchSize <= 0)
+    g_seam_status = C3D_OK;
+    if (!g_h_ctx || g_h_ctx->t.nm != nm || g_h_ctx->t.nq != nq || batchSize < 0 ||
+        (batchSize > 0 && (!alphas || !hs))) {
+        seam_fail("solveH_hip: not initialised for these sizes (solveHInit_hip) or bad args",
+                  C3D_ERR_ARG, hipSuccess);
         return;
-    double *dA = nullptr, *dH = nullptr;
-    size_t na = (size_t)batchSize * nm, nh = (size_t)batchSize * nm * nm;
-    if (hipMalloc(&dA, na * sizeof(double)) != hipSuccess ||
-        hipMalloc(&dH, nh * sizeof(double)) != hipSuccess) {
-        std::fprintf(stderr, "solveH_hip: allocation failed\n");
-    } else if (hipMemcpy(dA, alphas, na * sizeof(double), hipMemcpyHostToDevice) == hipSuccess) {
-        int rc = c3d_hessian(g_h_ctx, batchSize, dA, dH, useGaunt, useGauntSparse, nullptr);
-        if (rc == C3D_OK && hipMemcpy(hs, dH, nh * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
-        } else {
-            std::fprintf(stderr, "solveH_hip: error %d\n", rc);
-        }
     }
-    if (dA)
-        (void)hipFree(dA);
-    if (dH)
-        (void)hipFree(dH);
+    if (batchSize == 0)
+        return;
+    const size_t na = (size_t)batchSize * nm, nh = (size_t)batchSize * nm * nm;
+    hipError_t e = grow(g_h_alpha, na * sizeof(double));
+    if (e == hipSuccess)
+        e = grow(g_h_hess, nh * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpy(g_h_alpha.p, alphas, na * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        seam_fail("solveH_hip", C3D_ERR_HIP, e);
+        return;
+    }
+    int rc = c3d_hessian(g_h_ctx, batchSize, (const double *)g_h_alpha.p, (double *)g_h_hess.p,
+                         useGaunt, useGauntSparse, nullptr);
+    if (rc != C3D_OK) {
+        seam_fail("solveH_hip", rc, hipSuccess);
+        return;
+    }
+    e = hipMemcpy(hs, g_h_hess.p, nh * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+        seam_fail("solveH_hip", C3D_ERR_HIP, e);
 }
 
 void solveFluxKinetic_hip(int nx, int ny, int nz, int nm, int nq, double dx, double dy,
@@ -812,49 +855,50 @@ void solveFluxKinetic_hip(int nx, int ny, int nz, int nm, int nq, double dx, dou
                           double *mu, double *w, double *p, int isPn)
 {
     std::lock_guard<std::mutex> lk(g_dropin_mtx);
+    g_seam_status = C3D_OK;
     int N = order_from_nm(nm), Q = quad_from_nq(nq);
-    if (N < 1 || Q < 1 || nx < 5 || ny < 5 || nz < 5) {
-        std::fprintf(stderr, "solveFluxKinetic_hip: unsupported sizes\n");
+    if (N < 1 || N > 4 || Q < 1 || nx < 5 || ny < 5 || nz < 5 || !alpha || !flux || !xi ||
+        !eta || !mu || !w || !p) {
+        seam_fail("solveFluxKinetic_hip: unsupported sizes or missing arrays", C3D_ERR_ARG,
+                  hipSuccess);
         return;
     }
     if (!g_f_ctx || g_f_nm != nm || g_f_nq != nq) {
         if (g_f_ctx)
             c3d_destroy(g_f_ctx);
         g_f_ctx = nullptr;
-        if (c3d_create_from_tables(&g_f_ctx, N, Q, w, xi, eta, mu, p, nullptr, nullptr, 0,
-                                   nullptr, nullptr, nullptr) != C3D_OK) {
-            std::fprintf(stderr, "solveFluxKinetic_hip: context creation failed\n");
+        int rc = c3d_create_from_tables(&g_f_ctx, N, Q, w, xi, eta, mu, p, nullptr, nullptr, 0,
+                                        nullptr, nullptr, nullptr);
+        if (rc != C3D_OK) {
+            seam_fail("solveFluxKinetic_hip: context creation", rc, hipSuccess);
             return;
         }
         g_f_nm = nm;
         g_f_nq = nq;
     }
     const size_t n = (size_t)nx * ny * nz * nm;
-    double *dA = nullptr, *dF = nullptr;
-    if (hipMalloc(&dA, n * sizeof(double)) == hipSuccess &&
-        hipMalloc(&dF, n * sizeof(double)) == hipSuccess &&
-        hipMemcpy(dA, alpha, n * sizeof(double), hipMemcpyHostToDevice) == hipSuccess) {
+    hipError_t e = grow(g_f_alpha, n * sizeof(double));
+    if (e == hipSuccess)
+        e = grow(g_f_flux, n * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpy(g_f_alpha.p, alpha, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
         std::lock_guard<std::mutex> lk2(g_f_ctx->mtx);
         const int ix = nx - 4, iy = ny - 4, iz = nz - 4;
         const size_t ng = (size_t)nx * ny * nz;
-        hipError_t e = grow(g_f_ctx->ansatz, ng * nq * sizeof(double));
+        e = grow(g_f_ctx->ansatz, ng * nq * sizeof(double));
         double *A = (double *)g_f_ctx->ansatz.p;
         if (e == hipSuccess)
-            e = launch_ansatz(g_f_ctx, ix, iy, iz, dA, A, isPn, nullptr);
+            e = launch_ansatz(g_f_ctx, ix, iy, iz, (const double *)g_f_alpha.p, A, isPn, nullptr);
+        // theta = 2, as src/cuda/cuda_flux_kinetic.cu:82 hard-codes it
         if (e == hipSuccess)
-            e = launch_stencil(g_f_ctx, ix, iy, iz, A, dF, nullptr, nullptr, nullptr, 0.0, dx, dy,
-                               dz, 2.0, 0, 1, nullptr);
-        if (e == hipSuccess)
-            e = hipMemcpy(flux, dF, n * sizeof(double), hipMemcpyDeviceToHost);
-        if (e != hipSuccess)
-            std::fprintf(stderr, "solveFluxKinetic_hip: %s\n", hipGetErrorString(e));
-    } else {
-        std::fprintf(stderr, "solveFluxKinetic_hip: allocation/copy failed\n");
+            e = launch_stencil(g_f_ctx, ix, iy, iz, A, (double *)g_f_flux.p, nullptr, nullptr,
+                               nullptr, 0.0, dx, dy, dz, 2.0, 0, 1, nullptr);
     }
-    if (dA)
-        (void)hipFree(dA);
-    if (dF)
-        (void)hipFree(dF);
+    if (e == hipSuccess)
+        e = hipMemcpy(flux, g_f_flux.p, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+        seam_fail("solveFluxKinetic_hip", C3D_ERR_HIP, e);
 }
 
 }  // extern "C"

@@ -128,6 +128,13 @@ void solveHInit_hip(int nm, int nm2, int nq, int batchSize, double *w, double *p
 void solveH_hip(int nm, int nm2, int nq, int batchSize, double *alphas, double *hs,
                 int useGaunt, int useGauntSparse);
 
+/* Status (C3D_OK or C3D_ERR_*) of the most recent call of one of the three functions above,
+ * which return void as the reference's do and print their failures.  Their device staging
+ * buffers are kept between calls and grown on demand (no allocation per call at a steady
+ * size).  No counterpart in the reference (whose CUDA errors are printed and ignored,
+ * src/cuda/cuda_solveH.cu:13-20). */
+int c3d_seam_status(void);
+
 /* ======================================================================
  * 2. Device-resident API
  * ====================================================================== */
@@ -296,7 +303,9 @@ int c3d_update(c3d_ctx ctx, int nx, int ny, int nz, double *u, double *alpha,
  * W1[25][128] b1[128] W2[128][128] b2[128] W3[128][25] b3[25], z_j = b_j + sum_i x_i W[i][j].
  * With the warm start enabled, c3d_closure evaluates the network for every cell (FP16
  * weights on the matrix cores) and starts Newton from the proposal only if it has a smaller
- * ||grad h||_2 (FP64) than the caller's alpha; otherwise from the caller's alpha. */
+ * ||grad h||_2 than the caller's alpha, both gradients summed in FP32 (the test only ranks the
+ * two candidates; a proposal whose exponentials overflow FP32 loses); otherwise from the
+ * caller's alpha.  Newton itself, and every stopping test, stays as the precision mode says. */
 #define C3D_MLP_IN 25
 #define C3D_MLP_HID 128
 #define C3D_MLP_NPARAM 23065

@@ -1,0 +1,184 @@
+"""The benchmark configurations' own problem shapes on the GPU, against the CPU oracle, at
+sizes the oracle finishes in seconds:
+
+  C2  M_2 (9 moments), Q = 8, theta = 1, the sigma = 0.05 pulse with its 1% multiplicative
+      noise, 24^3 periodic cells: four SSP-RK2 steps in the reference-order mode bit for bit
+      against the oracle (deterministic exp/log), closure status included; P_2 (moment flux,
+      linear closure) on the same grid and state with its own dt (init.cpp:237), bit for bit
+      in any mode; the two densities at equal time.
+  C3  M_4 (25 moments), Q = 40 (3200 ordinates), the mixed-precision closure, 8^3 cells:
+      each of three steps started from the oracle's state, so the reported difference is
+      one step's; alpha, moments and density against the oracle's FP64 step, next to the
+      GPU's own FP64 step as the yardstick of the FP64 closure's rounding sensitivity.
+  C4  the orders 1..4 at Q = 24 with the warm start trained on solver pairs and the mixed
+      closure: every order's cells against the oracle's FP64 closure of the same inputs."""
+import math
+
+import numpy as np
+import pytest
+
+from test_gpu_closure import moment_residual
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev_arr(x, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def _inner(x, n, nm):
+    return x.reshape(n + 4, n + 4, n + 4, nm)[2:-2, 2:-2, 2:-2].reshape(-1, nm)
+
+
+def _orc_step(O, T, n, u, a, sT, sS, d, theta, dt, solver):
+    ng = u.shape[0]
+    return O.lib().orc_update(n, n, n, T.nm, T.nm2, T.nq, solver, 0, u, a, np.zeros_like(u),
+                              np.zeros_like(u), np.zeros(ng * T.nq), sT, sS, O.C.byref(T.c),
+                              T.ops, O.C.byref(O.opts()), d, d, d, theta, dt, None, 0)
+
+
+def test_c2_pulse_steps_bit_exact_and_pn_same_grid(P, O, dev):
+    import torch
+    T = O.Tables(2, 8)
+    n = 24
+    d = 2.0 / n
+    theta = 1.0
+    u, sT, sS = O.init_grid(n, n, n, T.nm, ic=1, sigma=0.05, floor=1e-4, noise=0.01,
+                            seed=0xC3D2015, cross=1.0)
+    alpha = O.init_alpha(u)
+    dt = 0.5 * (2.0 / (2.0 + theta)) / (3.0 / d)
+    ctx = P.Ctx(tables=T)
+    # the device initial state is the oracle's (counter-RNG noise identical, exp to an ulp)
+    ud0 = torch.zeros((u.shape[0], T.nm), dtype=torch.float64, device=dev)
+    st0, ss0 = (torch.zeros(u.shape[0], dtype=torch.float64, device=dev) for _ in range(2))
+    ctx.init_grid(n, n, n, ud0, st0, ss0, ic=1, sigma=0.05, floor=1e-4, noise=0.01,
+                  seed=0xC3D2015, cross=1.0)
+    torch.cuda.synchronize()
+    assert np.allclose(ud0.cpu().numpy(), u, rtol=2e-15, atol=0)
+    ctx.set_precision(P.PREC_FP64_REFORDER)
+    ur, ar = u.copy(), alpha.copy()
+    ud, ad = _dev_arr(u, dev), _dev_arr(alpha, dev)
+    sTd, sSd = _dev_arr(sT, dev), _dev_arr(sS, dev)
+    steps = 4
+    with O.exp_mode(1):
+        for k in range(steps):
+            rc = _orc_step(O, T, n, ur, ar, sT, sS, d, theta, dt, 1)
+            ctx.update(n, n, n, ud, ad, sTd, sSd, d, d, d, theta, dt, 1, 0, P.closure_opts())
+            assert ctx.sync() == (P.C3D_ERR_LADDER if rc else 0), k
+            assert np.array_equal(ud.cpu().numpy(), ur), k
+            assert np.array_equal(ad.cpu().numpy(), ar), k
+    # P_2 to the same time with its own dt, in the default mode: no exponential, bit for bit
+    ctx.set_precision(P.PREC_FP64)
+    t_end = steps * dt
+    dt_pn = 0.5 * 0.5 * d
+    up = u.copy()
+    upd = _dev_arr(u, dev)
+    t = 0.0
+    while t < t_end - 1e-14 * t_end:
+        h = min(dt_pn, t_end - t)
+        _orc_step(O, T, n, up, np.zeros_like(up), sT, sS, d, theta, h, 0)
+        ctx.update(n, n, n, upd, None, sTd, sSd, d, d, d, theta, h, 0, 0, P.closure_opts())
+        t += h
+    torch.cuda.synchronize()
+    assert np.array_equal(upd.cpu().numpy(), up)
+    rm, rp = _inner(ur, n, T.nm)[:, 0], _inner(up, n, T.nm)[:, 0]
+    rel = np.linalg.norm(rp - rm) / np.linalg.norm(rm)
+    print(f"C2 24^3: M_2 vs P_2 density at t={t_end:.4f}: rel L2 {rel:.3e}, "
+          f"min rho M_2 {rm.min() / (2 * math.sqrt(math.pi)):.3e} "
+          f"P_2 {rp.min() / (2 * math.sqrt(math.pi)):.3e}")
+    # the two closures model the same transport from one state: different, not far apart
+    # (oracle at this size: 0.107); M_N keeps the density positive where P_2 does not (P_2
+    # reaches -0.007); both conserve mass (sigma_a = 0; oracle: equal to 5.5e-13)
+    assert 0.01 < rel < 0.3
+    assert rm.min() > 0
+    assert abs(rm.sum() - rp.sum()) <= 1e-11 * rm.sum()
+    ctx.close()
+
+
+def test_c3_mixed_m4_step_error_against_oracle(P, O, dev):
+    import torch
+    T = O.Tables(4, 40)
+    nm = T.nm
+    n = 8
+    d = 2.0 / n
+    theta = 1.0
+    u, sT, sS = O.init_grid(n, n, n, nm, ic=1, sigma=0.25, floor=1e-4, noise=0.01, seed=3)
+    alpha = O.init_alpha(u)
+    dt = 0.5 * (2.0 / (2.0 + theta)) / (3.0 / d)
+    ctx = P.Ctx(tables=T)
+    sTd, sSd = _dev_arr(sT, dev), _dev_arr(sS, dev)
+    uo, ao = u.copy(), alpha.copy()
+    for k in range(3):
+        u_in, a_in = uo.copy(), ao.copy()
+        rc = _orc_step(O, T, n, uo, ao, sT, sS, d, theta, dt, 1)
+        out = {}
+        for prec in (P.PREC_FP64, P.PREC_MIXED):
+            ctx.set_precision(prec)
+            ud, ad = _dev_arr(u_in, dev), _dev_arr(a_in, dev)
+            ctx.update(n, n, n, ud, ad, sTd, sSd, d, d, d, theta, dt, 1, 0, P.closure_opts())
+            rcg = ctx.sync()
+            torch.cuda.synchronize()
+            out[prec] = (ud.cpu().numpy(), ad.cpu().numpy(), rcg)
+        ui, ai = _inner(uo, n, nm), _inner(ao, n, nm)
+
+        def err(v):
+            ug, ag = _inner(v[0], n, nm), _inner(v[1], n, nm)
+            du = np.abs(ug - ui).max(axis=1) / np.abs(ui[:, 0])
+            da = np.abs(ag - ai).max(axis=1) / np.abs(ai).max(axis=1)
+            drho = np.abs(ug[:, 0] - ui[:, 0]) / np.abs(ui[:, 0])
+            return da.max(), du.max(), drho.max(), (du <= 1e-6).mean()
+
+        em, ef = err(out[P.PREC_MIXED]), err(out[P.PREC_FP64])
+        print(f"C3 step {k + 1}: mixed alpha {em[0]:.2e} moments {em[1]:.2e} rho {em[2]:.2e} "
+              f"within 1e-6 {em[3]:.4f} | fp64 alpha {ef[0]:.2e} moments {ef[1]:.2e} "
+              f"rho {ef[2]:.2e} within 1e-6 {ef[3]:.4f} | oracle rc {rc}")
+        assert np.isfinite(out[P.PREC_MIXED][0]).all()
+        assert out[P.PREC_MIXED][2] == out[P.PREC_FP64][2]
+        # one step's difference: the mixed closure stays within the FP64 closure's own
+        # rounding sensitivity (a cell whose rung differs moves by r >= 1e-3 of u)
+        assert em[2] <= max(10 * ef[2], 1e-9), (em, ef)
+        assert em[3] >= min(ef[3] - 0.02, 0.97), (em, ef)
+    ctx.close()
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+def test_c4_warm_started_mixed_orders_against_oracle(P, O, dev, N):
+    import torch
+    Q = 24
+    T = O.Tables(N, Q)
+    ctx = P.Ctx(tables=T)
+    # solver-generated training pairs (FP64 closure from the isotropic alpha)
+    npairs = 100_000
+    ut = torch.empty((npairs, T.nm), dtype=torch.float64, device=dev)
+    at = torch.empty_like(ut)
+    ctx.make_closure_batch(npairs, 0xA11CE + N, 0.5, 8.0, ut, at)
+    ctx.closure(ut, at, P.closure_opts())
+    ctx.sync()
+    l0, l1 = ctx.mlp_train(ut, at, mse_steps=800, dual_steps=800, batch=8192, lr=3e-3, seed=N)
+    del ut, at
+    ctx.set_warmstart(True)
+    ctx.set_precision(P.PREC_MIXED)
+    n = 1500
+    a_star, u, a0 = O.closure_batch_input(T, n, 0xC3D2015 + 40 + N, 0.5, 8.0)
+    uo, ao = u.copy(), a0.copy()
+    rc_o, so = O.closure(T, uo, ao, O.opts())
+    ud, ad = _dev_arr(u, dev), _dev_arr(a0, dev)
+    st = P.alloc_stats(n, dev)
+    ctx.closure(ud, ad, P.closure_opts(), stats=st)
+    rc = ctx.sync()
+    ug, ag, sg = ud.cpu().numpy(), ad.cpu().numpy(), P.stats_to_numpy(st)
+    ctx.close()
+    conv = (sg["flags"] & P.FLAG_LADDER) == 0
+    assert (sg["err"][conv] <= 1e-8).all()
+    assert moment_residual(T, ug[conv], ag[conv]).max() <= 1.5e-8
+    du = np.abs(ug - uo).max(axis=1) / np.abs(u[:, 0])
+    same_rung = (sg["regIndex"] == so["regIndex"]).mean()
+    within = (du <= 1e-6).mean()
+    took = ((sg["flags"] & P.FLAG_WARM) != 0).mean()
+    print(f"C4 M_{N}: train {l0:.3f}->{l1:.3f} proposal taken {took:.3f} iters "
+          f"{sg['totalIter'].mean():.2f} (oracle from isotropic {so['totalIter'].mean():.2f}) "
+          f"rung-agree {same_rung:.4f} moments within 1e-6 {within:.4f}")
+    assert rc == (P.C3D_ERR_LADDER if rc_o else 0) or rc == 0
+    assert took > 0.5 and sg["totalIter"].mean() < so["totalIter"].mean()
+    assert same_rung >= 0.97 and within >= 0.97
