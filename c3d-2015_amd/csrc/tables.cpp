@@ -91,10 +91,21 @@ double real_sh(int m, int n, double mu, double phi)
     return sph_plm(n, 0, mu);
 }
 
-// Least-squares beta for every (i,j) by Householder QR of the nq x nm2 matrix M2.
+// Least-squares beta for every (i,j) by Householder QR of the nq x nm2 matrix M2.  With fewer
+// ordinates than degree-2N functions (nq < nm2) the problem is underdetermined (GSL's QR solve
+// of init.cpp:287-337 refuses it too): no coefficients, nnz = 0, and a context then declines
+// Gaunt-form requests (its check that beta reproduces the basis products fails).
 void gaunt(HostTables &t)
 {
     const int nq = t.nq, nm = t.nm, nm2 = t.nm2;
+    if (nq < nm2) {
+        t.beta.assign((size_t)nm * nm * nm2, 0.0);
+        t.betaRowPtr.assign(nm * nm + 1, 0);
+        t.betaVal.clear();
+        t.betaColInd.clear();
+        t.nnz = 0;
+        return;
+    }
     std::vector<double> R((size_t)nq * nm2);  // column-major working copy
     for (int q = 0; q < nq; ++q)
         for (int k = 0; k < nm2; ++k)

@@ -219,6 +219,14 @@ void orc_quadrature(int N, int Q, double *w, double *xi, double *eta, double *mu
 int orc_gaunt(int nm, int nm2, int nq, const double *M2, double *beta, double *betaVal,
               int32_t *betaColInd, int32_t *betaRowPtr)
 {
+    /* fewer ordinates than degree-2N functions: underdetermined (GSL's QR least squares
+     * refuses it); beta stays zero, and the CSR (if asked for) is empty */
+    if (nq < nm2) {
+        memset(beta, 0, (size_t)nm * nm * nm2 * sizeof(double));
+        if (betaRowPtr)
+            memset(betaRowPtr, 0, (size_t)(nm * nm + 1) * sizeof(int32_t));
+        return 0;
+    }
     /* A = M2 (nq x nm2), column-major copy for Householder */
     double *A = (double *)malloc((size_t)nq * nm2 * sizeof(double));
     double *tau = (double *)malloc(nm2 * sizeof(double));

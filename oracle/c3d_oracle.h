@@ -78,6 +78,8 @@ double orc_sph_plm(int l, int m, double x);
 double orc_real_sh(int m, int n, double mu, double phi);
 void   orc_quadrature(int N, int Q, double *w, double *xi, double *eta, double *mu,
                       double *M, double *M2);
+/* least-squares Gaunt beta (init.cpp:287-337); returns nnz, or 0 with beta zero when
+ * nq < nm2 (underdetermined) */
 int    orc_gaunt(int nm, int nm2, int nq, const double *M2, double *beta,
                  double *betaVal, int32_t *betaColInd, int32_t *betaRowPtr);
 void   orc_pn_operators(int nm, int nq, const double *w, const double *xi, const double *eta,

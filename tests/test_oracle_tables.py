@@ -97,3 +97,16 @@ def test_product_tables_match_oracle(O, N, Q):
     assert np.array_equal(t["betaRowPtr"], T.betaRowPtr)
     assert np.array_equal(t["betaColInd"], T.betaColInd[:T.nnz])
     assert np.abs(t["pnOps"] - T.ops).max() < 1e-14
+
+
+def test_gaunt_underdetermined_quadrature_is_declined(O):
+    """Fewer ordinates than degree-2N basis functions (N = 4, Q = 6: 72 < 81): the least
+    squares of init.cpp:287-337 is underdetermined; the oracle and the library both build no
+    coefficients (nnz = 0, beta zero) instead of writing past their work arrays."""
+    import numpy as np
+    from conftest import product
+    T = O.Tables(4, 6)
+    assert T.nq < T.nm2 and T.nnz == 0 and not T.beta.any()
+    h = product().tables_host(4, 6)
+    assert h["nnz"] == 0 and not h["beta"].any()
+    assert np.allclose(h["w"].sum(), 4 * np.pi)
