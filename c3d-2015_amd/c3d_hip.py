@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libc3d_hip.so")
 
 (C3D_OK, C3D_ERR_ARG, C3D_ERR_HIP, C3D_ERR_U0, C3D_ERR_LADDER, C3D_ERR_UNSUPPORTED,
- C3D_ERR_BUDGET) = range(7)
+ C3D_ERR_BUDGET, C3D_ERR_SCRATCH) = range(8)
+NOALLOC_SCRATCH_BYTES = 128 << 20
 FLAG_ISO_RESTART, FLAG_U0_NONPOS, FLAG_LADDER, FLAG_BUDGET, FLAG_FP64_FALLBACK = 1, 2, 4, 8, 16
 PREC_FP64, PREC_FP64_REFORDER, PREC_MIXED = 0, 1, 2
 
@@ -74,6 +75,8 @@ SIGNATURES = [
     ("c3d_sizes", C.c_int, [_vp] + [C.POINTER(C.c_int)] * 4),
     ("c3d_get_tables", C.c_int, [_vp] * 12),
     ("c3d_closure", C.c_int, [_vp, _i64, _vp, _vp, C.POINTER(ClosureOpts), _vp, _vp]),
+    ("c3d_closure_noalloc", C.c_int, [_vp, _i64, _vp, _vp, C.POINTER(ClosureOpts), _vp,
+                                      _vp]),
     ("c3d_set_profiling", C.c_int, [_vp, C.c_int]),
     ("c3d_get_closure_profile", C.c_int, [_vp, C.POINTER(ClosureProfile)]),
     ("c3d_set_tail", C.c_int, [_vp, _i32, _i32]),

@@ -105,7 +105,9 @@ hipError_t grow(DevBuf &b, size_t bytes);
 // closure_fp64.hip
 hipError_t launch_closure_fp64(c3d_ctx_s *ctx, int64_t ncells, double *u, double *alpha,
                                const c3d_closure_opts &o, c3d_cell_stats *stats,
-                               hipStream_t s);
+                               hipStream_t s, const DevBuf *ext = nullptr);
+// the device address of c3d_closure_noalloc's scratch array, looked up once per device
+hipError_t closure_scratch_setup(int device);
 hipError_t launch_dense_solve(int nm, int64_t n, const double *H, const double *g, double *d,
                               double *rcond, int *info, double *L, int lapackOrder, hipStream_t s);
 hipError_t launch_hessian_fp64(c3d_ctx_s *ctx, int64_t ncells, const double *alpha,

@@ -46,6 +46,8 @@ int finish_ctx(c3d_ctx_s *ctx)
     if (hipDeviceGetAttribute(&ctx->numCU, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
         hipSuccess)
         return C3D_ERR_HIP;
+    if (closure_scratch_setup(ctx->device) != hipSuccess)
+        return C3D_ERR_HIP;
     ctx->cc.m0 = t.M[0];
     ctx->cc.isoAlpha0 = 2.0 * sqrt(M_PI) * log(0.5 / sqrt(M_PI));
     {

@@ -2427,17 +2427,46 @@ hipEvent_t *take_events(c3d_ctx_s *ctx, int n, int kind)
     return ev;
 }
 
+// Device bytes of a closure over n cells of order NM: the per-cell Newton state, H, the
+// worklists and their counters, then (warm start) the features and proposals at the next
+// 256-byte boundary.
+size_t closure_state_bytes(size_t n, int nm)
+{
+    return n * (kVec * nm + kScal + (size_t)nm * nm + 1) * sizeof(double) +
+           n * (kInt + 4) * sizeof(int) + 8 * sizeof(int);
+}
+size_t warm_feature_bytes(size_t n) { return (n * C3D_MLP_IN * sizeof(float) + 255) / 256 * 256; }
+size_t closure_total_bytes(size_t n, int nm, bool warm)
+{
+    size_t b = closure_state_bytes(n, nm);
+    if (warm)
+        b = (b + 255) / 256 * 256 + warm_feature_bytes(n) + n * nm * sizeof(double);
+    return b;
+}
+
+// ext: a caller-provided device buffer of ext->bytes for all of the closure's scratch (the
+// no-allocation entry point), or null for the context's own buffers, grown on demand.
+// With ext the closure allocates nothing and records no profiling events.
 template <int NM>
 hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alpha,
-                       c3d_cell_stats *stats, int maxPasses, hipStream_t s)
+                       c3d_cell_stats *stats, int maxPasses, hipStream_t s, const DevBuf *ext)
 {
     const size_t n = (size_t)P.ncells;
-    size_t bytes = n * (kVec * NM + kScal + (size_t)NM * NM + 1) * sizeof(double) +
-                   n * (kInt + 4) * sizeof(int) + 8 * sizeof(int);
-    hipError_t e = grow(ctx->closureState, bytes);
-    if (e != hipSuccess)
-        return e;
-    char *base = (char *)ctx->closureState.p;
+    const bool warm = ctx->warm && ctx->mlp.have;
+    const size_t bytes = closure_state_bytes(n, NM);
+    hipError_t e = hipSuccess;
+    char *base;
+    if (ext) {
+        if (closure_total_bytes(n, NM, warm) > ext->bytes)
+            return hipErrorInvalidValue;
+        base = (char *)ext->p;
+    } else {
+        e = grow(ctx->closureState, bytes);
+        if (e != hipSuccess)
+            return e;
+        base = (char *)ctx->closureState.p;
+    }
+    const bool prof = ctx->profiling && !ext;
     State S;
     S.vec = (double *)base;
     S.scal = S.vec + n * kVec * NM;
@@ -2461,21 +2490,27 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
     ctx->lastPasses = 0;
     ctx->lastTailLaunches = 0;
     ctx->lastStream = s;
-    unsigned long long *cntBulk = ctx->profiling ? ctx->d_sweepCount : nullptr;
-    unsigned long long *cntTail = ctx->profiling ? ctx->d_sweepCount + 1 : nullptr;
-    if (ctx->profiling)
+    unsigned long long *cntBulk = prof ? ctx->d_sweepCount : nullptr;
+    unsigned long long *cntTail = prof ? ctx->d_sweepCount + 1 : nullptr;
+    if (prof)
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(3), 0, s, ctx->d_sweepCount);
     hipLaunchKernelGGL(k_init<NM>, dim3(grid_cap(ctx, 16)), dim3(kBlock), 0, s, P, u, alpha, S,
                        Lg[0], stats, ctx->d_status);
-    if (ctx->warm && ctx->mlp.have) {
-        const size_t xb = (n * C3D_MLP_IN * sizeof(float) + 255) / 256 * 256;
-        e = grow(ctx->warmBuf, xb + n * NM * sizeof(double));
-        if (e != hipSuccess)
-            return e;
-        float *X = (float *)ctx->warmBuf.p;
-        double *anet = (double *)((char *)ctx->warmBuf.p + xb);
-        hipEvent_t *ev = ctx->profiling ? take_events(ctx, 2, 2) : nullptr;
-        if (ctx->profiling && !ev)
+    if (warm) {
+        const size_t xb = warm_feature_bytes(n);
+        char *wbase;
+        if (ext) {
+            wbase = base + (bytes + 255) / 256 * 256;
+        } else {
+            e = grow(ctx->warmBuf, xb + n * NM * sizeof(double));
+            if (e != hipSuccess)
+                return e;
+            wbase = (char *)ctx->warmBuf.p;
+        }
+        float *X = (float *)wbase;
+        double *anet = (double *)(wbase + xb);
+        hipEvent_t *ev = prof ? take_events(ctx, 2, 2) : nullptr;
+        if (prof && !ev)
             return hipErrorOutOfMemory;
         if (ev)
             (void)hipEventRecord(ev[0], s);
@@ -2499,8 +2534,8 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
         const bool tail = ctx->tailCells > 0 && left <= ctx->tailCells;
         if (tail) {
             const int K = std::min(ctx->tailPasses, maxPasses - pass);
-            hipEvent_t *ev = ctx->profiling ? take_events(ctx, 2, 1) : nullptr;
-            if (ctx->profiling && !ev)
+            hipEvent_t *ev = prof ? take_events(ctx, 2, 1) : nullptr;
+            if (prof && !ev)
                 return hipErrorOutOfMemory;
             if (ev)
                 (void)hipEventRecord(ev[0], s);
@@ -2529,8 +2564,8 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
             pass += K;
             ++ctx->lastTailLaunches;
         } else {
-            hipEvent_t *ev = ctx->profiling ? take_events(ctx, 4, 0) : nullptr;
-            if (ctx->profiling && !ev)
+            hipEvent_t *ev = prof ? take_events(ctx, 4, 0) : nullptr;
+            if (prof && !ev)
                 return hipErrorOutOfMemory;
             if (ev)
                 (void)hipEventRecord(ev[0], s);
@@ -2579,7 +2614,7 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
 
 hipError_t launch_closure_fp64(c3d_ctx_s *ctx, int64_t ncells, double *u, double *alpha,
                                const c3d_closure_opts &o, c3d_cell_stats *stats,
-                               hipStream_t s)
+                               hipStream_t s, const DevBuf *ext)
 {
     KParams P;
     P.ncells = ncells;
@@ -2603,10 +2638,10 @@ hipError_t launch_closure_fp64(c3d_ctx_s *ctx, int64_t ncells, double *u, double
         return hipErrorInvalidValue;
     const int maxPasses = ctx->maxPasses;
     switch (ctx->t.nm) {
-    case 4: return run_closure<4>(ctx, P, u, alpha, stats, maxPasses, s);
-    case 9: return run_closure<9>(ctx, P, u, alpha, stats, maxPasses, s);
-    case 16: return run_closure<16>(ctx, P, u, alpha, stats, maxPasses, s);
-    case 25: return run_closure<25>(ctx, P, u, alpha, stats, maxPasses, s);
+    case 4: return run_closure<4>(ctx, P, u, alpha, stats, maxPasses, s, ext);
+    case 9: return run_closure<9>(ctx, P, u, alpha, stats, maxPasses, s, ext);
+    case 16: return run_closure<16>(ctx, P, u, alpha, stats, maxPasses, s, ext);
+    case 25: return run_closure<25>(ctx, P, u, alpha, stats, maxPasses, s, ext);
     default: return hipErrorInvalidValue;
     }
 }
@@ -2658,3 +2693,82 @@ hipError_t launch_hessian_fp64(c3d_ctx_s *ctx, int64_t ncells, const double *alp
 }
 
 }  // namespace c3d
+
+// ---------------------------------------------------------------- no-allocation closure
+// c3d_closure_noalloc keeps all of a closure's scratch (Newton state, Hessians, worklists,
+// warm-start features and proposals) in this fixed array; every device the library runs on
+// holds it until the process ends.  Host code never passes the array's name to a kernel or a
+// copy: it uses the device address looked up once per device at context creation.
+static __device__ __align__(256) unsigned char scratch[C3D_NOALLOC_SCRATCH_BYTES];
+
+namespace c3d {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMtx;
+void *g_scratchPtr[kMaxDevices] = {};
+
+void *scratch_ptr(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratchMtx);
+    return g_scratchPtr[device];
+}
+}  // namespace
+
+hipError_t closure_scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratchMtx);
+    if (g_scratchPtr[device])
+        return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess)
+        return e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess)
+        return e;
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) {
+        const hipError_t e2 = hipSetDevice(cur);
+        if (e == hipSuccess)
+            e = e2;
+    }
+    if (e == hipSuccess && !ptr)
+        e = hipErrorInvalidSymbol;
+    if (e == hipSuccess)
+        g_scratchPtr[device] = ptr;
+    return e;
+}
+
+}  // namespace c3d
+
+extern "C" int c3d_closure_noalloc(c3d_ctx ctx, int64_t ncells, double *u, double *alpha,
+                                   const c3d_closure_opts *opts, c3d_cell_stats *stats,
+                                   void *stream)
+{
+    using namespace c3d;
+    if (!ctx || !opts || ncells < 0 || (ncells > 0 && (!u || !alpha)) || opts->maxIter < 1)
+        return C3D_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    if (opts->useGaunt) {
+        // served by the direct form only once c3d_closure / c3d_hessian has checked beta
+        if (ctx->precision == C3D_PREC_FP64_REFORDER ||
+            ctx->gauntDirect[opts->useGauntSparse ? 1 : 0] != 1)
+            return C3D_ERR_UNSUPPORTED;
+    }
+    const bool warm = ctx->warm && ctx->mlp.have;
+    if (closure_total_bytes((size_t)ncells, ctx->t.nm, warm) > (size_t)C3D_NOALLOC_SCRATCH_BYTES)
+        return C3D_ERR_SCRATCH;
+    void *ptr = scratch_ptr(ctx->device);
+    if (!ptr)
+        return C3D_ERR_HIP;
+    DevBuf ext;
+    ext.p = ptr;
+    ext.bytes = C3D_NOALLOC_SCRATCH_BYTES;
+    const hipError_t e =
+        launch_closure_fp64(ctx, ncells, u, alpha, *opts, stats, (hipStream_t)stream, &ext);
+    return e == hipSuccess ? C3D_OK : C3D_ERR_HIP;
+}

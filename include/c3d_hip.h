@@ -38,6 +38,7 @@ extern "C" {
 #define C3D_ERR_UNSUPPORTED 5  /* option combination not implemented */
 #define C3D_ERR_BUDGET 6       /* cells still unconverged after the pass budget
                                   (c3d_set_pass_budget); their u, alpha are left as given */
+#define C3D_ERR_SCRATCH 7      /* c3d_closure_noalloc: the cells do not fit its fixed scratch */
 
 /* per-cell outcome flags (c3d_cell_stats.flags) */
 #define C3D_FLAG_ISO_RESTART 1 /* isotropic restart fired (optBatch.cpp:208-218) */
@@ -181,6 +182,25 @@ int c3d_get_tables(c3d_ctx ctx, double *w, double *xi, double *eta, double *mu, 
  * c3d_sync.  Scratch: about (8*nm + nm*nm + 12)*8 bytes per cell, owned by the context. */
 int c3d_closure(c3d_ctx ctx, int64_t ncells, double *u, double *alpha,
                 const c3d_closure_opts *opts, c3d_cell_stats *stats, void *stream);
+
+/* c3d_closure without any device allocation, for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart
+ * in the reference.  Same arguments, semantics and results as c3d_closure (the same kernels;
+ * bit-identical outputs), but all of its scratch lives in one fixed device array of
+ * C3D_NOALLOC_SCRATCH_BYTES that every device the library runs on holds from its first
+ * context until the process ends; its address is looked up at context creation.  Limits:
+ *   bytes(ncells) = ncells*((8 nm + 12 + nm^2)*8 + 44) + 32, plus, with the warm start on,
+ *   256-byte alignment + ncells*(100 + 8 nm);  it must not exceed the array, which gives at
+ *   most about 256k (nm 4), 98k (nm 9), 41.7k (nm 16), 19.9k (nm 25) cells, or 204k, 87k,
+ *   39k, 19k with the warm start.  Larger batches return C3D_ERR_SCRATCH and do nothing.
+ * No profiling events are recorded (c3d_get_closure_profile reports passes only).  A Gaunt
+ * request is served only after c3d_closure or c3d_hessian has checked the context's beta
+ * (else C3D_ERR_UNSUPPORTED).  Every call on one device shares that device's array and
+ * overwrites it (each call initialises what it reads): calls that overlap on one device --
+ * from other threads, streams or contexts -- race and must be serialised by the caller. */
+#define C3D_NOALLOC_SCRATCH_BYTES (128u << 20)
+int c3d_closure_noalloc(c3d_ctx ctx, int64_t ncells, double *u, double *alpha,
+                        const c3d_closure_opts *opts, c3d_cell_stats *stats, void *stream);
 
 /* Select the arithmetic of the closure and of the kinetic-flux ansatz (C3D_PREC_*).
  * C3D_PREC_FP64_REFORDER evaluates H(alpha) entry by entry in ordinate order
