@@ -77,6 +77,11 @@ def main():
                     default="mixed+nn")
     ap.add_argument("--mixed-cond", type=float, default=1e8)
     ap.add_argument("--mixed-iter", type=int, default=40)
+    ap.add_argument("--mixed-waves", type=int, choices=(2, 3), default=None,
+                    help="waves per SIMD of the mixed sweep (default: the library's)")
+    ap.add_argument("--finish-cond", type=int, default=None,
+                    help="10 log10 of the condition indicator above which a mixed cell "
+                         "finishes on FP64 Hessians (default: the library's)")
     ap.add_argument("--train-pairs", type=int, default=1 << 22)
     ap.add_argument("--train-steps", type=int, default=4000, help="MSE steps")
     ap.add_argument("--dual-steps", type=int, default=40000, help="dual-gap steps")
@@ -114,6 +119,10 @@ def main():
     n = args.cells
     ctx = P.Ctx(N, Q)
     ctx.set_mixed(args.mixed_cond, args.mixed_iter)
+    if args.mixed_waves is not None:
+        ctx.set_launch(P.LAUNCH_MIXED_WAVES, args.mixed_waves)
+    if args.finish_cond is not None:
+        ctx.set_launch(P.LAUNCH_MIXED_FINISH_COND, args.finish_cond)
     nm, nq = ctx.nm, ctx.nq
     u0 = torch.empty((n, nm), dtype=torch.float64, device=dev)
     a0 = torch.empty_like(u0)

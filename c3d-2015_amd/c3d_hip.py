@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libc3d_hip.so")
 NOALLOC_SCRATCH_BYTES = 128 << 20
 FLAG_ISO_RESTART, FLAG_U0_NONPOS, FLAG_LADDER, FLAG_BUDGET, FLAG_FP64_FALLBACK = 1, 2, 4, 8, 16
 PREC_FP64, PREC_FP64_REFORDER, PREC_MIXED = 0, 1, 2
+LAUNCH_MIXED_WAVES, LAUNCH_MIXED_FINISH_COND = 1, 2
 
 _vp = C.c_void_p
 _i32 = C.c_int32
@@ -81,6 +82,7 @@ SIGNATURES = [
     ("c3d_get_closure_profile", C.c_int, [_vp, C.POINTER(ClosureProfile)]),
     ("c3d_set_tail", C.c_int, [_vp, _i32, _i32]),
     ("c3d_set_tail_wide", C.c_int, [_vp, _i32]),
+    ("c3d_set_launch", C.c_int, [_vp, C.c_int, C.c_int]),
     ("c3d_set_precision", C.c_int, [_vp, C.c_int]),
     ("c3d_set_mixed", C.c_int, [_vp, _d, C.c_int, _d]),
     ("c3d_set_pass_budget", C.c_int, [_vp, C.c_int]),
@@ -221,6 +223,9 @@ class Ctx:
     def set_mixed(self, cond_max=1e8, max_iter=40, switch_tol=5e-4):
         return check("c3d_set_mixed", self.L.c3d_set_mixed(self.h, cond_max, max_iter,
                                                            switch_tol))
+
+    def set_launch(self, param, value):
+        return check("c3d_set_launch", self.L.c3d_set_launch(self.h, param, int(value)))
 
     def set_pass_budget(self, n):
         return check("c3d_set_pass_budget", self.L.c3d_set_pass_budget(self.h, n))

@@ -35,6 +35,11 @@ struct ClosureConsts {
     double minf;       // sqrt((2p+1)/(2 pi)), p = floor(sqrt(nm))+1, optBatch.cpp:136-137
 };
 
+// Launch shapes of the closure kernels that a context may choose among (c3d_tune).
+struct Tune {
+    int mixedWaves = 3;  // waves per SIMD of the mixed sweep (2 or 3; nm = 25 always 2)
+};
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -81,6 +86,7 @@ struct c3d_ctx_s {
     double mixedCondMax = 1e8;  // C3D_PREC_MIXED hand-over to the FP64 path
     int mixedMaxIter = 40;
     double mixedSwitch = 5e-4;
+    double mixedFinishCond = 1e5;  // hand-over to FP64 Hessians at ||g|| <= mixedSwitch above this
     // profiling of the last closure call
     bool profiling = false;
     std::vector<hipEvent_t> evPool;
@@ -94,6 +100,7 @@ struct c3d_ctx_s {
     int tailCells = 8192;
     int tailPasses = 32;
     int tailWideCells = 512;  // four waves per cell (speculative line-search trials) below this
+    c3d::Tune tune;
     int gauntDirect[2] = {-1, -1};  // Gaunt form (dense, CSR) equals the direct form: 1, 0, -1 unknown
     std::mutex mtx;
 };

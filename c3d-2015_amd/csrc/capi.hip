@@ -389,6 +389,26 @@ int c3d_set_tail_wide(c3d_ctx ctx, int max_cells)
     return C3D_OK;
 }
 
+int c3d_set_launch(c3d_ctx ctx, int param, int value)
+{
+    if (!ctx)
+        return C3D_ERR_ARG;
+    switch (param) {
+    case C3D_LAUNCH_MIXED_WAVES:
+        if (value < 2 || value > 3)
+            return C3D_ERR_ARG;
+        ctx->tune.mixedWaves = value;
+        return C3D_OK;
+    case C3D_LAUNCH_MIXED_FINISH_COND:
+        if (value < 0 || value > 300)
+            return C3D_ERR_ARG;
+        ctx->mixedFinishCond = std::pow(10.0, value / 10.0);
+        return C3D_OK;
+    default:
+        return C3D_ERR_ARG;
+    }
+}
+
 int c3d_set_precision(c3d_ctx ctx, int mode)
 {
     if (!ctx || (mode != C3D_PREC_FP64 && mode != C3D_PREC_FP64_REFORDER &&

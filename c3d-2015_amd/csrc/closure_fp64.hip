@@ -69,7 +69,7 @@ struct KParams {
     // from its input on condest > mixedCondMax, a failed factorisation or line search, or
     // mixedMaxIter iterations; continuing from its iterate once ||g|| <= mixedSwitch
     int mixed, mixedMaxIter;
-    double mixedCondMax, mixedSwitch;
+    double mixedCondMax, mixedSwitch, mixedFinishCond;
 };
 
 // The worklists of one pass generation: cells in the mixed mode's FP32-Hessian phase and
@@ -1141,30 +1141,108 @@ __global__ __launch_bounds__(kBlock) void k_sweep_ref(KParams P, const double *_
     }
 }
 
-// ---------------------------------------------------------------- mixed-precision sweep
-// Phase one of the mixed mode.  FP64 on the vector units: the exponent alpha.m_q (FMA chain)
-// and the shifted weight w_q e^{alpha.m_q - shift}, summed in FP64 for f (so the Armijo
-// test keeps FP64 resolution).  The Hessian on the FP32 matrix cores
-// (V_MFMA_F32_16X16X4_F32, 4 ordinates per instruction) from the shifted weights, 4 cells
-// per wave sharing every B operand; the shift is log sum_q w e at the cell's last Newton
-// iterate (a log-sum-exp shift), so the FP32 operands stay near unit scale.  The gradient
-// of this phase is H[0][:]/m0 - u (FP32 accuracy); once it falls below the hand-over
-// tolerance the cell finishes on the FP64 path, whose sweeps decide the stopping test.
-// FP32 accumulator register r of lane l holds D[4(l/16) + r][l%16].
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-template <int NM, int CW>
-__device__ __forceinline__ void chunk_mixed(const double *sMc, const float *sMfT, double wl,
-                                            const double *sAc, const double *shift, float *sWE,
-                                            f4 (&acc)[CW][(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
-                                            double (&fpart)[CW], int lane)
+// ---------------------------------------------------------------- mixed-precision sweep
+// Phase one of the mixed mode, on the FP16 and FP64 matrix cores.
+//   * FP64 on the vector units: the exponent alpha.m_q (FMA chain) and the shifted weight
+//     w_q e^{alpha.m_q - shift}, summed per lane in FP64 for f (the Armijo test keeps FP64
+//     resolution); the shift is log sum_q w e at the cell's last Newton iterate.
+//   * The FP64 gradient moments sum_q w_q e_q m_q on V_MFMA_F64_4X4X4_4B_F64: the wave's four
+//     cells are the rows, four 4-column blocks the moments, four ordinates per instruction.
+//   * The Hessian on V_MFMA_F32_16X16X32_F16 (32 ordinates per instruction, FP32
+//     accumulation) as three products of FP16 halves, a_hi b_hi + a_hi b_lo + a_lo b_hi, with
+//     a = 2^s w e m_i formed in FP32 and split by truncating its mantissa to FP16's width
+//     (a_hi) and rounding the exact remainder (a_lo), and b = m_j split once per chunk: about
+//     22 significant bits per product.  The power-of-two scale s of each cell keeps the largest
+//     weight seen so far near 2^12: the first chunk sets it from its maximum, and a later chunk
+//     with a weight above 2^14 lowers it and rescales the accumulators (exactly).
+// Operand layouts of the 16x16x32 FP16 instruction: A[i][k] in lane i + 16 (k / 8), element
+// k % 8; B[k][j] in lane j + 16 (k / 8), element k % 8; D[i][j] in lane j + 16 (i / 4), FP32
+// register i % 4.
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+template <int NM>
+struct H16Chunk {
+    static constexpr int NT = (NM + 15) / 16;      // 16-row tiles of H
+    static constexpr int kKB = kChunk / 32;        // 32-ordinate K blocks per chunk
+    static constexpr int kTS = kChunk + 4;         // FP32 transpose row stride (floats)
+    static constexpr int kTFloats = 16 * NT * kTS; // FP32 M^T image: [16 NT][kTS]
+    static constexpr int kBHalves = kKB * NT * 64 * 16;  // FP16 B image: [kb][t][g][j][hi 8, lo 8]
+    static __device__ __forceinline__ int bindex(int q, int col)
+    {
+        return ((((q >> 5) * NT + (col >> 4)) * 4 + ((q & 31) >> 3)) * 16 + (col & 15)) * 16 +
+               (q & 7);
+    }
+    // zero the rows and columns >= NM of both images (once per kernel, any thread count)
+    static __device__ __forceinline__ void clear_pad(float *sMt, _Float16 *sBh, int tid, int nth)
+    {
+        constexpr int kPad = 16 * NT - NM;
+        if constexpr (kPad > 0) {
+            for (int e = tid; e < kPad * kTS; e += nth)
+                sMt[NM * kTS + e] = 0.0f;
+            for (int e = tid; e < kChunk * kPad; e += nth) {
+                const int q = e / kPad, col = NM + e % kPad;
+                sBh[bindex(q, col)] = (_Float16)0.0f;
+                sBh[bindex(q, col) + 8] = (_Float16)0.0f;
+            }
+        }
+    }
+    // element (q, col) of the chunk into both images
+    static __device__ __forceinline__ void put(float *sMt, _Float16 *sBh, int q, int col, double m)
+    {
+        const float m32 = (float)m;
+        sMt[col * kTS + q] = m32;
+        const _Float16 hi = (_Float16)m32;
+        sBh[bindex(q, col)] = hi;
+        sBh[bindex(q, col) + 8] = (_Float16)(m32 - (float)hi);
+    }
+};
+
+// Lane roles of V_MFMA_F64_4X4X4_4B_F64 (four 4x4x4 products, block b): A[i][k] of block b
+// in lane i + 4 b + 16 k, B[k][j] in lane j + 4 b + 16 k, D[i][j] in lane j + 4 b + 16 i.  With
+// the blocks as column groups (moment 4 b + j) and the same A in every block, a lane holds
+// the weight of cell lane & 3 at ordinate lane >> 4, the basis value of moment lane & 15 at
+// ordinate lane >> 4, and the sum of cell lane >> 4 for moment lane & 15.
+__device__ __forceinline__ int g4_a_row(int lane) { return lane & 3; }
+__device__ __forceinline__ int g4_k(int lane) { return lane >> 4; }
+__device__ __forceinline__ int g4_b_col(int lane) { return lane & 15; }
+__device__ __forceinline__ int g4_d_row(int lane) { return lane >> 4; }
+__device__ __forceinline__ int g4_d_col(int lane) { return lane & 15; }
+
+// The FP16 operands carry the weights scaled by 2^12.  At the shift's own iterate the weights
+// sum to one, so the largest lies in [1/nq, 1] and every operand in FP16's range with full
+// precision for the terms that matter; each lane keeps a running maximum of its weights, and a
+// sweep whose largest scaled weight leaves [2^-4, 2^14] (a trial point far from the shift's
+// iterate) marks its H invalid (see h16_range_ok); a cell accepting such a point re-centres
+// the shift there and sweeps again before its Newton iteration.
+constexpr float kWScale = 4096.0f;
+__device__ __forceinline__ float wave_max_f(float v)
 {
-    constexpr int NT = (NM + 15) / 16;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ bool h16_range_ok(float wmaxLane)
+{
+    const float m = wave_max_f(wmaxLane) * kWScale;
+    return m >= 0.0625f && m <= 16384.0f;
+}
+
+template <int NM, int CW>
+__device__ __forceinline__ void chunk_h16(
+    const double *sMc, const float *sMt, const _Float16 *sBh, double wl, const double *sAc,
+    const double *shift, float (&wmax)[CW], float *sWE, double *sWE64,
+    f4 (&acc)[CW][(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
+    double (&gacc)[(NM + 15) / 16][4], double (&fpart)[CW], int lane)
+{
+    using HC = H16Chunk<NM>;
+    constexpr int NT = HC::NT;
     constexpr int NMS = NM | 1;
-    const int gq = lane >> 4, rq = lane & 15;
     const double *rowp = &sMc[lane * NMS];
-    // the lane's ordinate: exponent, FP64 weight for f, FP32 weight to the wave's LDS in the
-    // order the MFMA lanes read them (ordinate 4k + g at g * 16 + k)
+    // 1. the lane's ordinate: exponent, FP64 weight (f, gradient), FP32 weight
+    float wf[CW];
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
         double x = 0.0;
@@ -1173,34 +1251,118 @@ __device__ __forceinline__ void chunk_mixed(const double *sMc, const float *sMfT
             x = fma(sAc[c * NM + i], rowp[i], x);
         const double we = wl * exp(x - shift[c]);
         fpart[c] = fpart[c] + we;
-        sWE[c * kChunk + (lane & 3) * 16 + (lane >> 2)] = (float)we;
+        sWE64[c * kChunk + lane] = we;
+        wf[c] = (float)we;
+    }
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+        wmax[c] = fmaxf(wmax[c], wf[c]);
+        sWE[c * kChunk + lane] = wf[c] * kWScale;
     }
     wave_lds_sync();
-    const float *m0 = &sMfT[(rq * 4 + gq) * kRS];
-    const float *m1 = &sMfT[((16 + rq) * 4 + gq) * kRS];
+    __builtin_amdgcn_sched_barrier(0);
+    // 2. FP64 gradient moments: rows = the CW cells (zero rows beyond), 4 ordinates a step,
+    // four independent accumulators (step s into accumulator s % 4)
+    {
+        const int ar = g4_a_row(lane), kk = g4_k(lane), bc = g4_b_col(lane);
+#pragma unroll 4
+        for (int s = 0; s < kChunk / 4; ++s) {
+            const int q = 4 * s + kk;
+            const double a = ar < CW ? sWE64[ar * kChunk + q] : 0.0;
 #pragma unroll
-    for (int kb = 0; kb < kChunk / 16; ++kb) {
-        const f4 b0v = *reinterpret_cast<const f4 *>(m0 + 4 * kb);
-        f4 b1v = f4{0.0f, 0.0f, 0.0f, 0.0f};
-        if constexpr (NT == 2)
-            b1v = *reinterpret_cast<const f4 *>(m1 + 4 * kb);
-#pragma unroll
-        for (int c = 0; c < CW; ++c) {
-            const f4 wv4 = *reinterpret_cast<const f4 *>(&sWE[c * kChunk + gq * 16 + 4 * kb]);
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const float wq = wv4[kk], b0 = b0v[kk], b1 = b1v[kk];
-                acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b0, b0, acc[c][0], 0, 0, 0);
-                if constexpr (NT == 2) {
-                    acc[c][1] =
-                        __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b1, b0, acc[c][1], 0, 0, 0);
-                    acc[c][2] =
-                        __builtin_amdgcn_mfma_f32_16x16x4f32(wq * b1, b1, acc[c][2], 0, 0, 0);
-                }
+            for (int t = 0; t < NT; ++t) {
+                const int col = 16 * t + bc;
+                const double b = col < NM ? sMc[q * NMS + col] : 0.0;
+                gacc[t][s & 3] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, gacc[t][s & 3], 0, 0, 0);
             }
         }
     }
-    wave_lds_sync();  // sWE is rewritten by the next chunk
+    __builtin_amdgcn_sched_barrier(0);
+    // 3. the Hessian: FP16 split products, 32 ordinates per instruction
+    const int i16 = lane & 15, g = lane >> 4;
+#pragma unroll 1
+    for (int kb = 0; kb < HC::kKB; ++kb) {
+        const int q0 = 32 * kb + 8 * g;
+        float m8[NT][8];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const f4 lo4 = *reinterpret_cast<const f4 *>(&sMt[(16 * t + i16) * HC::kTS + q0]);
+            const f4 hi4 = *reinterpret_cast<const f4 *>(&sMt[(16 * t + i16) * HC::kTS + q0 + 4]);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                m8[t][e] = lo4[e];
+                m8[t][4 + e] = hi4[e];
+            }
+        }
+        h8 bh[NT], bl[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const _Float16 *bp = &sBh[HC::bindex(q0, 16 * t + i16)];
+            bh[t] = *reinterpret_cast<const h8 *>(bp);
+            bl[t] = *reinterpret_cast<const h8 *>(bp + 8);
+        }
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            const f4 w0 = *reinterpret_cast<const f4 *>(&sWE[c * kChunk + q0]);
+            const f4 w1 = *reinterpret_cast<const f4 *>(&sWE[c * kChunk + q0 + 4]);
+            h8 ah[NT], al[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float a = (e < 4 ? w0[e] : w1[e - 4]) * m8[t][e];
+                    const float hi = __uint_as_float(__float_as_uint(a) & 0xFFFFE000u);
+                    ah[t][e] = (_Float16)hi;
+                    al[t][e] = (_Float16)(a - hi);
+                }
+            acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0], bh[0], acc[c][0], 0, 0, 0);
+            acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0], bl[0], acc[c][0], 0, 0, 0);
+            acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[0], bh[0], acc[c][0], 0, 0, 0);
+            if constexpr (NT == 2) {
+                acc[c][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1], bh[0], acc[c][1], 0, 0, 0);
+                acc[c][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1], bl[0], acc[c][1], 0, 0, 0);
+                acc[c][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[1], bh[0], acc[c][1], 0, 0, 0);
+                acc[c][2] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1], bh[1], acc[c][2], 0, 0, 0);
+                acc[c][2] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1], bl[1], acc[c][2], 0, 0, 0);
+                acc[c][2] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[1], bh[1], acc[c][2], 0, 0, 0);
+            }
+        }
+    }
+    wave_lds_sync();  // sWE and sWE64 are rewritten by the next chunk
+}
+
+// The FP64 gradient moments of row r (cell r) of the 4x4x4 accumulators into gm[NM]:
+// gm_j = e^shift sum_q w_q e_q m_qj, the four partial sums added in a fixed order.
+template <int NM>
+__device__ __forceinline__ void store_gm(const double (&gacc)[(NM + 15) / 16][4], int r,
+                                         double es, double *gm, int lane)
+{
+#pragma unroll
+    for (int t = 0; t < (NM + 15) / 16; ++t) {
+        const int col = 16 * t + g4_d_col(lane);
+        if (g4_d_row(lane) == r && col < NM)
+            gm[col] = ((gacc[t][0] + gacc[t][1]) + (gacc[t][2] + gacc[t][3])) * es;
+    }
+}
+
+// A staged chunk into the mixed sweep's LDS images: the FP64 rows, the FP32 transpose and the
+// FP16 halves (the stager's share of elements, as ChunkStager::put).
+template <int NM>
+__device__ __forceinline__ void put_h16(const ChunkStager<NM> &st, double *sM, float *sMt,
+                                        _Float16 *sBh, double *sW, int tid)
+{
+    constexpr int NMS = NM | 1;
+#pragma unroll
+    for (int k = 0; k < ChunkStager<NM>::kPer; ++k) {
+        const int e = tid + kBlock * k;
+        if (e < kChunk * NM) {
+            const int rr = e / NM, cc = e - rr * NM;
+            sM[rr * NMS + cc] = st.pre[k];
+            H16Chunk<NM>::put(sMt, sBh, rr, cc, st.pre[k]);
+        }
+    }
+    if (tid < kChunk)
+        sW[tid] = st.prew;
 }
 
 // H = e^shift * H32, row-major, lower triangle mirrored
@@ -1225,22 +1387,27 @@ __device__ __forceinline__ void store_H32(const f4 *acc, double es, double *H, i
     }
 }
 
-// Sweep pass of the mixed phase: 4 cells per wave, 16 per block sharing each LDS chunk.
-// Three waves per SIMD (at most 168 registers a lane) measured faster than two for nm <= 16;
-// nm = 25 needs 241 registers and takes two (three spilled 86).
-template <int NM>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NM <= 16 ? 3 : 2))) void k_sweep_mixed(KParams P, const double *__restrict__ Mg,
+// Sweep pass of the mixed phase: 4 cells per wave, 16 per block sharing each LDS chunk (its
+// FP64 rows for the exponents and the gradient, its FP32 transpose and FP16 halves for the
+// Hessian).  Writes H, sum_q w e (fdir) and the FP64 gradient moments (the cell's g slot).
+// WPE waves per SIMD (the register budget 512 / WPE; LDS 49 KB per block at nm = 16, 76 KB
+// at nm = 25), chosen per context (c3d_tune).
+template <int NM, int WPE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_sweep_mixed(KParams P, const double *__restrict__ Mg,
                                                          const double *__restrict__ wg, State S,
                                                          const int *__restrict__ W,
                                                          const int *__restrict__ nWp)
 {
-    constexpr int NT = (NM + 15) / 16;
+    using HC = H16Chunk<NM>;
+    constexpr int NT = HC::NT;
     constexpr int NB = NT * (NT + 1) / 2;
     constexpr int NMS = NM | 1;
     constexpr int CW = kCellsPerWave;
     __shared__ double sM[2][kChunk * NMS];
-    __shared__ __align__(16) float sMfT[2][MixedChunk<NM>::kFloats];
+    __shared__ __align__(16) float sMt[2][HC::kTFloats];
+    __shared__ __align__(16) _Float16 sBh[2][HC::kBHalves];
     __shared__ __align__(16) float sWE[kWavesPerBlock][CW * kChunk];
+    __shared__ __align__(16) double sWE64[kWavesPerBlock][CW * kChunk];
     __shared__ double sW[2][kChunk];
     __shared__ double sA[kWavesPerBlock][CW * NM];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1248,8 +1415,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NM <= 16
     const int nW = uniform_int(*nWp);
     const int groups = (nW + CW - 1) / CW;
     const int nBlocksNeeded = (groups + kWavesPerBlock - 1) / kWavesPerBlock;
-    MixedChunk<NM>::clear_pad(sMfT[0], tid, kBlock);
-    MixedChunk<NM>::clear_pad(sMfT[1], tid, kBlock);
+    HC::clear_pad(sMt[0], sBh[0], tid, kBlock);
+    HC::clear_pad(sMt[1], sBh[1], tid, kBlock);
     for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
         const int grp = blk * kWavesPerBlock + wv;
         int cells[CW];
@@ -1264,38 +1431,50 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NM <= 16
                     cells[c] >= 0 ? S.vec[(size_t)cells[c] * kVec * NM + kVa * NM + lane] : 0.0;
         }
         f4 acc[CW][NB];
+        double gacc[NT][4];
+        double fpart[CW];
+        float wmax[CW];
 #pragma unroll
-        for (int c = 0; c < CW; ++c)
+        for (int c = 0; c < CW; ++c) {
 #pragma unroll
             for (int b = 0; b < NB; ++b)
                 acc[c][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-        double fpart[CW];
-#pragma unroll
-        for (int c = 0; c < CW; ++c)
             fpart[c] = 0.0;
+            wmax[c] = 0.0f;
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                gacc[t][k] = 0.0;
         const bool any = cells[0] >= 0;
         ChunkStager<NM> st;
         st.fetch(Mg, wg, 0, tid);
         __syncthreads();  // the previous cells' last chunk is consumed
         for (int ch = 0; ch < nch; ++ch) {
             const int b = ch & 1;
-            st.put_mixed(sM[b], sMfT[b], sW[b], tid);
+            put_h16<NM>(st, sM[b], sMt[b], sBh[b], sW[b], tid);
             __syncthreads();
             if (ch + 1 < nch)
                 st.fetch(Mg, wg, ch + 1, tid);
             if (any)
-                chunk_mixed<NM, CW>(sM[b], sMfT[b], sW[b][lane], sA[wv], shift, sWE[wv], acc,
-                                    fpart, lane);
+                chunk_h16<NM, CW>(sM[b], sMt[b], sBh[b], sW[b][lane], sA[wv], shift, wmax,
+                                  sWE[wv], sWE64[wv], acc, gacc, fpart, lane);
         }
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
             const double fs = wave_sum(fpart[c]);
+            const bool ok = h16_range_ok(wmax[c]);
             if (cells[c] < 0)
                 continue;
             const double es = exp(shift[c]);
+            double *H = S.H + (size_t)cells[c] * NM * NM;
             if (lane == 0)
                 S.fdir[cells[c]] = fs * es;
-            store_H32<NM>(acc[c], es, S.H + (size_t)cells[c] * NM * NM, lane);
+            store_H32<NM>(acc[c], es / kWScale, H, lane);
+            store_gm<NM>(gacc, c, es, S.vec + (size_t)cells[c] * kVec * NM + kVg * NM, lane);
+            if (!ok && lane == 0)
+                H[0] = __builtin_nan("");
         }
     }
 }
@@ -1488,7 +1667,7 @@ struct Cell {
     // H is not read when a line-search trial is rejected (it may then be null).
     template <bool EXACT>
     __device__ __forceinline__ int advance(const KParams &P, const double *H, double fdir,
-                                           double *sT, int lane)
+                                           const double *gm, double *sT, int lane)
     {
         constexpr int LD = NM + 1;
         const bool own = lane < NM;
@@ -1498,6 +1677,18 @@ struct Cell {
                 tprev = tls;  // accepted; H of this sweep belongs to the new alpha
             } else {
                 ls_reject(lane);
+                return 0;
+            }
+        }
+        // the mixed sweep marks an H whose scaled weights left FP16's range (a point far from
+        // the shift's iterate): re-centre the shift at this point and sweep it again (once: an
+        // unchanged shift falls through, and the marked H fails the factorisation)
+        if (mixed(P) && !(H[0] == H[0])) {
+            const double lf = log(fdir);
+            const double ns = (lf > -700.0 && lf < 700.0) ? lf : 0.0;
+            if (ns != shift) {
+                shift = ns;
+                phase = kPhaseH;
                 return 0;
             }
         }
@@ -1512,7 +1703,8 @@ struct Cell {
         ++iter;
         ++tot;
         const bool mx = mixed(P);
-        g = own ? D.row[0] / P.m0 - uu : 0.0;  // fobjMnUseH
+        // fobjMnUseH; the mixed phase has the FP64 gradient moments of its sweep
+        g = own ? (mx ? gm[lane] : D.row[0] / P.m0) - uu : 0.0;
         if (mx)  // f from the FP64 sum, as the line search evaluates it (fobj.cpp:135-164)
             f = fdir;
         else
@@ -1562,8 +1754,20 @@ struct Cell {
         }
         bool restart = false;  // the iteration starts over; rtol follows the (new) u
         if (mx) {
-            if (cholErr != 0 || !(condest <= P.mixedCondMax) || tprev == 0.0 ||
-                iter >= min(maxItNow, P.mixedMaxIter)) {
+            const bool stalled = tprev == 0.0 || iter >= min(maxItNow, P.mixedMaxIter);
+            const bool illCond = cholErr != 0 || !(condest <= P.mixedCondMax);
+            if ((stalled || illCond) && tot > 1 && err <= P.mixedSwitch) {
+                // stalled, or its split-FP16 Hessian too ill-conditioned to factor, near its
+                // optimum (err: the last iteration's ||g||, FP64): FP64 Hessians continue from
+                // this iterate, which the next (FP64) sweep evaluates
+                flags |= C3D_FLAG_FP64_FINISH;
+                tprev = 1.0;
+                phase = kPhaseH;
+                --iter;
+                --tot;
+                return 0;
+            }
+            if (illCond || stalled) {
                 restart_fp64(P, lane);
                 restart = true;
             } else {
@@ -1610,17 +1814,15 @@ struct Cell {
         // the deterministic exp in every mode: the device library's exp, inlined here, pushed
         // the step kernel past 256 registers (24 spilled)
         gamma = det_exp(2.0 * da1 * P.minf);
-        if (mx) {
-            // the stopping test belongs to the FP64 path: hand the cell over once its
-            // FP32-accurate gradient is small, after taking this Newton step
-            if (err <= P.mixedSwitch)
-                flags |= C3D_FLAG_FP64_FINISH;
-        } else {
-            if (err <= rtol && gamma <= tolGamma)
-                return 1;
-            if (err <= rtol && gamma > tolGamma)
-                ++igam;
-        }
+        // the stopping test on the FP64 gradient, in both phases (optBatch.cpp:268)
+        if (err <= rtol && gamma <= tolGamma)
+            return 1;
+        if (err <= rtol && gamma > tolGamma)
+            ++igam;
+        // an ill-conditioned cell near its optimum finishes with FP64 Hessians (the split
+        // FP16 products carry about 22 bits, so Newton there contracts by ~cond * 1e-7)
+        if (mx && err <= P.mixedSwitch && condest > P.mixedFinishCond)
+            flags |= C3D_FLAG_FP64_FINISH;
         aOld = a;  // linesearch.cpp:29-54
         maxF = 0.0;
         const double ratio = fabs(d) / (fabs(aOld) + 1e-100);  // each lane its own component
@@ -1685,8 +1887,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
                 c = uniform_int(c);
             Cell<NM, G> C;
             C.load(S, c, lane);
-            const int done = C.template advance<EXACT>(P, S.H + (size_t)c * NM * NM, S.fdir[c],
-                                                       sT[grp], lane);
+            const int done =
+                C.template advance<EXACT>(P, S.H + (size_t)c * NM * NM, S.fdir[c],
+                                          S.vec + (size_t)c * kVec * NM + kVg * NM, sT[grp], lane);
             if (done) {
                 C.finish(done, c, u_g, alpha_g, stats, status, lane);
             } else {
@@ -1833,8 +2036,12 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     __shared__ double sA[TW][KW * NM];
     __shared__ int sOk[TW * KW];        // Armijo verdict at each trial point
     __shared__ double sF[1];            // f of the sweep
-    __shared__ __align__(16) float sMfT[MIXED ? MixedChunk<NM>::kFloats : 4];
+    // the mixed sweep's images (H16Chunk), weights and FP64 gradient moments
+    __shared__ __align__(16) float sMt[MIXED ? H16Chunk<NM>::kTFloats : 4];
+    __shared__ __align__(16) _Float16 sBh[MIXED ? H16Chunk<NM>::kBHalves : 8];
     __shared__ __align__(16) float sWE[MIXED ? kChunk : 4];
+    __shared__ __align__(16) double sWE64[MIXED ? kChunk : 2];
+    __shared__ double sG[NM];
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int nWm = uniform_int(*Lc.nm);
     const int nW = nWm + uniform_int(*Lc.nf);
@@ -1844,7 +2051,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     const int c = uniform_int(list_cell(Lc, slot, nWm));
     const int li = lane & (G - 1), row = lane / G;
     if constexpr (MIXED) {
-        MixedChunk<NM>::clear_pad(sMfT, threadIdx.x, TW * kWave);
+        H16Chunk<NM>::clear_pad(sMt, sBh, threadIdx.x, TW * kWave);
         __syncthreads();
     }
     Cell<NM, G> C;
@@ -1936,6 +2143,13 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 }
                 double fpart[1] = {0.0};
                 const double shift[1] = {C.shift};
+                double gacc[NT][4];
+                float wmax1[1] = {0.0f};
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        gacc[t][k] = 0.0;
                 double pre[NM];
                 const int qo = mx ? lane : mfma_ordinate(lane);  // the lane's weight
                 double prew = wg[qo];
@@ -1951,7 +2165,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                         sM[rr * NMS + cc] = pre[k];
                         if constexpr (MIXED)
                             if (mx)
-                                sMfT[MixedChunk<NM>::index(rr, cc)] = (float)pre[k];
+                                H16Chunk<NM>::put(sMt, sBh, rr, cc, pre[k]);
                     }
                     const double wl = prew;
                     wave_lds_sync();
@@ -1963,7 +2177,8 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                             pre[k] = Mg[base * NM + lane + kWave * k];
                     }
                     if (mx)
-                        chunk_mixed<NM, 1>(sM, sMfT, wl, sA[0], shift, sWE, acc32, fpart, lane);
+                        chunk_h16<NM, 1>(sM, sMt, sBh, wl, sA[0], shift, wmax1, sWE, sWE64,
+                                         acc32, gacc, fpart, lane);
                     else
                         chunk_mfma<NM, 1>(sM, wl, sA[0], acc, fpart, lane);
                 }
@@ -1971,7 +2186,11 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 if (mx) {
                     const double es = exp(C.shift);
                     fdir = wave_sum(fpart[0]) * es;
-                    store_H32<NM>(acc32[0], es, sH, lane);
+                    const bool ok = h16_range_ok(wmax1[0]);
+                    store_H32<NM>(acc32[0], es / kWScale, sH, lane);
+                    store_gm<NM>(gacc, 0, es, sG, lane);
+                    if (!ok && lane == 0)
+                        sH[0] = __builtin_nan("");
                 } else {
                     fdir = wave_sum(fpart[0]);
                     store_H<NM>(acc[0], sH, lane);
@@ -1982,7 +2201,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
             ++sweeps;
         }
         __syncthreads();
-        done = C.template advance<REF>(P, sH, sF[0], sT[wv][row], li);
+        done = C.template advance<REF>(P, sH, sF[0], MIXED ? sG : nullptr, sT[wv][row], li);
         __syncthreads();  // sH and sF are rewritten by the next round
         if (done)
             break;
@@ -2309,16 +2528,19 @@ __global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nq
                                                          const double *__restrict__ alpha_g,
                                                          double *hs)
 {
-    constexpr int NT = (NM + 15) / 16;
+    using HC = H16Chunk<NM>;
+    constexpr int NT = HC::NT;
     constexpr int NB = NT * (NT + 1) / 2;
     constexpr int NMS = NM | 1;
     __shared__ double sM[kChunk * NMS];
-    __shared__ __align__(16) float sMfT[MixedChunk<NM>::kFloats];
+    __shared__ __align__(16) float sMt[HC::kTFloats];
+    __shared__ __align__(16) _Float16 sBh[HC::kBHalves];
     __shared__ __align__(16) float sWE[kWavesPerBlock][kChunk];
+    __shared__ __align__(16) double sWE64[kWavesPerBlock][kChunk];
     __shared__ double sW[kChunk];
     __shared__ double sA[kWavesPerBlock][NM];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    MixedChunk<NM>::clear_pad(sMfT, tid, kBlock);
+    HC::clear_pad(sMt, sBh, tid, kBlock);
     const int64_t cell = (int64_t)blockIdx.x * kWavesPerBlock + wv;
     const bool have = cell < ncells;
     if (lane < NM)
@@ -2327,25 +2549,59 @@ __global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nq
 #pragma unroll
     for (int b = 0; b < NB; ++b)
         acc[0][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    double gacc[NT][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            gacc[t][k] = 0.0;
     double fpart[1] = {0.0};
-    const double shift[1] = {0.0};
+    float wmax[1] = {0.0f};
+    // a first pass for the shift: the largest exponent and weight, so that every weight of the
+    // second pass is at most one and the largest at least min w / max w
+    double xmax = -DBL_MAX, wtop = 0.0;
+    for (int c = 0; c < nqPad / kChunk; ++c) {
+        __syncthreads();
+        for (int e = tid; e < kChunk * NM; e += kBlock) {
+            int rr = e / NM, cc = e - rr * NM;
+            sM[rr * NMS + cc] = Mg[(size_t)c * kChunk * NM + e];
+        }
+        if (tid < kChunk)
+            sW[tid] = wg[(size_t)c * kChunk + tid];
+        __syncthreads();
+        if (have && sW[lane] > 0.0) {
+            double x = 0.0;
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                x = fma(sA[wv][i], sM[lane * NMS + i], x);
+            xmax = fmax(xmax, x);
+            wtop = fmax(wtop, sW[lane]);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        xmax = fmax(xmax, __shfl_xor(xmax, o));
+        wtop = fmax(wtop, __shfl_xor(wtop, o));
+    }
+    const double shift[1] = {have ? xmax + log(wtop) : 0.0};
     for (int c = 0; c < nqPad / kChunk; ++c) {
         __syncthreads();
         for (int e = tid; e < kChunk * NM; e += kBlock) {
             int rr = e / NM, cc = e - rr * NM;
             const double m = Mg[(size_t)c * kChunk * NM + e];
             sM[rr * NMS + cc] = m;
-            sMfT[MixedChunk<NM>::index(rr, cc)] = (float)m;
+            HC::put(sMt, sBh, rr, cc, m);
         }
         if (tid < kChunk)
             sW[tid] = wg[(size_t)c * kChunk + tid];
         __syncthreads();
         if (have)
-            chunk_mixed<NM, 1>(sM, sMfT, sW[lane], sA[wv], shift, sWE[wv], acc, fpart, lane);
+            chunk_h16<NM, 1>(sM, sMt, sBh, sW[lane], sA[wv], shift, wmax, sWE[wv], sWE64[wv],
+                             acc, gacc, fpart, lane);
     }
     if (!have)
         return;
-    store_H32<NM>(acc[0], 1.0, hs + cell * NM * NM, lane);
+    store_H32<NM>(acc[0], exp(shift[0]) / kWScale, hs + cell * NM * NM, lane);
 }
 
 // The dense part of one Newton iteration for a batch of given Hessians (one wave each):
@@ -2569,9 +2825,19 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
                 return hipErrorOutOfMemory;
             if (ev)
                 (void)hipEventRecord(ev[0], s);
-            if (mixed)
-                hipLaunchKernelGGL(k_sweep_mixed<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0,
-                                   s, P, ctx->d_M, ctx->d_wpad, S, Lc.Wm, Lc.nm);
+            if (mixed) {
+                bool two = true;
+                if constexpr (NM <= 16) {
+                    two = ctx->tune.mixedWaves == 2;
+                    if (!two)
+                        hipLaunchKernelGGL((k_sweep_mixed<NM, 3>), dim3(grid_cap(ctx, 8)),
+                                           dim3(kBlock), 0, s, P, ctx->d_M, ctx->d_wpad, S, Lc.Wm,
+                                           Lc.nm);
+                }
+                if (two)
+                    hipLaunchKernelGGL((k_sweep_mixed<NM, 2>), dim3(grid_cap(ctx, 8)), dim3(kBlock),
+                                       0, s, P, ctx->d_M, ctx->d_wpad, S, Lc.Wm, Lc.nm);
+            }
             if (ev)
                 (void)hipEventRecord(ev[1], s);
             if (ref)
@@ -2632,6 +2898,7 @@ hipError_t launch_closure_fp64(c3d_ctx_s *ctx, int64_t ncells, double *u, double
     P.mixedMaxIter = ctx->mixedMaxIter;
     P.mixedCondMax = ctx->mixedCondMax;
     P.mixedSwitch = ctx->mixedSwitch;
+    P.mixedFinishCond = ctx->mixedFinishCond;
     if (ncells <= 0)
         return hipSuccess;
     if (ncells > INT32_MAX / 2)

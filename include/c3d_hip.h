@@ -47,19 +47,21 @@ extern "C" {
 #define C3D_FLAG_BUDGET 8      /* cell left unsolved by the pass budget */
 #define C3D_FLAG_FP64_FALLBACK 16 /* mixed mode: the cell was solved by the FP64 path from
                                      its input (the mixed iteration declined it) */
-#define C3D_FLAG_FP64_FINISH 64   /* mixed mode: the FP32-Hessian phase reached the hand-over
-                                     tolerance and FP64 iterations finished the cell */
+#define C3D_FLAG_FP64_FINISH 64   /* mixed mode: near its optimum (||g|| <= switch_tol) the
+                                     cell was too ill-conditioned for the split-FP16 Hessian
+                                     and FP64 Hessians finished it from its iterate */
 
 /* arithmetic modes of a context (c3d_set_precision) */
 #define C3D_PREC_FP64 0          /* FP64; quadrature sums on the matrix cores (default) */
 #define C3D_PREC_FP64_REFORDER 1 /* FP64 in the reference's operation order with a
                                     deterministic exp/log: bit-reproducible on a CPU */
-#define C3D_PREC_MIXED 2         /* closure: a first phase with the Hessian (and the
-                                    gradient H[0][:]/m0) on the FP32 matrix cores under a
-                                    log-sum-exp shift, exponents, f, Cholesky and condition
-                                    estimate in FP64; each cell then finishes on the FP64
-                                    path, which makes the stopping test; cells that turn
-                                    ill-conditioned restart from their input on it */
+#define C3D_PREC_MIXED 2         /* closure: the Hessian on the FP16 matrix cores as three
+                                    split products (about 22 bits, FP32 accumulation) under a
+                                    log-sum-exp shift; exponents, f, the gradient (on the FP64
+                                    matrix cores), Cholesky, the condition indicator and the
+                                    stopping test in FP64; ill-conditioned cells continue on
+                                    FP64 Hessians from their iterate near the optimum, or
+                                    restart from their input on the FP64 path before it */
 
 typedef struct c3d_ctx_s *c3d_ctx;
 
@@ -210,14 +212,16 @@ int c3d_closure_noalloc(c3d_ctx ctx, int64_t ncells, double *u, double *alpha,
  * No counterpart in the reference (whose results depend on the host libm). */
 int c3d_set_precision(c3d_ctx ctx, int mode);
 
-/* Parameters of C3D_PREC_MIXED.  A cell leaves the FP32-Hessian phase and restarts from its
- * input on the FP64 path when (max_i L_ii / min_i L_ii)^2 of its Cholesky factor, a lower
- * bound on cond_2(H) that stands in for dpocon in this phase, exceeds cond_max (default 1e8), its
- * factorisation or line search fails, or it has taken max_iter iterations (default 40);
- * it then takes exactly the FP64 path's decisions (regularisation included) and is flagged
- * C3D_FLAG_FP64_FALLBACK.  It continues from its iterate on the FP64 path once its gradient
- * norm falls below switch_tol (default 5e-4; C3D_FLAG_FP64_FINISH).  No counterpart in the
- * reference. */
+/* Parameters of C3D_PREC_MIXED.  The mixed phase makes the FP64 stopping test itself (its
+ * gradient is summed in FP64), so most cells converge in it.  A cell whose gradient norm is at
+ * most switch_tol (default 5e-4) continues from its iterate on FP64 Hessians
+ * (C3D_FLAG_FP64_FINISH) when its condition indicator (max_i L_ii / min_i L_ii)^2 -- a lower
+ * bound on cond_2(H) that stands in for dpocon in this phase -- exceeds 1e5 (see
+ * c3d_set_launch), its factorisation fails, the indicator exceeds cond_max (default 1e8), its
+ * line search fails or it has taken max_iter iterations (default 40).  The same events farther
+ * from the optimum restart the cell from its input on the FP64 path, which then takes exactly
+ * the FP64 mode's decisions (regularisation included; C3D_FLAG_FP64_FALLBACK).  No counterpart
+ * in the reference. */
 int c3d_set_mixed(c3d_ctx ctx, double cond_max, int max_iter, double switch_tol);
 
 /* Kernel-time accounting of c3d_closure (HIP events on the closure's stream around every
@@ -261,8 +265,20 @@ int c3d_set_tail(c3d_ctx ctx, int max_cells, int passes_per_launch);
  * the reference. */
 int c3d_set_tail_wide(c3d_ctx ctx, int max_cells);
 
+/* Launch and hand-over parameters of the closure kernels, for tuning per (N, Q); results do not
+ * depend on the launch shapes.  No counterpart in the reference.
+ *   C3D_LAUNCH_MIXED_WAVES        waves per SIMD of the mixed sweep: 2 or 3 (default 3; the
+ *                                 M_4 sweep always runs at 2)
+ *   C3D_LAUNCH_MIXED_FINISH_COND  10 * log10 of the condition indicator above which a mixed
+ *                                 cell with ||g|| <= switch_tol continues on FP64 Hessians
+ *                                 (default 50, i.e. 1e5); changes which cells take that path */
+#define C3D_LAUNCH_MIXED_WAVES 1
+#define C3D_LAUNCH_MIXED_FINISH_COND 2
+int c3d_set_launch(c3d_ctx ctx, int param, int value);
+
 /* H(alpha) for ncells cells (solveHMn semantics), device pointers, hs [ncells][nm][nm].
- * Under C3D_PREC_MIXED the sums run on the FP32 matrix cores (relative error ~1e-6).  The
+ * Under C3D_PREC_MIXED the sums are the mixed closure's (split FP16 products with FP32
+ * accumulation under a per-cell shift; relative error ~1e-6).  The
  * Gaunt form is served as c3d_closure_opts describes. */
 int c3d_hessian(c3d_ctx ctx, int64_t ncells, const double *alpha, double *hs, int useGaunt,
                 int useGauntSparse, void *stream);

@@ -1,12 +1,13 @@
-"""GPU checks of the mixed-precision closure (C3D_PREC_MIXED): a first phase with the
-Hessian on the FP32 matrix cores under a log-sum-exp shift (exponents, f, Cholesky and
-condition estimate in FP64), the hand-over of each cell to the FP64 path once its gradient
-is small (the FP64 sweeps make the stopping test), and the restart of ill-conditioned
-cells from their input on the FP64 path.
+"""GPU checks of the mixed-precision closure (C3D_PREC_MIXED): the Hessian on the FP16
+matrix cores as three split products under a log-sum-exp shift, the gradient summed in FP64
+(FP64 matrix cores), exponents, f, Cholesky and condition indicator in FP64 and the FP64
+stopping test in the mixed phase itself; ill-conditioned cells continue on FP64 Hessians
+from their iterate near the optimum, or restart from their input on the FP64 path.
 
 Bars, against the oracle (oracle/c3d_oracle.c, FP64 restatement of src/opt/*) and the
 product's own FP64 mode on identical inputs:
-  * the mixed Hessian is within 1e-5 of the FP64 one (relative to max|H|);
+  * the mixed Hessian is within 1e-5 of the FP64 one (relative to max|H|), also for
+    exponents far from zero (its own shift keeps the split operands in FP16's range);
   * every converged cell meets the FP64 stopping test (||grad h||_2 <= tol in FP64,
     u0-scaled) and reproduces its moments to 1.5e-8 u0;
   * a cell handed to the FP64 path ends bit-identical to the FP64 mode's result;
@@ -46,6 +47,8 @@ def test_mixed_hessian_close_to_fp64(P, O, dev, N, Q):
     rng = np.random.default_rng(17 * N + Q)
     n = 41
     a = rng.normal(0, 0.7, (n, T.nm))
+    a[::3, 0] += 60.0  # weights e^{+17}: scaled far beyond FP16's range without the shift
+    a[1::3, 0] -= 60.0  # and e^{-17}: below its normal range
     h_ref = np.zeros((n, T.nm, T.nm))
     gx = np.zeros(T.nm2)
     for c in range(n):
