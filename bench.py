@@ -8,12 +8,14 @@ gamma <= 1.1 (optBatch.cpp:268), MN_MAX_ITER 100, MN_COND_H_MAX 1e10.  The input
 generated on the device once and stay in HBM; a step restores u and alpha from pristine
 device copies and solves the whole batch.
 
-Paths (--path): "fp64" plain FP64 (matrix-core Hessian sums); "mixed" FP32 matrix-core
-Hessian under a log-sum-exp shift with FP64 gradient/f/Cholesky/stopping test and FP64
-hand-over of ill-conditioned cells; "+nn" adds the learned warm start, trained first (not
-timed) on --train-pairs (u, alpha) pairs that the FP64 closure solves from a different seed.
-The plain FP64 path is always timed beside the chosen one (same inputs, same process), and
-the chosen path's outputs are compared with its outputs cell by cell.
+Paths (--path): "fp64" plain FP64 (matrix-core Hessian sums); "mixed" the Hessian as split
+FP16 matrix-core products under a log-sum-exp shift with the gradient, f, Cholesky and the
+stopping test in FP64 (ill-conditioned cells continue on FP64 Hessians); "+nn" adds the
+learned warm start, trained first (not timed) on --train-pairs (u, alpha) pairs that the FP64
+closure solves from a different seed.  The plain FP64 path and the reference-order FP64 path
+(bit-identical to the oracle) are timed beside the chosen one (same inputs, same process);
+every path's outputs are compared with the FP64 path's, and with the oracle's on the CPU
+sample.  Launch shapes come from c3d-2015_amd/tuning.json (tune.py) unless --no-tuning.
 
 One JSON line: value = cells solved per second by the chosen path; roofline of the path's
 matrix-core sweep kernel (HIP events on the closure's stream, cell-sweeps counted on the
@@ -37,6 +39,7 @@ METRIC = ("M_N cell-updates/s at ‖∇h‖≤1e-8: batched mixed+NN vs plain FP
           "one GPU")
 FP64_MFMA_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix
 FP32_MFMA_PEAK = 157.3  # TFLOP/s, MI355X dense FP32 matrix
+FP16_MFMA_PEAK = 2500.0  # TFLOP/s, MI355X dense FP16 matrix
 
 
 def front_end():
@@ -48,17 +51,74 @@ def front_end():
     return mod
 
 
-def cpu_baseline(u_host, a_host, N, Q, nthreads):
-    """The oracle (plain FP64 C restatement, OpenMP over cells) on a sample of the inputs."""
+def host_cpu_info():
+    """CPU model and physical core count of this host (from /proc/cpuinfo), and the threads
+    the oracle may use (OMP_NUM_THREADS, else the logical CPUs)."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None and core is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if phys is not None and core is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    logical = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or logical
+    return {"cpu_model": model, "physical_cores": len(cores) or None,
+            "logical_cpus": logical, "threads": threads}
+
+
+def cpu_baseline(u_host, a_host, N, Q, nthreads, n_one=4096, reps=3):
+    """The oracle (plain FP64 C restatement of src/opt, OpenMP schedule(dynamic) over cells)
+    on a sample of the inputs: one warm-up, then the median of `reps` timed runs on all
+    `nthreads` threads, and one run on a single thread over the first `n_one` cells.
+    Returns (rate, seconds, rate_1thread, u_out, a_out, stats)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     T = O.Tables(N, Q)
     n = u_host.shape[0]
-    O.closure(T, u_host[:64].copy(), a_host[:64].copy(), O.opts(), nthreads)
+    O.closure(T, u_host[:256].copy(), a_host[:256].copy(), O.opts(), nthreads)
+    times = []
+    for _ in range(reps):
+        uo, ao = u_host.copy(), a_host.copy()
+        t0 = time.perf_counter()
+        _, st = O.closure(T, uo, ao, O.opts(), nthreads)
+        times.append(time.perf_counter() - t0)
+    dt = sorted(times)[len(times) // 2]
+    m = min(n_one, n)
     t0 = time.perf_counter()
-    O.closure(T, u_host.copy(), a_host.copy(), O.opts(), nthreads)
-    dt = time.perf_counter() - t0
-    return n / dt, dt
+    O.closure(T, u_host[:m].copy(), a_host[:m].copy(), O.opts(), 1)
+    dt1 = time.perf_counter() - t0
+    return n / dt, dt, m / dt1, uo, ao, st
+
+
+def oracle_agreement(T, u_in, u_gpu, a_gpu, reg_gpu, flags_gpu, u_orc, a_orc, st_orc):
+    """A GPU path's cells against the oracle's results on the same inputs: the share whose
+    written-back moments agree to 1e-6 u0 and whose regularisation rung agrees; alpha's relative
+    difference on cells of equal rung; and the FP64 stopping test of the GPU result, recomputed
+    here by numpy on its unflagged cells (||<m e^{alpha.m}> - u||_2 / u0)."""
+    du = np.abs(u_gpu - u_orc).max(axis=1) / np.abs(u_in[:, 0])
+    same = reg_gpu == st_orc["regIndex"]
+    da = np.abs(a_gpu - a_orc).max(axis=1) / np.abs(a_orc).max(axis=1)
+    ok = (flags_gpu & (2 | 4 | 8)) == 0  # no u0 <= 0, ladder or budget flag
+    g = ((np.exp(a_gpu[ok] @ T.M.T) * T.w) @ T.M - u_gpu[ok]) / u_gpu[ok, :1]
+    return {"cells": int(len(u_in)),
+            "moments_within_1e-6": round(float((du <= 1e-6).mean()), 6),
+            "rung_agreement": round(float(same.mean()), 6),
+            "alpha_max_rel_same_rung": float(da[same].max()) if same.any() else None,
+            "alpha_median_rel_same_rung": float(np.median(da[same])) if same.any() else None,
+            "stop_test_max_grad_norm": float(np.linalg.norm(g, axis=1).max()) if ok.any() else None}
 
 
 def log(msg):
@@ -77,6 +137,10 @@ def main():
                     default="mixed+nn")
     ap.add_argument("--mixed-cond", type=float, default=1e8)
     ap.add_argument("--mixed-iter", type=int, default=40)
+    ap.add_argument("--reforder-steps", type=int, default=2,
+                    help="timed solves of the reference-order FP64 path (0: skip it)")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="ignore c3d-2015_amd/tuning.json (the library's default launch shapes)")
     ap.add_argument("--mixed-waves", type=int, choices=(2, 3), default=None,
                     help="waves per SIMD of the mixed sweep (default: the library's)")
     ap.add_argument("--finish-cond", type=int, default=None,
@@ -156,9 +220,17 @@ def main():
         del ut, at
         torch.cuda.empty_cache()
 
+    tuning = {}
+
     def measure(path, steps, warmup):
-        ctx.set_precision(P.PREC_MIXED if path.startswith("mixed") else P.PREC_FP64)
+        prec = {"mixed": P.PREC_MIXED, "fp64": P.PREC_FP64,
+                "fp64_reforder": P.PREC_FP64_REFORDER}[path.split("+")[0]]
+        ctx.set_precision(prec)
         ctx.set_warmstart(path.endswith("+nn"))
+        if not args.no_tuning:
+            tuning[path] = ctx.apply_tuning("mixed" if path.startswith("mixed") else "fp64")
+        if args.mixed_waves is not None:  # explicit values win over the table
+            ctx.set_launch(P.LAUNCH_MIXED_WAVES, args.mixed_waves)
 
         def step():
             u.copy_(u0)
@@ -188,12 +260,16 @@ def main():
                "reg_hist": np.bincount(st["regIndex"], minlength=5).tolist(),
                "fallback": float(((st["flags"] & P.FLAG_FP64_FALLBACK) != 0).mean()),
                "warm_taken": float(((st["flags"] & P.FLAG_WARM) != 0).mean()),
-               "u": u.cpu().numpy(), "a": a.cpu().numpy(), "reg": st["regIndex"].copy()}
+               "u": u.cpu().numpy(), "a": a.cpu().numpy(), "reg": st["regIndex"].copy(),
+               "flags": st["flags"].copy()}
         log(f"{path}: {out['ms_per_step']:.1f} ms/step iters {out['iters_mean']:.2f}")
         return out
 
     main_r = measure(args.path, args.steps, args.warmup)
     base_r = main_r if args.path == "fp64" else measure("fp64", args.steps, args.warmup)
+    # the reference-order FP64 mode: the one bit-identical to the oracle (deterministic exp)
+    ref_r = measure("fp64_reforder", max(1, args.reforder_steps), 0) \
+        if args.reforder_steps > 0 else None
     value = n / (main_r["ms_per_step"] / 1e3)
     fp64_value = n / (base_r["ms_per_step"] / 1e3)
 
@@ -202,22 +278,36 @@ def main():
     agree = {"moments_within_1e-6": round(float((du <= 1e-6).mean()), 6),
              "rung_agreement": round(float((main_r["reg"] == base_r["reg"]).mean()), 6)}
 
-    # roofline of the path's matrix-core sweep kernel: H = M^T diag(w e) M, 2 nq nm^2 flops
-    # per cell-sweep; the mixed phase runs it on the FP32 matrix cores, the FP64 path (and
-    # the FP64 finish of the mixed cells) on the FP64 ones
+    # roofline of the path's matrix-core sweep kernel.  Algorithmic work: the Hessian
+    # H = M^T diag(w e) M, 2 nq nm^2 flops per cell-sweep.  The mixed sweep computes it to FP32
+    # accuracy (about 22-bit split products, FP32 accumulation) as three FP16 matrix-core
+    # products, with the gradient on the FP64 matrix cores; its line is against the FP32 matrix
+    # peak (the accuracy class delivered), with the executed FP16 products against the FP16
+    # peak beside it.  The FP64 path's sweep runs on the FP64 matrix cores.
     flops_per_sweep = 2.0 * nq * nm * nm
+    mixed_waves = (tuning.get(args.path) or {}).get("mixed_waves", 3) if args.mixed_waves is None \
+        else args.mixed_waves
 
     def sweep_roofline(prof, mixed_kernel):
         cs = prof["mixed_cell_sweeps"] if mixed_kernel else prof["cell_sweeps"]
         ms = prof["mixed_sweep_ms"] if mixed_kernel else prof["sweep_ms"]
         peak = FP32_MFMA_PEAK if mixed_kernel else FP64_MFMA_PEAK
         ach = flops_per_sweep * cs / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
-        return {"bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(ach / peak, 4), "traffic": None,
-                "kernel": ("k_sweep_mixed<16> (FP32 MFMA Hessian, mixed phase)" if mixed_kernel
-                           else "k_sweep<16> (FP64 MFMA Hessian)"),
-                "flops_per_cell_sweep": flops_per_sweep, "cell_sweeps_per_step": int(cs),
-                "kernel_ms_per_step": round(ms, 3)}
+        r = {"bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
+             "frac": round(ach / peak, 4), "traffic": None,
+             "kernel": (f"k_sweep_mixed<{nm}, {mixed_waves}> (Hessian as split FP16 matrix-core "
+                        f"products, FP64 matrix-core gradient, mixed phase)" if mixed_kernel
+                        else f"k_sweep<{nm}> (FP64 MFMA Hessian)"),
+             "flops_per_cell_sweep": flops_per_sweep, "cell_sweeps_per_step": int(cs),
+             "kernel_ms_per_step": round(ms, 3)}
+        if mixed_kernel:
+            ex = 3.0 * ach
+            r["fp16_matrix_executed"] = {"achieved": round(ex, 3), "peak": FP16_MFMA_PEAK,
+                                         "frac": round(ex / FP16_MFMA_PEAK, 4),
+                                         "note": "three FP16 products per Hessian entry; the "
+                                                 "kernel is bound by its FP64 vector work "
+                                                 "(exponent, exp) and the operand split"}
+        return r
 
     # HBM bytes per dispatch of the sweep kernels, from the committed rocprofv3 PMC summary
     # (FETCH_SIZE / WRITE_SIZE runs calibrated on a copy of known size)
@@ -228,7 +318,7 @@ def main():
             traffic[k] = round(v["fetch_bytes_per_dispatch"] + v["write_bytes_per_dispatch"])
 
     def with_traffic(r):
-        key = r["kernel"].split(" ")[0]
+        key = r["kernel"].split(" (")[0]
         r["traffic"] = traffic.get(key)
         if r["traffic"] is not None:
             r["traffic_source"] = "profiles/closure_C1_pmc_traffic.md (avg bytes per dispatch)"
@@ -247,25 +337,55 @@ def main():
         out.update(passes=int(p["passes"]), tail_launches=int(p["tail_launches"]))
         return out
 
-    cpu = None
+    cpu, oracle_cmp = None, None
     if args.cpu_sample > 0:
         ns = min(args.cpu_sample, n)
         uh = u0[:ns].cpu().numpy().copy()
         ah = a0[:ns].cpu().numpy().copy()
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-        cps, secs = cpu_baseline(uh, ah, N, Q, threads)
+        info = host_cpu_info()
+        threads = info["threads"]
+        cps, secs, cps1, uo, ao, so = cpu_baseline(uh, ah, N, Q, threads)
         cpu = {"value": round(cps, 1), "unit": "cell-updates/s", "cores": threads,
                "kind": "port",
-               "sample": f"first {ns} cells of the same batch, oracle (C, -O3, OpenMP "
-                         f"dynamic over cells, plain FP64), {secs:.1f} s"}
+               "sample": f"first {ns} cells of the same batch: oracle (C, -O3, OpenMP "
+                         f"schedule(dynamic) over cells, plain FP64, the reference's libm exp) "
+                         f"on {threads} threads, median of 3 timed runs after a warm-up: "
+                         f"{secs:.2f} s; one thread on the first {min(4096, ns)} cells",
+               "value_1thread": round(cps1, 1), **info}
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        T = O.Tables(N, Q)
+        oracle_cmp = {}
+        for name, r in (("mixed+nn" if args.path == "mixed+nn" else args.path, main_r),
+                        ("fp64", base_r)) + ((("fp64_reforder", ref_r),) if ref_r else ()):
+            oracle_cmp[name] = oracle_agreement(T, uh, r["u"][:ns], r["a"][:ns], r["reg"][:ns],
+                                                r["flags"][:ns], uo, ao, so)
+        if ref_r is not None:
+            # bit for bit: the oracle with the same deterministic exp/log as the device, on the
+            # library's own quadrature and basis tables (the two table builds agree to ~1 ulp,
+            # not bit for bit, and the closure's bits follow its tables)
+            nb = min(4096, ns)
+            ub, ab = uh[:nb].copy(), ah[:nb].copy()
+            Tl = O.Tables(N, Q)
+            lt = P.tables_host(N, Q)
+            for k in ("w", "xi", "eta", "mu", "M", "M2"):
+                getattr(Tl, k)[...] = lt[k]  # in place: the oracle's table struct points here
+            with O.exp_mode(1):
+                _, sb = O.closure(Tl, ub, ab, O.opts(), threads)
+            same = (np.all(ref_r["u"][:nb] == ub, axis=1) & np.all(ref_r["a"][:nb] == ab, axis=1)
+                    & (ref_r["reg"][:nb] == sb["regIndex"]))
+            oracle_cmp["fp64_reforder"]["bitwise_cells"] = nb
+            oracle_cmp["fp64_reforder"]["bitwise_identical_frac"] = round(float(same.mean()), 6)
+        log(f"oracle agreement: {oracle_cmp}")
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(main_r["ms_per_step"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        # the mixed phase sums the Hessian on the FP32 matrix cores; the rest is FP64
-        "dtype": "f32+f64" if args.path.startswith("mixed") else "f64",
+        # the mixed phase: the Hessian as split FP16 products with FP32 accumulation; the rest
+        # (exponents, f, gradient, Cholesky, stopping test) FP64
+        "dtype": "f16+f32+f64" if args.path.startswith("mixed") else "f64",
         "data": "synthetic",
         "config": {"workload": "closure-only batch: 1048576 independent M_3 dual problems, "
                                "24x48=1152 ordinates, alpha*~N(0,s^2), s~U[0.5,8], "
@@ -285,7 +405,16 @@ def main():
                                  "regularisation_hist": base_r["reg_hist"],
                                  "kernel_ms_per_step": kernel_ms(base_r)},
                    "speedup_vs_fp64": round(value / fp64_value, 3),
+                   "fp64_reforder_path": None if ref_r is None else {
+                       "value": round(n / (ref_r["ms_per_step"] / 1e3), 1),
+                       "ms_per_step": round(ref_r["ms_per_step"], 3),
+                       "iters_per_cell_mean": round(ref_r["iters_mean"], 3),
+                       "regularisation_hist": ref_r["reg_hist"],
+                       "speedup_of_main_path": round(ref_r["ms_per_step"]
+                                                     / main_r["ms_per_step"], 3)},
                    "agreement_with_fp64": agree,
+                   "agreement_with_oracle": oracle_cmp,
+                   "tuning": tuning or None,
                    "warm_start_training": train},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -167,6 +167,19 @@ def check(fn, rc):
     return rc
 
 
+TUNING_TABLE = os.path.join(HERE, "tuning.json")
+
+
+def tuning_for(N, Q, precision, path=None):
+    """The tuned launch shapes of the closure for (N, Q, "fp64" | "mixed") from the table that
+    tune.py writes (None if the table or the entry is absent)."""
+    import json
+    path = path or TUNING_TABLE
+    if not os.path.exists(path):
+        return None
+    return json.load(open(path)).get(f"{N},{Q},{precision}")
+
+
 class Ctx:
     """A c3d context (tables resident on the current device)."""
 
@@ -223,6 +236,20 @@ class Ctx:
     def set_mixed(self, cond_max=1e8, max_iter=40, switch_tol=5e-4):
         return check("c3d_set_mixed", self.L.c3d_set_mixed(self.h, cond_max, max_iter,
                                                            switch_tol))
+
+    def apply_tuning(self, precision, path=None):
+        """Set the closure's launch shapes from the tuning table for this context's (N, Q) and
+        `precision` ("fp64" | "mixed"); returns the entry applied, or None."""
+        import math
+        N = int(round(math.sqrt(self.nm))) - 1
+        Q = int(round(math.sqrt(self.nq / 2)))
+        t = tuning_for(N, Q, precision, path)
+        if t is None:
+            return None
+        self.set_tail(t["tail_cells"], t["tail_passes"], wide_cells=t["tail_wide"])
+        if "mixed_waves" in t:
+            self.set_launch(LAUNCH_MIXED_WAVES, t["mixed_waves"])
+        return t
 
     def set_launch(self, param, value):
         return check("c3d_set_launch", self.L.c3d_set_launch(self.h, param, int(value)))

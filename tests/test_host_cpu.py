@@ -57,3 +57,29 @@ def test_deck_reader_rules(tmp_path):
     assert _value(p, "KEY_ONLY") == "1.0e-4"
     assert _value(p, "NOT_THERE") is None
     assert _value(tmp_path / "missing.deck", "SOLVER") is None
+
+
+def test_tuning_table_lookup_and_candidates(tmp_path):
+    """The tuner's candidate grid keeps the four-wave threshold at or below the tail switch
+    and covers the mixed sweep's wave counts; the front end reads entries by (N, Q,
+    precision) and reports absent ones as None."""
+    import importlib.util
+    import json
+    import os
+    from conftest import ROOT, product
+    spec = importlib.util.spec_from_file_location(
+        "c3d_tune", os.path.join(ROOT, "c3d-2015_amd", "tune.py"))
+    tune = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(tune)
+    mixed = list(tune.candidates("mixed"))
+    fp64 = list(tune.candidates("fp64"))
+    assert all(c["tail_wide"] <= c["tail_cells"] for c in mixed + fp64)
+    assert {c["mixed_waves"] for c in mixed} == {2, 3}
+    assert all("mixed_waves" not in c for c in fp64)
+    P = product()
+    path = tmp_path / "t.json"
+    path.write_text(json.dumps({"3,24,mixed": {"tail_cells": 8192, "tail_passes": 32,
+                                               "tail_wide": 512, "mixed_waves": 3}}))
+    assert P.tuning_for(3, 24, "mixed", str(path))["mixed_waves"] == 3
+    assert P.tuning_for(3, 24, "fp64", str(path)) is None
+    assert P.tuning_for(3, 24, "mixed", str(tmp_path / "absent.json")) is None
