@@ -880,21 +880,6 @@ struct ChunkStager {
         if (tid < kChunk)
             sW[tid] = prew;
     }
-    // the same, plus the FP32 operand copy of the chunk for the mixed sweep
-    __device__ __forceinline__ void put_mixed(double *sM, float *sMfT, double *sW, int tid) const
-    {
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int e = tid + kBlock * k;
-            if (e < kChunk * NM) {
-                const int rr = e / NM, cc = e - rr * NM;
-                sM[rr * NMS + cc] = pre[k];
-                sMfT[MixedChunk<NM>::index(rr, cc)] = (float)pre[k];
-            }
-        }
-        if (tid < kChunk)
-            sW[tid] = prew;
-    }
 };
 
 // The chunk ordinate whose exponent and weight lane l computes in the FP64 sweep.
@@ -1152,50 +1137,66 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 //     cells are the rows, four 4-column blocks the moments, four ordinates per instruction.
 //   * The Hessian on V_MFMA_F32_16X16X32_F16 (32 ordinates per instruction, FP32
 //     accumulation) as three products of FP16 halves, a_hi b_hi + a_hi b_lo + a_lo b_hi, with
-//     a = 2^s w e m_i formed in FP32 and split by truncating its mantissa to FP16's width
-//     (a_hi) and rounding the exact remainder (a_lo), and b = m_j split once per chunk: about
-//     22 significant bits per product.  The power-of-two scale s of each cell keeps the largest
-//     weight seen so far near 2^12: the first chunk sets it from its maximum, and a later chunk
-//     with a weight above 2^14 lowers it and rescales the accumulators (exactly).
+//     b = m_j split once per chunk into b_hi + b_lo (round to nearest, then the rounded
+//     remainder) and a = v m_i formed from the halves of v = 2^12 w e (split once per ordinate
+//     and cell) in packed FP16: a_hi = v_hi b_hi rounded, and a_lo the exact rounding error of
+//     that product (a fused multiply-add) plus v_hi b_lo + v_lo b_hi.  About 21 significant
+//     bits per product; the fixed scale 2^12 and the range check below keep the operands in
+//     FP16's range.
 // Operand layouts of the 16x16x32 FP16 instruction: A[i][k] in lane i + 16 (k / 8), element
 // k % 8; B[k][j] in lane j + 16 (k / 8), element k % 8; D[i][j] in lane j + 16 (i / 4), FP32
 // register i % 4.
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+// Row stride (doubles) of the FP64 weights a wave keeps per cell for the gradient products:
+// the four cells' rows start in different LDS banks.
+constexpr int kWE64S = kChunk + 4;
 
 template <int NM>
 struct H16Chunk {
     static constexpr int NT = (NM + 15) / 16;      // 16-row tiles of H
     static constexpr int kKB = kChunk / 32;        // 32-ordinate K blocks per chunk
-    static constexpr int kTS = kChunk + 4;         // FP32 transpose row stride (floats)
-    static constexpr int kTFloats = 16 * NT * kTS; // FP32 M^T image: [16 NT][kTS]
-    static constexpr int kBHalves = kKB * NT * 64 * 16;  // FP16 B image: [kb][t][g][j][hi 8, lo 8]
+    // FP16 image [kb][t][g][hi, lo][j][8]: the ordinates q0..q0+7 (q0 = 32 kb + 8 g) of column
+    // 16 t + j as one 16-byte vector, the 16 vectors of a 16-lane row contiguous
+    static constexpr int kBHalves = kKB * NT * 4 * 2 * 16 * 8;
+    static constexpr int kLo = 16 * 8;  // the lo plane's offset from the hi plane
     static __device__ __forceinline__ int bindex(int q, int col)
     {
-        return ((((q >> 5) * NT + (col >> 4)) * 4 + ((q & 31) >> 3)) * 16 + (col & 15)) * 16 +
+        return ((((q >> 5) * NT + (col >> 4)) * 4 + ((q & 31) >> 3)) * 2 * 16 + (col & 15)) * 8 +
                (q & 7);
     }
-    // zero the rows and columns >= NM of both images (once per kernel, any thread count)
-    static __device__ __forceinline__ void clear_pad(float *sMt, _Float16 *sBh, int tid, int nth)
+    // zero the columns >= NM of the image (once per kernel, any thread count)
+    static __device__ __forceinline__ void clear_pad(_Float16 *sBh, int tid, int nth)
     {
         constexpr int kPad = 16 * NT - NM;
         if constexpr (kPad > 0) {
-            for (int e = tid; e < kPad * kTS; e += nth)
-                sMt[NM * kTS + e] = 0.0f;
             for (int e = tid; e < kChunk * kPad; e += nth) {
                 const int q = e / kPad, col = NM + e % kPad;
                 sBh[bindex(q, col)] = (_Float16)0.0f;
-                sBh[bindex(q, col) + 8] = (_Float16)0.0f;
+                sBh[bindex(q, col) + kLo] = (_Float16)0.0f;
             }
         }
     }
-    // element (q, col) of the chunk into both images
-    static __device__ __forceinline__ void put(float *sMt, _Float16 *sBh, int q, int col, double m)
+    // element (q, col) of the chunk into the image: its FP16 halves (the FP32 difference
+    // m32 - hi is exact)
+    static __device__ __forceinline__ void put(_Float16 *sBh, int q, int col, double m)
     {
         const float m32 = (float)m;
-        sMt[col * kTS + q] = m32;
         const _Float16 hi = (_Float16)m32;
         sBh[bindex(q, col)] = hi;
-        sBh[bindex(q, col) + 8] = (_Float16)(m32 - (float)hi);
+        sBh[bindex(q, col) + kLo] = (_Float16)(m32 - (float)hi);
+    }
+    // elements (q, col) and (q + 1, col), q even, as one 32-bit store per plane
+    static __device__ __forceinline__ void put_pair(_Float16 *sBh, int q, int col, double m0,
+                                                    double m1)
+    {
+        const float a = (float)m0, b = (float)m1;
+        const _Float16 ha = (_Float16)a, hb = (_Float16)b;
+        const h2 hi = {ha, hb};
+        const h2 lo = {(_Float16)(a - (float)ha), (_Float16)(b - (float)hb)};
+        *reinterpret_cast<h2 *>(&sBh[bindex(q, col)]) = hi;
+        *reinterpret_cast<h2 *>(&sBh[bindex(q, col) + kLo]) = lo;
     }
 };
 
@@ -1230,10 +1231,12 @@ __device__ __forceinline__ bool h16_range_ok(float wmaxLane)
     return m >= 0.0625f && m <= 16384.0f;
 }
 
+// sWH holds, per cell c, the FP16 halves of the scaled weights: hi at [2 c][q], lo at
+// [2 c + 1][q] (kChunk halves each).
 template <int NM, int CW>
 __device__ __forceinline__ void chunk_h16(
-    const double *sMc, const float *sMt, const _Float16 *sBh, double wl, const double *sAc,
-    const double *shift, float (&wmax)[CW], float *sWE, double *sWE64,
+    const double *sMc, const _Float16 *sBh, double wl, const double *sAc,
+    const double *shift, float (&wmax)[CW], _Float16 *sWH, double *sWE64,
     f4 (&acc)[CW][(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
     double (&gacc)[(NM + 15) / 16][4], double (&fpart)[CW], int lane)
 {
@@ -1241,7 +1244,8 @@ __device__ __forceinline__ void chunk_h16(
     constexpr int NT = HC::NT;
     constexpr int NMS = NM | 1;
     const double *rowp = &sMc[lane * NMS];
-    // 1. the lane's ordinate: exponent, FP64 weight (f, gradient), FP32 weight
+    // 1. the lane's ordinate: exponent, FP64 weight (f, gradient), FP16 halves of the scaled
+    // weight (the FP32 difference v - hi is exact)
     float wf[CW];
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
@@ -1251,13 +1255,25 @@ __device__ __forceinline__ void chunk_h16(
             x = fma(sAc[c * NM + i], rowp[i], x);
         const double we = wl * exp(x - shift[c]);
         fpart[c] = fpart[c] + we;
-        sWE64[c * kChunk + lane] = we;
+        sWE64[c * kWE64S + lane] = we;
         wf[c] = (float)we;
     }
+    // the halves of lanes 2k and 2k + 1 as two 32-bit stores: the even lane writes the hi pair,
+    // the odd lane the lo pair (the partner's half by a DPP swap within the pair)
+    const bool odd = lane & 1;
+    unsigned *sWH32 = reinterpret_cast<unsigned *>(sWH);
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
         wmax[c] = fmaxf(wmax[c], wf[c]);
-        sWE[c * kChunk + lane] = wf[c] * kWScale;
+        const float v = wf[c] * kWScale;
+        const _Float16 hi = (_Float16)v;
+        const _Float16 lo = (_Float16)(v - (float)hi);
+        const unsigned hb = __builtin_bit_cast(unsigned short, hi);
+        const unsigned lb = __builtin_bit_cast(unsigned short, lo);
+        const unsigned give = odd ? hb : lb;
+        const unsigned got = (unsigned)__builtin_amdgcn_mov_dpp((int)give, 0xB1, 0xF, 0xF, false);
+        const unsigned word = odd ? (got | (lb << 16)) : (hb | (got << 16));
+        sWH32[(2 * c + (odd ? 1 : 0)) * (kChunk / 2) + (lane >> 1)] = word;
     }
     wave_lds_sync();
     __builtin_amdgcn_sched_barrier(0);
@@ -1268,7 +1284,7 @@ __device__ __forceinline__ void chunk_h16(
 #pragma unroll 4
         for (int s = 0; s < kChunk / 4; ++s) {
             const int q = 4 * s + kk;
-            const double a = ar < CW ? sWE64[ar * kChunk + q] : 0.0;
+            const double a = ar < CW ? sWE64[ar * kWE64S + q] : 0.0;
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
                 const int col = 16 * t + bc;
@@ -1278,43 +1294,32 @@ __device__ __forceinline__ void chunk_h16(
         }
     }
     __builtin_amdgcn_sched_barrier(0);
-    // 3. the Hessian: FP16 split products, 32 ordinates per instruction
+    // 3. the Hessian: FP16 split products, 32 ordinates per instruction.  The A operand of
+    // tile t (rows 16 t + i16, ordinates q0..q0+7 of the lane) holds the same basis values as
+    // the B operand of tile t, so one image serves both.
     const int i16 = lane & 15, g = lane >> 4;
 #pragma unroll 1
     for (int kb = 0; kb < HC::kKB; ++kb) {
         const int q0 = 32 * kb + 8 * g;
-        float m8[NT][8];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const f4 lo4 = *reinterpret_cast<const f4 *>(&sMt[(16 * t + i16) * HC::kTS + q0]);
-            const f4 hi4 = *reinterpret_cast<const f4 *>(&sMt[(16 * t + i16) * HC::kTS + q0 + 4]);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                m8[t][e] = lo4[e];
-                m8[t][4 + e] = hi4[e];
-            }
-        }
         h8 bh[NT], bl[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const _Float16 *bp = &sBh[HC::bindex(q0, 16 * t + i16)];
             bh[t] = *reinterpret_cast<const h8 *>(bp);
-            bl[t] = *reinterpret_cast<const h8 *>(bp + 8);
+            bl[t] = *reinterpret_cast<const h8 *>(bp + HC::kLo);
         }
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
-            const f4 w0 = *reinterpret_cast<const f4 *>(&sWE[c * kChunk + q0]);
-            const f4 w1 = *reinterpret_cast<const f4 *>(&sWE[c * kChunk + q0 + 4]);
+            const h8 vh = *reinterpret_cast<const h8 *>(&sWH[(2 * c) * kChunk + q0]);
+            const h8 vl = *reinterpret_cast<const h8 *>(&sWH[(2 * c + 1) * kChunk + q0]);
             h8 ah[NT], al[NT];
 #pragma unroll
-            for (int t = 0; t < NT; ++t)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const float a = (e < 4 ? w0[e] : w1[e - 4]) * m8[t][e];
-                    const float hi = __uint_as_float(__float_as_uint(a) & 0xFFFFE000u);
-                    ah[t][e] = (_Float16)hi;
-                    al[t][e] = (_Float16)(a - hi);
-                }
+            for (int t = 0; t < NT; ++t) {
+                ah[t] = vh * bh[t];
+                h8 r = __builtin_elementwise_fma(vh, bh[t], -ah[t]);  // exact: vh bh - ah
+                r = __builtin_elementwise_fma(vh, bl[t], r);
+                al[t] = __builtin_elementwise_fma(vl, bh[t], r);
+            }
             acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0], bh[0], acc[c][0], 0, 0, 0);
             acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0], bl[0], acc[c][0], 0, 0, 0);
             acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[0], bh[0], acc[c][0], 0, 0, 0);
@@ -1328,7 +1333,7 @@ __device__ __forceinline__ void chunk_h16(
             }
         }
     }
-    wave_lds_sync();  // sWE and sWE64 are rewritten by the next chunk
+    wave_lds_sync();  // sWH and sWE64 are rewritten by the next chunk
 }
 
 // The FP64 gradient moments of row r (cell r) of the 4x4x4 accumulators into gm[NM]:
@@ -1345,25 +1350,45 @@ __device__ __forceinline__ void store_gm(const double (&gacc)[(NM + 15) / 16][4]
     }
 }
 
-// A staged chunk into the mixed sweep's LDS images: the FP64 rows, the FP32 transpose and the
-// FP16 halves (the stager's share of elements, as ChunkStager::put).
+// The mixed sweep's chunk stager: each thread holds the elements (q, col) and (q + 1, col) of
+// its (ordinate pair, column) items, so that the FP16 image takes 32-bit stores; put() writes
+// the FP64 rows, the FP16 image and the weights.
 template <int NM>
-__device__ __forceinline__ void put_h16(const ChunkStager<NM> &st, double *sM, float *sMt,
-                                        _Float16 *sBh, double *sW, int tid)
-{
-    constexpr int NMS = NM | 1;
+struct PairStager {
+    static constexpr int NMS = NM | 1;
+    static constexpr int kItems = kChunk / 2 * NM;
+    static constexpr int kPer = (kItems + kBlock - 1) / kBlock;
+    double pre[kPer][2];
+    double prew;
+    __device__ __forceinline__ void fetch(const double *__restrict__ Mg,
+                                          const double *__restrict__ wg, int ch, int tid)
+    {
+        const double *src = Mg + (size_t)ch * kChunk * NM;
 #pragma unroll
-    for (int k = 0; k < ChunkStager<NM>::kPer; ++k) {
-        const int e = tid + kBlock * k;
-        if (e < kChunk * NM) {
-            const int rr = e / NM, cc = e - rr * NM;
-            sM[rr * NMS + cc] = st.pre[k];
-            H16Chunk<NM>::put(sMt, sBh, rr, cc, st.pre[k]);
+        for (int k = 0; k < kPer; ++k) {
+            const int e = tid + kBlock * k;
+            const int p = e / NM, col = e - p * NM;
+            pre[k][0] = e < kItems ? src[(2 * p) * NM + col] : 0.0;
+            pre[k][1] = e < kItems ? src[(2 * p + 1) * NM + col] : 0.0;
         }
+        prew = tid < kChunk ? wg[(size_t)ch * kChunk + tid] : 0.0;
     }
-    if (tid < kChunk)
-        sW[tid] = st.prew;
-}
+    __device__ __forceinline__ void put(double *sM, _Float16 *sBh, double *sW, int tid) const
+    {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int e = tid + kBlock * k;
+            if (e < kItems) {
+                const int p = e / NM, col = e - p * NM;
+                sM[(2 * p) * NMS + col] = pre[k][0];
+                sM[(2 * p + 1) * NMS + col] = pre[k][1];
+                H16Chunk<NM>::put_pair(sBh, 2 * p, col, pre[k][0], pre[k][1]);
+            }
+        }
+        if (tid < kChunk)
+            sW[tid] = prew;
+    }
+};
 
 // H = e^shift * H32, row-major, lower triangle mirrored
 template <int NM>
@@ -1404,10 +1429,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     constexpr int NMS = NM | 1;
     constexpr int CW = kCellsPerWave;
     __shared__ double sM[2][kChunk * NMS];
-    __shared__ __align__(16) float sMt[2][HC::kTFloats];
     __shared__ __align__(16) _Float16 sBh[2][HC::kBHalves];
-    __shared__ __align__(16) float sWE[kWavesPerBlock][CW * kChunk];
-    __shared__ __align__(16) double sWE64[kWavesPerBlock][CW * kChunk];
+    __shared__ __align__(16) _Float16 sWH[kWavesPerBlock][2 * CW * kChunk];
+    __shared__ __align__(16) double sWE64[kWavesPerBlock][CW * kWE64S];
     __shared__ double sW[2][kChunk];
     __shared__ double sA[kWavesPerBlock][CW * NM];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1415,8 +1439,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const int nW = uniform_int(*nWp);
     const int groups = (nW + CW - 1) / CW;
     const int nBlocksNeeded = (groups + kWavesPerBlock - 1) / kWavesPerBlock;
-    HC::clear_pad(sMt[0], sBh[0], tid, kBlock);
-    HC::clear_pad(sMt[1], sBh[1], tid, kBlock);
+    HC::clear_pad(sBh[0], tid, kBlock);
+    HC::clear_pad(sBh[1], tid, kBlock);
     for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
         const int grp = blk * kWavesPerBlock + wv;
         int cells[CW];
@@ -1448,18 +1472,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             for (int k = 0; k < 4; ++k)
                 gacc[t][k] = 0.0;
         const bool any = cells[0] >= 0;
-        ChunkStager<NM> st;
+        PairStager<NM> st;
         st.fetch(Mg, wg, 0, tid);
         __syncthreads();  // the previous cells' last chunk is consumed
         for (int ch = 0; ch < nch; ++ch) {
             const int b = ch & 1;
-            put_h16<NM>(st, sM[b], sMt[b], sBh[b], sW[b], tid);
+            st.put(sM[b], sBh[b], sW[b], tid);
             __syncthreads();
             if (ch + 1 < nch)
                 st.fetch(Mg, wg, ch + 1, tid);
             if (any)
-                chunk_h16<NM, CW>(sM[b], sMt[b], sBh[b], sW[b][lane], sA[wv], shift, wmax,
-                                  sWE[wv], sWE64[wv], acc, gacc, fpart, lane);
+                chunk_h16<NM, CW>(sM[b], sBh[b], sW[b][lane], sA[wv], shift, wmax, sWH[wv],
+                                  sWE64[wv], acc, gacc, fpart, lane);
         }
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
@@ -2037,10 +2061,9 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     __shared__ int sOk[TW * KW];        // Armijo verdict at each trial point
     __shared__ double sF[1];            // f of the sweep
     // the mixed sweep's images (H16Chunk), weights and FP64 gradient moments
-    __shared__ __align__(16) float sMt[MIXED ? H16Chunk<NM>::kTFloats : 4];
     __shared__ __align__(16) _Float16 sBh[MIXED ? H16Chunk<NM>::kBHalves : 8];
-    __shared__ __align__(16) float sWE[MIXED ? kChunk : 4];
-    __shared__ __align__(16) double sWE64[MIXED ? kChunk : 2];
+    __shared__ __align__(16) _Float16 sWH[MIXED ? 2 * kChunk : 8];
+    __shared__ __align__(16) double sWE64[MIXED ? kWE64S : 2];
     __shared__ double sG[NM];
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int nWm = uniform_int(*Lc.nm);
@@ -2051,7 +2074,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     const int c = uniform_int(list_cell(Lc, slot, nWm));
     const int li = lane & (G - 1), row = lane / G;
     if constexpr (MIXED) {
-        H16Chunk<NM>::clear_pad(sMt, sBh, threadIdx.x, TW * kWave);
+        H16Chunk<NM>::clear_pad(sBh, threadIdx.x, TW * kWave);
         __syncthreads();
     }
     Cell<NM, G> C;
@@ -2165,7 +2188,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                         sM[rr * NMS + cc] = pre[k];
                         if constexpr (MIXED)
                             if (mx)
-                                H16Chunk<NM>::put(sMt, sBh, rr, cc, pre[k]);
+                                H16Chunk<NM>::put(sBh, rr, cc, pre[k]);
                     }
                     const double wl = prew;
                     wave_lds_sync();
@@ -2177,8 +2200,8 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                             pre[k] = Mg[base * NM + lane + kWave * k];
                     }
                     if (mx)
-                        chunk_h16<NM, 1>(sM, sMt, sBh, wl, sA[0], shift, wmax1, sWE, sWE64,
-                                         acc32, gacc, fpart, lane);
+                        chunk_h16<NM, 1>(sM, sBh, wl, sA[0], shift, wmax1, sWH, sWE64, acc32,
+                                         gacc, fpart, lane);
                     else
                         chunk_mfma<NM, 1>(sM, wl, sA[0], acc, fpart, lane);
                 }
@@ -2533,14 +2556,13 @@ __global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nq
     constexpr int NB = NT * (NT + 1) / 2;
     constexpr int NMS = NM | 1;
     __shared__ double sM[kChunk * NMS];
-    __shared__ __align__(16) float sMt[HC::kTFloats];
     __shared__ __align__(16) _Float16 sBh[HC::kBHalves];
-    __shared__ __align__(16) float sWE[kWavesPerBlock][kChunk];
-    __shared__ __align__(16) double sWE64[kWavesPerBlock][kChunk];
+    __shared__ __align__(16) _Float16 sWH[kWavesPerBlock][2 * kChunk];
+    __shared__ __align__(16) double sWE64[kWavesPerBlock][kWE64S];
     __shared__ double sW[kChunk];
     __shared__ double sA[kWavesPerBlock][NM];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    HC::clear_pad(sMt, sBh, tid, kBlock);
+    HC::clear_pad(sBh, tid, kBlock);
     const int64_t cell = (int64_t)blockIdx.x * kWavesPerBlock + wv;
     const bool have = cell < ncells;
     if (lane < NM)
@@ -2590,14 +2612,14 @@ __global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nq
             int rr = e / NM, cc = e - rr * NM;
             const double m = Mg[(size_t)c * kChunk * NM + e];
             sM[rr * NMS + cc] = m;
-            HC::put(sMt, sBh, rr, cc, m);
+            HC::put(sBh, rr, cc, m);
         }
         if (tid < kChunk)
             sW[tid] = wg[(size_t)c * kChunk + tid];
         __syncthreads();
         if (have)
-            chunk_h16<NM, 1>(sM, sMt, sBh, sW[lane], sA[wv], shift, wmax, sWE[wv], sWE64[wv],
-                             acc, gacc, fpart, lane);
+            chunk_h16<NM, 1>(sM, sBh, sW[lane], sA[wv], shift, wmax, sWH[wv], sWE64[wv], acc,
+                             gacc, fpart, lane);
     }
     if (!have)
         return;
