@@ -166,6 +166,22 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Sum over the lane's 16-lane row (butterfly: every lane of the row gets the same bits).
+__device__ __forceinline__ double row16_sum(double v)
+{
+#pragma unroll
+    for (int o = 8; o >= 1; o >>= 1)
+        v = v + __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ float row16_max(float v)
+{
+#pragma unroll
+    for (int o = 8; o >= 1; o >>= 1)
+        v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
 // A cell's lanes: the whole wave (G = 64, one cell per wave; broadcasts by readlane) or one
 // 16-lane row (G = 16, four cells per wave; broadcasts by DPP row_newbcast).  Lane indices
 // passed to the algebra below are indices within the group.  The arithmetic a cell sees is
@@ -1225,47 +1241,80 @@ __device__ __forceinline__ float wave_max_f(float v)
         v = fmaxf(v, __shfl_xor(v, o));
     return v;
 }
-__device__ __forceinline__ bool h16_range_ok(float wmaxLane)
+__device__ __forceinline__ bool h16_range_ok(float wmax)
 {
-    const float m = wave_max_f(wmaxLane) * kWScale;
+    const float m = wmax * kWScale;
     return m >= 0.0625f && m <= 16384.0f;
 }
 
+// Exponent operands of V_MFMA_F64_4X4X4_4B_F64 with the cells as rows: the A operand of K
+// step s is alpha[lane & 3][4 s + (lane >> 4)] (zero beyond NM and for rows >= the cells
+// present), kept in registers for a whole sweep.
+template <int NM>
+struct ExpA {
+    static constexpr int NKS = (NM + 3) / 4;
+    double a[NKS];
+    // alpha of row r at sA + r * NM for rows r < rows
+    __device__ __forceinline__ void load(const double *sA, int rows, int lane)
+    {
+        const int r = lane & 3, k = lane >> 4;
+#pragma unroll
+        for (int s = 0; s < NKS; ++s) {
+            const int m = 4 * s + k;
+            a[s] = (r < rows && m < NM) ? sA[r * NM + m] : 0.0;
+        }
+    }
+};
+
+// The exponents x[i][q] = sum_m alpha[i][m] M[q][m] of ordinates q = 16 g + (lane & 15) for
+// row i = lane >> 4, as a chain of K steps on the FP64 matrix cores; b(s) returns
+// M[q][4 s + (lane >> 4)], zero for moments >= NM.
+template <int NM, typename BF>
+__device__ __forceinline__ double exp_arg(const ExpA<NM> &A, BF b)
+{
+    double x = 0.0;
+#pragma unroll
+    for (int s = 0; s < ExpA<NM>::NKS; ++s)
+        x = __builtin_amdgcn_mfma_f64_4x4x4f64(A.a[s], b(s), x, 0, 0, 0);
+    return x;
+}
+
 // sWH holds, per cell c, the FP16 halves of the scaled weights: hi at [2 c][q], lo at
-// [2 c + 1][q] (kChunk halves each).
+// [2 c + 1][q] (kChunk halves each).  Lane L works on cell L >> 4 (rows >= CW compute
+// nothing that is kept) at ordinates 16 g + (L & 15), g = 0..3: fOwn and wmaxOwn are its
+// running sum and maximum of the weights, shiftOwn its cell's shift, sWc the chunk's
+// quadrature weights.
 template <int NM, int CW>
 __device__ __forceinline__ void chunk_h16(
-    const double *sMc, const _Float16 *sBh, double wl, const double *sAc,
-    const double *shift, float (&wmax)[CW], _Float16 *sWH, double *sWE64,
+    const double *sMc, const _Float16 *sBh, const double *sWc, const ExpA<NM> &ax,
+    double shiftOwn, float &wmaxOwn, _Float16 *sWH, double *sWE64,
     f4 (&acc)[CW][(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
-    double (&gacc)[(NM + 15) / 16][4], double (&fpart)[CW], int lane)
+    double (&gacc)[(NM + 15) / 16][4], double &fOwn, int lane)
 {
     using HC = H16Chunk<NM>;
     constexpr int NT = HC::NT;
     constexpr int NMS = NM | 1;
-    const double *rowp = &sMc[lane * NMS];
-    // 1. the lane's ordinate: exponent, FP64 weight (f, gradient), FP16 halves of the scaled
-    // weight (the FP32 difference v - hi is exact)
-    float wf[CW];
-#pragma unroll
-    for (int c = 0; c < CW; ++c) {
-        double x = 0.0;
-#pragma unroll
-        for (int i = 0; i < NM; ++i)
-            x = fma(sAc[c * NM + i], rowp[i], x);
-        const double we = wl * exp(x - shift[c]);
-        fpart[c] = fpart[c] + we;
-        sWE64[c * kWE64S + lane] = we;
-        wf[c] = (float)we;
-    }
-    // the halves of lanes 2k and 2k + 1 as two 32-bit stores: the even lane writes the hi pair,
-    // the odd lane the lo pair (the partner's half by a DPP swap within the pair)
+    // 1. per group of 16 ordinates: the exponents on the matrix cores, the FP64 weights (f,
+    // gradient) and the FP16 halves of the scaled weights (the FP32 difference v - hi is
+    // exact); the halves of lanes 2k and 2k + 1 (ordinates q, q + 1 of one cell) go out as two
+    // 32-bit stores, the even lane writing the hi pair, the odd lane the lo pair (the
+    // partner's half by a DPP swap within the pair)
+    const int row = lane >> 4, r16 = lane & 15;
     const bool odd = lane & 1;
     unsigned *sWH32 = reinterpret_cast<unsigned *>(sWH);
 #pragma unroll
-    for (int c = 0; c < CW; ++c) {
-        wmax[c] = fmaxf(wmax[c], wf[c]);
-        const float v = wf[c] * kWScale;
+    for (int g = 0; g < 4; ++g) {
+        const int q = 16 * g + r16;
+        const double *mrow = &sMc[q * NMS];
+        const double x = exp_arg<NM>(ax, [&](int s) {
+            const int m = 4 * s + (lane >> 4);
+            return (NM % 4 == 0 || m < NM) ? mrow[m] : 0.0;
+        });
+        const double we = sWc[q] * exp(x - shiftOwn);
+        fOwn = fOwn + we;
+        const float wf = (float)we;
+        wmaxOwn = fmaxf(wmaxOwn, wf);
+        const float v = wf * kWScale;
         const _Float16 hi = (_Float16)v;
         const _Float16 lo = (_Float16)(v - (float)hi);
         const unsigned hb = __builtin_bit_cast(unsigned short, hi);
@@ -1273,7 +1322,10 @@ __device__ __forceinline__ void chunk_h16(
         const unsigned give = odd ? hb : lb;
         const unsigned got = (unsigned)__builtin_amdgcn_mov_dpp((int)give, 0xB1, 0xF, 0xF, false);
         const unsigned word = odd ? (got | (lb << 16)) : (hb | (got << 16));
-        sWH32[(2 * c + (odd ? 1 : 0)) * (kChunk / 2) + (lane >> 1)] = word;
+        if (row < CW) {
+            sWE64[row * kWE64S + q] = we;
+            sWH32[(2 * row + (odd ? 1 : 0)) * (kChunk / 2) + (q >> 1)] = word;
+        }
     }
     wave_lds_sync();
     __builtin_amdgcn_sched_barrier(0);
@@ -1456,21 +1508,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         }
         f4 acc[CW][NB];
         double gacc[NT][4];
-        double fpart[CW];
-        float wmax[CW];
 #pragma unroll
-        for (int c = 0; c < CW; ++c) {
+        for (int c = 0; c < CW; ++c)
 #pragma unroll
             for (int b = 0; b < NB; ++b)
                 acc[c][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-            fpart[c] = 0.0;
-            wmax[c] = 0.0f;
-        }
 #pragma unroll
         for (int t = 0; t < NT; ++t)
 #pragma unroll
             for (int k = 0; k < 4; ++k)
                 gacc[t][k] = 0.0;
+        // the lane's row: cell lane >> 4
+        double shiftOwn = 0.0, fOwn = 0.0;
+        float wmaxOwn = 0.0f;
+#pragma unroll
+        for (int c = 0; c < CW; ++c)
+            if ((lane >> 4) == c)
+                shiftOwn = shift[c];
+        wave_lds_sync();  // sA
+        ExpA<NM> ax;
+        ax.load(sA[wv], CW, lane);
         const bool any = cells[0] >= 0;
         PairStager<NM> st;
         st.fetch(Mg, wg, 0, tid);
@@ -1482,13 +1539,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             if (ch + 1 < nch)
                 st.fetch(Mg, wg, ch + 1, tid);
             if (any)
-                chunk_h16<NM, CW>(sM[b], sBh[b], sW[b][lane], sA[wv], shift, wmax, sWH[wv],
-                                  sWE64[wv], acc, gacc, fpart, lane);
+                chunk_h16<NM, CW>(sM[b], sBh[b], sW[b], ax, shiftOwn, wmaxOwn, sWH[wv],
+                                  sWE64[wv], acc, gacc, fOwn, lane);
         }
+        const double fRow = row16_sum(fOwn);
+        const float wRow = row16_max(wmaxOwn);
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
-            const double fs = wave_sum(fpart[c]);
-            const bool ok = h16_range_ok(wmax[c]);
+            const double fs = __shfl(fRow, 16 * c);
+            const bool ok = h16_range_ok(__shfl(wRow, 16 * c));
             if (cells[c] < 0)
                 continue;
             const double es = exp(shift[c]);
@@ -1931,10 +1990,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
 // operations the sweep kernels use for that sum, so that a cell's result does not depend on
 // which kernel evaluated it -- kFp64: chunk_mfma's exponent, per-lane partials over the
 // chunks and final wave_sum; kFRef: chunk_ref's deterministic exp and ordinate-order sum
-// (one point); kFMixed: chunk_mixed's fused exponent, shifted exp and e^shift rescale.  M is
-// read from the transposed table MT [NM][nqPad] (one coalesced load per component), two
-// chunks in flight, each chunk's M shared by the kw points.
-enum : int { kFp64 = 0, kFRef = 1, kFMixed = 2 };
+// (one point).  (The mixed phase's points: f_trials_mixed.)  M is read from the transposed
+// table MT [NM][nqPad] (one coalesced load per component), two chunks in flight, each chunk's
+// M shared by the kw points.
+enum : int { kFp64 = 0, kFRef = 1 };
 template <int NM, int MODE, int KW>
 __device__ __forceinline__ void f_trials(const KParams &P, const double *__restrict__ MT,
                                          const double *__restrict__ wg, const double *sA,
@@ -1965,12 +2024,7 @@ __device__ __forceinline__ void f_trials(const KParams &P, const double *__restr
                 continue;
             const double *a = sA + j * NM;
             double x = 0.0;
-            if constexpr (MODE == kFMixed) {
-#pragma unroll
-                for (int i = 0; i < NM; ++i)
-                    x = fma(a[i], m[i], x);
-                fpart[j] = fpart[j] + w * exp(x - shift);
-            } else {
+            {
                 if constexpr (MODE == kFRef) {
 #pragma unroll
                     for (int i = 0; i < NM; ++i)
@@ -2013,11 +2067,74 @@ __device__ __forceinline__ void f_trials(const KParams &P, const double *__restr
             continue;
         if constexpr (MODE == kFRef)
             out[j] = fpart[j];
-        else if constexpr (MODE == kFMixed)
-            out[j] = wave_sum(fpart[j]) * exp(shift);
         else
             out[j] = wave_sum(fpart[j]);
     }
+}
+
+// f at up to four trial points in the mixed phase, as chunk_h16 evaluates it: the points are
+// the rows of the exponent products (alpha of point j at sA + j * NM, j < kw), lane L takes
+// point L >> 4 at ordinates 16 g + (L & 15) of each chunk and sums its weights in chunk and
+// group order, then the 16-lane row sum and the e^shift rescale.  M is read from its
+// transposed copy MT[m][q] (leading dimension nqPad), two chunks in flight (one at nm = 25).
+template <int NM>
+__device__ __forceinline__ void f_trials_mixed(const KParams &P, const double *__restrict__ MT,
+                                               const double *__restrict__ wg, const double *sA,
+                                               int kw, double shift, int lane, double (&out)[4])
+{
+    constexpr int NKS = ExpA<NM>::NKS;
+    const int nch = P.nqPad / kChunk;
+    const size_t ld = (size_t)P.nqPad;
+    const int r16 = lane & 15, kk = lane >> 4;
+    ExpA<NM> ax;
+    ax.load(sA, kw, lane);
+    auto load = [&](double(&b)[4][NKS], double(&w)[4], int ch) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const size_t q = (size_t)ch * kChunk + 16 * g + r16;
+#pragma unroll
+            for (int t = 0; t < NKS; ++t) {
+                const int m = 4 * t + kk;
+                b[g][t] = (NM % 4 == 0 || m < NM) ? MT[m * ld + q] : 0.0;
+            }
+            w[g] = wg[q];
+        }
+    };
+    double fOwn = 0.0;
+    auto term = [&](const double(&b)[4][NKS], const double(&w)[4]) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const double x = exp_arg<NM>(ax, [&](int t) { return b[g][t]; });
+            fOwn = fOwn + w[g] * exp(x - shift);
+        }
+    };
+    if constexpr (NKS <= 4) {
+        double bA[4][NKS], bB[4][NKS], wA[4], wB[4];
+        load(bA, wA, 0);
+        if (nch > 1)
+            load(bB, wB, 1);
+        for (int ch = 0; ch < nch; ch += 2) {
+            term(bA, wA);
+            if (ch + 2 < nch)
+                load(bA, wA, ch + 2);
+            if (ch + 1 < nch) {
+                term(bB, wB);
+                if (ch + 3 < nch)
+                    load(bB, wB, ch + 3);
+            }
+        }
+    } else {  // nm = 25: one chunk in flight (registers)
+        double bA[4][NKS], wA[4];
+        for (int ch = 0; ch < nch; ++ch) {
+            load(bA, wA, ch);
+            term(bA, wA);
+        }
+    }
+    const double fRow = row16_sum(fOwn);
+    const double es = exp(shift);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        out[j] = __shfl(fRow, 16 * j) * es;
 }
 
 // ---------------------------------------------------------------- fused tail pass
@@ -2047,6 +2164,9 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     // trial points per wave in a wide round: as many as keep a round's latency near one f
     // evaluation's (f costs nq (NM + exp) per point)
     constexpr int KW = REF ? 1 : (NM <= 9 ? 8 : (NM <= 16 ? 2 : 1));
+    // in the mixed phase the points are the rows of the exponent products: four per wave
+    constexpr int KWM = NM <= 16 ? 4 : 2;
+    constexpr int KWA = MIXED && KWM > KW ? KWM : KW;
     static_assert(!(REF && MIXED), "one arithmetic mode");
     __shared__ double sM[kChunk * NMS];
     __shared__ double sH[NM * NM];
@@ -2057,8 +2177,8 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     __shared__ double sT[TW][RW][NM * (NM + 1)];
     __shared__ double sW[TW][kChunk];
     __shared__ double sE[TW][kChunk];
-    __shared__ double sA[TW][KW * NM];
-    __shared__ int sOk[TW * KW];        // Armijo verdict at each trial point
+    __shared__ double sA[TW][KWA * NM];
+    __shared__ int sOk[TW * KWA];       // Armijo verdict at each trial point
     __shared__ double sF[1];            // f of the sweep
     // the mixed sweep's images (H16Chunk), weights and FP64 gradient moments
     __shared__ __align__(16) _Float16 sBh[MIXED ? H16Chunk<NM>::kBHalves : 8];
@@ -2086,13 +2206,13 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
         const bool mx = MIXED && C.mixed(P);
         if (C.phase == kPhaseLS) {
             // points k = wv * kw + j at tls / 2^k, formed as ls_next() forms them
-            const int kw = C.tls < 1.0 ? KW : 1;
+            const int kw = C.tls < 1.0 ? (mx ? KWM : KW) : 1;
             double tk = C.tls;
             for (int k = 0; k < wv * kw; ++k)
                 tk = 0.5 * tk;
-            double aj[KW], tj[KW];
+            double aj[KWA], tj[KWA];
 #pragma unroll
-            for (int j = 0; j < KW; ++j) {
+            for (int j = 0; j < KWA; ++j) {
                 tj[j] = tk;
                 aj[j] = (wv == 0 && j == 0) ? C.a : (li < NM ? C.aOld + tk * C.d : 0.0);
                 if (j < kw && lane < NM)
@@ -2100,16 +2220,24 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 tk = 0.5 * tk;
             }
             wave_lds_sync();
-            double ft[KW];
-            if (mx)
-                f_trials<NM, kFMixed, KW>(P, MT, wg, sA[wv], kw, C.shift, sW[wv], sE[wv], lane,
-                                          ft);
-            else
+            double ft[KWA];
+            if (mx) {
+                double f4p[4];
+                f_trials_mixed<NM>(P, MT, wg, sA[wv], kw, C.shift, lane, f4p);
+#pragma unroll
+                for (int j = 0; j < KWA; ++j)
+                    ft[j] = j < 4 ? f4p[j] : 0.0;
+            } else {
+                double fk[KW];
                 f_trials<NM, REF ? kFRef : kFp64, KW>(P, MT, wg, sA[wv], kw, 0.0, sW[wv],
-                                                      sE[wv], lane, ft);
+                                                      sE[wv], lane, fk);
+#pragma unroll
+                for (int j = 0; j < KWA; ++j)
+                    ft[j] = j < KW ? fk[j] : 0.0;
+            }
             // each wave judges its own points; the search then walks the verdicts in order
 #pragma unroll
-            for (int j = 0; j < KW; ++j) {
+            for (int j = 0; j < KWA; ++j) {
                 if (j < kw) {
                     const bool ok = C.template accepts_at<REF>(ft[j], aj[j], tj[j]);
                     if (lane == 0)
@@ -2165,14 +2293,20 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                     acc32[0][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
                 }
                 double fpart[1] = {0.0};
-                const double shift[1] = {C.shift};
                 double gacc[NT][4];
-                float wmax1[1] = {0.0f};
+                float wmaxOwn = 0.0f;
+                double fOwn = 0.0;
+                const double shiftOwn = (lane >> 4) == 0 ? C.shift : 0.0;
 #pragma unroll
                 for (int t = 0; t < NT; ++t)
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
                         gacc[t][k] = 0.0;
+                ExpA<NM> ax;
+                if (mx) {
+                    wave_lds_sync();  // sA[0]
+                    ax.load(sA[0], 1, lane);
+                }
                 double pre[NM];
                 const int qo = mx ? lane : mfma_ordinate(lane);  // the lane's weight
                 double prew = wg[qo];
@@ -2191,6 +2325,8 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                                 H16Chunk<NM>::put(sBh, rr, cc, pre[k]);
                     }
                     const double wl = prew;
+                    if (mx)
+                        sW[0][lane] = wl;  // the chunk's weights (qo = lane)
                     wave_lds_sync();
                     if (ch + 1 < nch) {
                         const size_t base = (size_t)(ch + 1) * kChunk;
@@ -2200,16 +2336,16 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                             pre[k] = Mg[base * NM + lane + kWave * k];
                     }
                     if (mx)
-                        chunk_h16<NM, 1>(sM, sBh, wl, sA[0], shift, wmax1, sWH, sWE64, acc32,
-                                         gacc, fpart, lane);
+                        chunk_h16<NM, 1>(sM, sBh, sW[0], ax, shiftOwn, wmaxOwn, sWH, sWE64,
+                                         acc32, gacc, fOwn, lane);
                     else
                         chunk_mfma<NM, 1>(sM, wl, sA[0], acc, fpart, lane);
                 }
                 wave_lds_sync();
                 if (mx) {
                     const double es = exp(C.shift);
-                    fdir = wave_sum(fpart[0]) * es;
-                    const bool ok = h16_range_ok(wmax1[0]);
+                    fdir = __shfl(row16_sum(fOwn), 0) * es;
+                    const bool ok = h16_range_ok(__shfl(row16_max(wmaxOwn), 0));
                     store_H32<NM>(acc32[0], es / kWScale, sH, lane);
                     store_gm<NM>(gacc, 0, es, sG, lane);
                     if (!ok && lane == 0)
@@ -2577,8 +2713,8 @@ __global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nq
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             gacc[t][k] = 0.0;
-    double fpart[1] = {0.0};
-    float wmax[1] = {0.0f};
+    double fOwn = 0.0;
+    float wmaxOwn = 0.0f;
     // a first pass for the shift: the largest exponent and weight, so that every weight of the
     // second pass is at most one and the largest at least min w / max w
     double xmax = -DBL_MAX, wtop = 0.0;
@@ -2605,7 +2741,10 @@ __global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nq
         xmax = fmax(xmax, __shfl_xor(xmax, o));
         wtop = fmax(wtop, __shfl_xor(wtop, o));
     }
-    const double shift[1] = {have ? xmax + log(wtop) : 0.0};
+    const double shift = have ? xmax + log(wtop) : 0.0;
+    const double shiftOwn = (lane >> 4) == 0 ? shift : 0.0;
+    ExpA<NM> ax;
+    ax.load(sA[wv], 1, lane);  // sA was published by the first pass's barriers
     for (int c = 0; c < nqPad / kChunk; ++c) {
         __syncthreads();
         for (int e = tid; e < kChunk * NM; e += kBlock) {
@@ -2618,12 +2757,12 @@ __global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nq
             sW[tid] = wg[(size_t)c * kChunk + tid];
         __syncthreads();
         if (have)
-            chunk_h16<NM, 1>(sM, sBh, sW[lane], sA[wv], shift, wmax, sWH[wv], sWE64[wv], acc,
-                             gacc, fpart, lane);
+            chunk_h16<NM, 1>(sM, sBh, sW, ax, shiftOwn, wmaxOwn, sWH[wv], sWE64[wv], acc, gacc,
+                             fOwn, lane);
     }
     if (!have)
         return;
-    store_H32<NM>(acc[0], exp(shift[0]) / kWScale, hs + cell * NM * NM, lane);
+    store_H32<NM>(acc[0], exp(shift) / kWScale, hs + cell * NM * NM, lane);
 }
 
 // The dense part of one Newton iteration for a batch of given Hessians (one wave each):
