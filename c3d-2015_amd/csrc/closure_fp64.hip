@@ -1266,17 +1266,21 @@ struct ExpA {
     }
 };
 
-// The exponents x[i][q] = sum_m alpha[i][m] M[q][m] of ordinates q = 16 g + (lane & 15) for
-// row i = lane >> 4, as a chain of K steps on the FP64 matrix cores; b(s) returns
-// M[q][4 s + (lane >> 4)], zero for moments >= NM.
+// The exponents x[i][q] = sum_m alpha[i][m] M[q][m] of ordinates q = 16 g + (lane & 15),
+// g = 0..3, for row i = lane >> 4: one chain of K steps on the FP64 matrix cores per group,
+// the four chains interleaved (K step outer); b(g, s) returns M[16 g + (lane & 15)][4 s +
+// (lane >> 4)], zero for moments >= NM.
 template <int NM, typename BF>
-__device__ __forceinline__ double exp_arg(const ExpA<NM> &A, BF b)
+__device__ __forceinline__ void exp_arg4(const ExpA<NM> &A, BF b, double (&x)[4])
 {
-    double x = 0.0;
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+        x[g] = 0.0;
 #pragma unroll
     for (int s = 0; s < ExpA<NM>::NKS; ++s)
-        x = __builtin_amdgcn_mfma_f64_4x4x4f64(A.a[s], b(s), x, 0, 0, 0);
-    return x;
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+            x[g] = __builtin_amdgcn_mfma_f64_4x4x4f64(A.a[s], b(g, s), x[g], 0, 0, 0);
 }
 
 // sWH holds, per cell c, the FP16 halves of the scaled weights: hi at [2 c][q], lo at
@@ -1302,14 +1306,15 @@ __device__ __forceinline__ void chunk_h16(
     const int row = lane >> 4, r16 = lane & 15;
     const bool odd = lane & 1;
     unsigned *sWH32 = reinterpret_cast<unsigned *>(sWH);
+    double xg[4];
+    exp_arg4<NM>(ax, [&](int g, int s) {
+        const int m = 4 * s + (lane >> 4);
+        return (NM % 4 == 0 || m < NM) ? sMc[(16 * g + r16) * NMS + m] : 0.0;
+    }, xg);
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
         const int q = 16 * g + r16;
-        const double *mrow = &sMc[q * NMS];
-        const double x = exp_arg<NM>(ax, [&](int s) {
-            const int m = 4 * s + (lane >> 4);
-            return (NM % 4 == 0 || m < NM) ? mrow[m] : 0.0;
-        });
+        const double x = xg[g];
         const double we = sWc[q] * exp(x - shiftOwn);
         fOwn = fOwn + we;
         const float wf = (float)we;
@@ -1360,10 +1365,15 @@ __device__ __forceinline__ void chunk_h16(
             bh[t] = *reinterpret_cast<const h8 *>(bp);
             bl[t] = *reinterpret_cast<const h8 *>(bp + HC::kLo);
         }
+        h8 vhn = *reinterpret_cast<const h8 *>(&sWH[q0]);
+        h8 vln = *reinterpret_cast<const h8 *>(&sWH[kChunk + q0]);
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
-            const h8 vh = *reinterpret_cast<const h8 *>(&sWH[(2 * c) * kChunk + q0]);
-            const h8 vl = *reinterpret_cast<const h8 *>(&sWH[(2 * c + 1) * kChunk + q0]);
+            const h8 vh = vhn, vl = vln;
+            if (c + 1 < CW) {  // the next cell's halves in flight
+                vhn = *reinterpret_cast<const h8 *>(&sWH[(2 * c + 2) * kChunk + q0]);
+                vln = *reinterpret_cast<const h8 *>(&sWH[(2 * c + 3) * kChunk + q0]);
+            }
             h8 ah[NT], al[NT];
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
@@ -2102,11 +2112,11 @@ __device__ __forceinline__ void f_trials_mixed(const KParams &P, const double *_
     };
     double fOwn = 0.0;
     auto term = [&](const double(&b)[4][NKS], const double(&w)[4]) {
+        double xg[4];
+        exp_arg4<NM>(ax, [&](int g, int t) { return b[g][t]; }, xg);
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const double x = exp_arg<NM>(ax, [&](int t) { return b[g][t]; });
-            fOwn = fOwn + w[g] * exp(x - shift);
-        }
+        for (int g = 0; g < 4; ++g)
+            fOwn = fOwn + w[g] * exp(xg[g] - shift);
     };
     if constexpr (NKS <= 4) {
         double bA[4][NKS], bB[4][NKS], wA[4], wB[4];
