@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libc3d_hip.so")
 NOALLOC_SCRATCH_BYTES = 128 << 20
 FLAG_ISO_RESTART, FLAG_U0_NONPOS, FLAG_LADDER, FLAG_BUDGET, FLAG_FP64_FALLBACK = 1, 2, 4, 8, 16
 PREC_FP64, PREC_FP64_REFORDER, PREC_MIXED = 0, 1, 2
-LAUNCH_MIXED_WAVES, LAUNCH_MIXED_FINISH_COND = 1, 2
+LAUNCH_MIXED_WAVES, LAUNCH_MIXED_FINISH_COND, LAUNCH_ASYNC = 1, 2, 3
 
 _vp = C.c_void_p
 _i32 = C.c_int32

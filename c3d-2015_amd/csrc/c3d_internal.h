@@ -100,6 +100,8 @@ struct c3d_ctx_s {
     int tailCells = 8192;
     int tailPasses = 32;
     int tailWideCells = 512;  // four waves per cell (speculative line-search trials) below this
+    // > 0: the asynchronous schedule with this many device-gated bulk passes (no host sync)
+    int asyncPasses = 0;
     c3d::Tune tune;
     int gauntDirect[2] = {-1, -1};  // Gaunt form (dense, CSR) equals the direct form: 1, 0, -1 unknown
     std::mutex mtx;

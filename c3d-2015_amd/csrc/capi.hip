@@ -404,6 +404,11 @@ int c3d_set_launch(c3d_ctx ctx, int param, int value)
             return C3D_ERR_ARG;
         ctx->mixedFinishCond = std::pow(10.0, value / 10.0);
         return C3D_OK;
+    case C3D_LAUNCH_ASYNC:
+        if (value < 0 || value > 10000)
+            return C3D_ERR_ARG;
+        ctx->asyncPasses = value;
+        return C3D_OK;
     default:
         return C3D_ERR_ARG;
     }

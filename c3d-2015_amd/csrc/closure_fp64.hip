@@ -2158,7 +2158,7 @@ __device__ __forceinline__ void f_trials_mixed(const KParams &P, const double *_
 // one round settles up to TW * KW trials of the sequential search.  Every wave keeps an
 // identical copy of the state machine (same inputs, same operations), which keeps control
 // flow uniform over the block; wave 0 sweeps H and writes the results.  Launch: TW * 64
-// threads per block, one block per listed cell (grid >= *nWp).
+// threads per block; the blocks stride over the listed cells (any grid).
 template <int NM, bool REF, bool MIXED, int TW>
 __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__restrict__ Mg,
                                                       const double *__restrict__ MT,
@@ -2198,193 +2198,195 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int nWm = uniform_int(*Lc.nm);
     const int nW = nWm + uniform_int(*Lc.nf);
-    const int slot = blockIdx.x;
-    if (slot >= nW)
+    if ((int)blockIdx.x >= nW)
         return;
-    const int c = uniform_int(list_cell(Lc, slot, nWm));
     const int li = lane & (G - 1), row = lane / G;
     if constexpr (MIXED) {
         H16Chunk<NM>::clear_pad(sBh, threadIdx.x, TW * kWave);
         __syncthreads();
     }
-    Cell<NM, G> C;
-    C.load(S, c, li);
-    const int nch = P.nqPad / kChunk;
-    int done = 0;
-    int sweeps = 0;
-    for (int p = 0; p < passes; ++p) {
-        const bool mx = MIXED && C.mixed(P);
-        if (C.phase == kPhaseLS) {
-            // points k = wv * kw + j at tls / 2^k, formed as ls_next() forms them
-            const int kw = C.tls < 1.0 ? (mx ? KWM : KW) : 1;
-            double tk = C.tls;
-            for (int k = 0; k < wv * kw; ++k)
-                tk = 0.5 * tk;
-            double aj[KWA], tj[KWA];
+    for (int slot = blockIdx.x; slot < nW; slot += gridDim.x) {
+        const int c = uniform_int(list_cell(Lc, slot, nWm));
+        Cell<NM, G> C;
+        C.load(S, c, li);
+        const int nch = P.nqPad / kChunk;
+        int done = 0;
+        int sweeps = 0;
+        for (int p = 0; p < passes; ++p) {
+            const bool mx = MIXED && C.mixed(P);
+            if (C.phase == kPhaseLS) {
+                // points k = wv * kw + j at tls / 2^k, formed as ls_next() forms them
+                const int kw = C.tls < 1.0 ? (mx ? KWM : KW) : 1;
+                double tk = C.tls;
+                for (int k = 0; k < wv * kw; ++k)
+                    tk = 0.5 * tk;
+                double aj[KWA], tj[KWA];
 #pragma unroll
-            for (int j = 0; j < KWA; ++j) {
-                tj[j] = tk;
-                aj[j] = (wv == 0 && j == 0) ? C.a : (li < NM ? C.aOld + tk * C.d : 0.0);
-                if (j < kw && lane < NM)
-                    sA[wv][j * NM + lane] = aj[j];
-                tk = 0.5 * tk;
-            }
-            wave_lds_sync();
-            double ft[KWA];
-            if (mx) {
-                double f4p[4];
-                f_trials_mixed<NM>(P, MT, wg, sA[wv], kw, C.shift, lane, f4p);
-#pragma unroll
-                for (int j = 0; j < KWA; ++j)
-                    ft[j] = j < 4 ? f4p[j] : 0.0;
-            } else {
-                double fk[KW];
-                f_trials<NM, REF ? kFRef : kFp64, KW>(P, MT, wg, sA[wv], kw, 0.0, sW[wv],
-                                                      sE[wv], lane, fk);
-#pragma unroll
-                for (int j = 0; j < KWA; ++j)
-                    ft[j] = j < KW ? fk[j] : 0.0;
-            }
-            // each wave judges its own points; the search then walks the verdicts in order
-#pragma unroll
-            for (int j = 0; j < KWA; ++j) {
-                if (j < kw) {
-                    const bool ok = C.template accepts_at<REF>(ft[j], aj[j], tj[j]);
-                    if (lane == 0)
-                        sOk[wv * kw + j] = ok;
+                for (int j = 0; j < KWA; ++j) {
+                    tj[j] = tk;
+                    aj[j] = (wv == 0 && j == 0) ? C.a : (li < NM ? C.aOld + tk * C.d : 0.0);
+                    if (j < kw && lane < NM)
+                        sA[wv][j * NM + lane] = aj[j];
+                    tk = 0.5 * tk;
                 }
+                wave_lds_sync();
+                double ft[KWA];
+                if (mx) {
+                    double f4p[4];
+                    f_trials_mixed<NM>(P, MT, wg, sA[wv], kw, C.shift, lane, f4p);
+#pragma unroll
+                    for (int j = 0; j < KWA; ++j)
+                        ft[j] = j < 4 ? f4p[j] : 0.0;
+                } else {
+                    double fk[KW];
+                    f_trials<NM, REF ? kFRef : kFp64, KW>(P, MT, wg, sA[wv], kw, 0.0, sW[wv],
+                                                          sE[wv], lane, fk);
+#pragma unroll
+                    for (int j = 0; j < KWA; ++j)
+                        ft[j] = j < KW ? fk[j] : 0.0;
+                }
+                // each wave judges its own points; the search then walks the verdicts in order
+#pragma unroll
+                for (int j = 0; j < KWA; ++j) {
+                    if (j < kw) {
+                        const bool ok = C.template accepts_at<REF>(ft[j], aj[j], tj[j]);
+                        if (lane == 0)
+                            sOk[wv * kw + j] = ok;
+                    }
+                }
+                __syncthreads();
+                bool accepted = false;
+                for (int k = 0; k < TW * kw; ++k) {
+                    if (sOk[k]) {
+                        accepted = true;
+                        break;
+                    }
+                    C.ls_reject(li);  // halve the step, or give up
+                    if (C.phase != kPhaseLS)
+                        break;
+                }
+                __syncthreads();  // sOk and sA are rewritten by the next round
+                if (!accepted)
+                    continue;
+            }
+            if (wv == 0) {
+                double fdir;
+                if (lane < NM)
+                    sA[0][lane] = C.a;
+                if constexpr (REF) {
+                    const RefEntries<NM> E(lane);
+                    double h[RefEntries<NM>::NS];
+#pragma unroll
+                    for (int s2 = 0; s2 < RefEntries<NM>::NS; ++s2)
+                        h[s2] = 0.0;
+                    double f = 0.0;
+                    for (int ch = 0; ch < nch; ++ch) {
+                        wave_lds_sync();
+                        for (int e = lane; e < kChunk * NM; e += kWave) {
+                            const int rr = e / NM, cc = e - rr * NM;
+                            sM[rr * NMS + cc] = Mg[(size_t)ch * kChunk * NM + e];
+                        }
+                        sW[0][lane] = wg[(size_t)ch * kChunk + lane];
+                        wave_lds_sync();
+                        chunk_ref<NM>(sM, sW[0], sE[0], sA[0], E, h, f,
+                                      min(kChunk, P.nq - ch * kChunk), lane);
+                    }
+                    wave_lds_sync();
+                    store_H_ref<NM>(E, h, sH, lane);
+                    fdir = f;
+                } else {
+                    d4 acc[1][NB];
+                    f4 acc32[1][NB];
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        acc[0][b] = d4{0.0, 0.0, 0.0, 0.0};
+                        acc32[0][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+                    }
+                    double fpart[1] = {0.0};
+                    double gacc[NT][4];
+                    float wmaxOwn = 0.0f;
+                    double fOwn = 0.0;
+                    const double shiftOwn = (lane >> 4) == 0 ? C.shift : 0.0;
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            gacc[t][k] = 0.0;
+                    ExpA<NM> ax;
+                    if (mx) {
+                        wave_lds_sync();  // sA[0]
+                        ax.load(sA[0], 1, lane);
+                    }
+                    double pre[NM];
+                    const int qo = mx ? lane : mfma_ordinate(lane);  // the lane's weight
+                    double prew = wg[qo];
+#pragma unroll
+                    for (int k = 0; k < NM; ++k)
+                        pre[k] = Mg[lane + kWave * k];
+                    for (int ch = 0; ch < nch; ++ch) {
+                        wave_lds_sync();
+#pragma unroll
+                        for (int k = 0; k < NM; ++k) {
+                            const int e = lane + kWave * k;
+                            const int rr = e / NM, cc = e - rr * NM;
+                            sM[rr * NMS + cc] = pre[k];
+                            if constexpr (MIXED)
+                                if (mx)
+                                    H16Chunk<NM>::put(sBh, rr, cc, pre[k]);
+                        }
+                        const double wl = prew;
+                        if (mx)
+                            sW[0][lane] = wl;  // the chunk's weights (qo = lane)
+                        wave_lds_sync();
+                        if (ch + 1 < nch) {
+                            const size_t base = (size_t)(ch + 1) * kChunk;
+                            prew = wg[base + qo];
+#pragma unroll
+                            for (int k = 0; k < NM; ++k)
+                                pre[k] = Mg[base * NM + lane + kWave * k];
+                        }
+                        if (mx)
+                            chunk_h16<NM, 1>(sM, sBh, sW[0], ax, shiftOwn, wmaxOwn, sWH, sWE64,
+                                             acc32, gacc, fOwn, lane);
+                        else
+                            chunk_mfma<NM, 1>(sM, wl, sA[0], acc, fpart, lane);
+                    }
+                    wave_lds_sync();
+                    if (mx) {
+                        const double es = exp(C.shift);
+                        fdir = __shfl(row16_sum(fOwn), 0) * es;
+                        const bool ok = h16_range_ok(__shfl(row16_max(wmaxOwn), 0));
+                        store_H32<NM>(acc32[0], es / kWScale, sH, lane);
+                        store_gm<NM>(gacc, 0, es, sG, lane);
+                        if (!ok && lane == 0)
+                            sH[0] = __builtin_nan("");
+                    } else {
+                        fdir = wave_sum(fpart[0]);
+                        store_H<NM>(acc[0], sH, lane);
+                    }
+                }
+                if (lane == 0)
+                    sF[0] = fdir;
+                ++sweeps;
             }
             __syncthreads();
-            bool accepted = false;
-            for (int k = 0; k < TW * kw; ++k) {
-                if (sOk[k]) {
-                    accepted = true;
-                    break;
-                }
-                C.ls_reject(li);  // halve the step, or give up
-                if (C.phase != kPhaseLS)
-                    break;
-            }
-            __syncthreads();  // sOk and sA are rewritten by the next round
-            if (!accepted)
-                continue;
+            done = C.template advance<REF>(P, sH, sF[0], MIXED ? sG : nullptr, sT[wv][row], li);
+            __syncthreads();  // sH and sF are rewritten by the next round
+            if (done)
+                break;
         }
-        if (wv == 0) {
-            double fdir;
-            if (lane < NM)
-                sA[0][lane] = C.a;
-            if constexpr (REF) {
-                const RefEntries<NM> E(lane);
-                double h[RefEntries<NM>::NS];
-#pragma unroll
-                for (int s2 = 0; s2 < RefEntries<NM>::NS; ++s2)
-                    h[s2] = 0.0;
-                double f = 0.0;
-                for (int ch = 0; ch < nch; ++ch) {
-                    wave_lds_sync();
-                    for (int e = lane; e < kChunk * NM; e += kWave) {
-                        const int rr = e / NM, cc = e - rr * NM;
-                        sM[rr * NMS + cc] = Mg[(size_t)ch * kChunk * NM + e];
-                    }
-                    sW[0][lane] = wg[(size_t)ch * kChunk + lane];
-                    wave_lds_sync();
-                    chunk_ref<NM>(sM, sW[0], sE[0], sA[0], E, h, f,
-                                  min(kChunk, P.nq - ch * kChunk), lane);
-                }
-                wave_lds_sync();
-                store_H_ref<NM>(E, h, sH, lane);
-                fdir = f;
+        if (wv == 0 && row == 0) {
+            if (lane == 0 && sweepCount)
+                atomicAdd(sweepCount, (unsigned long long)sweeps);
+            if (done) {
+                C.finish(done, c, u_g, alpha_g, stats, status, li);
             } else {
-                d4 acc[1][NB];
-                f4 acc32[1][NB];
-#pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    acc[0][b] = d4{0.0, 0.0, 0.0, 0.0};
-                    acc32[0][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-                }
-                double fpart[1] = {0.0};
-                double gacc[NT][4];
-                float wmaxOwn = 0.0f;
-                double fOwn = 0.0;
-                const double shiftOwn = (lane >> 4) == 0 ? C.shift : 0.0;
-#pragma unroll
-                for (int t = 0; t < NT; ++t)
-#pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        gacc[t][k] = 0.0;
-                ExpA<NM> ax;
-                if (mx) {
-                    wave_lds_sync();  // sA[0]
-                    ax.load(sA[0], 1, lane);
-                }
-                double pre[NM];
-                const int qo = mx ? lane : mfma_ordinate(lane);  // the lane's weight
-                double prew = wg[qo];
-#pragma unroll
-                for (int k = 0; k < NM; ++k)
-                    pre[k] = Mg[lane + kWave * k];
-                for (int ch = 0; ch < nch; ++ch) {
-                    wave_lds_sync();
-#pragma unroll
-                    for (int k = 0; k < NM; ++k) {
-                        const int e = lane + kWave * k;
-                        const int rr = e / NM, cc = e - rr * NM;
-                        sM[rr * NMS + cc] = pre[k];
-                        if constexpr (MIXED)
-                            if (mx)
-                                H16Chunk<NM>::put(sBh, rr, cc, pre[k]);
-                    }
-                    const double wl = prew;
-                    if (mx)
-                        sW[0][lane] = wl;  // the chunk's weights (qo = lane)
-                    wave_lds_sync();
-                    if (ch + 1 < nch) {
-                        const size_t base = (size_t)(ch + 1) * kChunk;
-                        prew = wg[base + qo];
-#pragma unroll
-                        for (int k = 0; k < NM; ++k)
-                            pre[k] = Mg[base * NM + lane + kWave * k];
-                    }
-                    if (mx)
-                        chunk_h16<NM, 1>(sM, sBh, sW[0], ax, shiftOwn, wmaxOwn, sWH, sWE64,
-                                         acc32, gacc, fOwn, lane);
-                    else
-                        chunk_mfma<NM, 1>(sM, wl, sA[0], acc, fpart, lane);
-                }
-                wave_lds_sync();
-                if (mx) {
-                    const double es = exp(C.shift);
-                    fdir = __shfl(row16_sum(fOwn), 0) * es;
-                    const bool ok = h16_range_ok(__shfl(row16_max(wmaxOwn), 0));
-                    store_H32<NM>(acc32[0], es / kWScale, sH, lane);
-                    store_gm<NM>(gacc, 0, es, sG, lane);
-                    if (!ok && lane == 0)
-                        sH[0] = __builtin_nan("");
-                } else {
-                    fdir = wave_sum(fpart[0]);
-                    store_H<NM>(acc[0], sH, lane);
-                }
+                C.store(S, c, li);
+                if (lane == 0)
+                    list_append(Ln, C.mixed(P), c);
             }
-            if (lane == 0)
-                sF[0] = fdir;
-            ++sweeps;
         }
-        __syncthreads();
-        done = C.template advance<REF>(P, sH, sF[0], MIXED ? sG : nullptr, sT[wv][row], li);
-        __syncthreads();  // sH and sF are rewritten by the next round
-        if (done)
-            break;
-    }
-    if (wv != 0 || row != 0)
-        return;
-    if (lane == 0 && sweepCount)
-        atomicAdd(sweepCount, (unsigned long long)sweeps);
-    if (done) {
-        C.finish(done, c, u_g, alpha_g, stats, status, li);
-    } else {
-        C.store(S, c, li);
-        if (lane == 0)
-            list_append(Ln, C.mixed(P), c);
+        __syncthreads();  // the block's LDS serves the next cell
     }
 }
 
@@ -2829,6 +2831,30 @@ __global__ __launch_bounds__(kBlock) void k_dense_solve(int64_t n, const double 
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
+
+// Pass control of the asynchronous schedule (one block): the pass's bulk kernels read their
+// list lengths from eff.  While more than skipLE cells are listed, eff is the list's own
+// lengths; at or below it the bulk kernels see empty lists and the list carries over to the
+// next generation unchanged, for the tail kernels.
+__global__ __launch_bounds__(256) void k_gate(Lists Lc, Lists Ln, int *eff, int skipLE)
+{
+    const int nm = *Lc.nm, nf = *Lc.nf;
+    const bool skip = nm + nf <= skipLE;
+    if (threadIdx.x == 0) {
+        eff[0] = skip ? 0 : nm;
+        eff[1] = skip ? 0 : nf;
+    }
+    if (!skip)
+        return;
+    for (int i = threadIdx.x; i < nm; i += blockDim.x)
+        Ln.Wm[i] = Lc.Wm[i];
+    for (int i = threadIdx.x; i < nf; i += blockDim.x)
+        Ln.Wf[i] = Lc.Wf[i];
+    if (threadIdx.x == 0) {
+        *Ln.nm = nm;
+        *Ln.nf = nf;
+    }
+}
 __global__ void k_set_int2(int *p, int v)
 {
     p[0] = v;
@@ -2952,7 +2978,120 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
         if (ev)
             (void)hipEventRecord(ev[1], s);
     }
+    // one bulk pass over the listed cells of Lc (list lengths read through Lc.nm / Lc.nf)
+    auto bulk_pass = [&](const Lists &Lc, const Lists &Ln) -> hipError_t {
+        hipEvent_t *ev = prof ? take_events(ctx, 4, 0) : nullptr;
+        if (prof && !ev)
+            return hipErrorOutOfMemory;
+        if (ev)
+            (void)hipEventRecord(ev[0], s);
+        if (mixed) {
+            bool two = true;
+            if constexpr (NM <= 16) {
+                two = ctx->tune.mixedWaves == 2;
+                if (!two)
+                    hipLaunchKernelGGL((k_sweep_mixed<NM, 3>), dim3(grid_cap(ctx, 8)),
+                                       dim3(kBlock), 0, s, P, ctx->d_M, ctx->d_wpad, S, Lc.Wm,
+                                       Lc.nm);
+            }
+            if (two)
+                hipLaunchKernelGGL((k_sweep_mixed<NM, 2>), dim3(grid_cap(ctx, 8)), dim3(kBlock),
+                                   0, s, P, ctx->d_M, ctx->d_wpad, S, Lc.Wm, Lc.nm);
+        }
+        if (ev)
+            (void)hipEventRecord(ev[1], s);
+        if (ref)
+            hipLaunchKernelGGL(k_sweep_ref<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P,
+                               ctx->d_M, ctx->d_wpad, S, Lc.Wf, Lc.nf);
+        else
+            hipLaunchKernelGGL(k_sweep<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P,
+                               ctx->d_M, ctx->d_wpad, S, Lc.Wf, Lc.nf);
+        if (ev)
+            (void)hipEventRecord(ev[2], s);
+        if (ref)
+            hipLaunchKernelGGL((k_step<NM, true>), dim3(grid_cap(ctx, 16)), dim3(kBlock), 0, s,
+                               P, S, Lc, Ln, u, alpha, stats, ctx->d_status, cntBulk);
+        else
+            hipLaunchKernelGGL((k_step<NM, false>), dim3(grid_cap(ctx, 16)), dim3(kBlock), 0,
+                               s, P, S, Lc, Ln, u, alpha, stats, ctx->d_status, cntBulk);
+        if (ev)
+            (void)hipEventRecord(ev[3], s);
+        return hipSuccess;
+    };
+    // one launch of the fused tail kernel: `grid` blocks stride over Lc's cells, K rounds each;
+    // wide: four waves per cell
+    auto tail_launch = [&](const Lists &Lc, const Lists &Ln, unsigned grid, int K,
+                           bool wide) -> hipError_t {
+        hipEvent_t *ev = prof ? take_events(ctx, 2, 1) : nullptr;
+        if (prof && !ev)
+            return hipErrorOutOfMemory;
+        if (ev)
+            (void)hipEventRecord(ev[0], s);
+#define C3D_TAIL(R, X, TW)                                                                   \
+    hipLaunchKernelGGL((k_tail<NM, R, X, TW>), dim3(grid), dim3(TW * kWave), 0, s, P, ctx->d_M,  \
+                       ctx->d_MT, ctx->d_wpad, S, Lc, Ln, u, alpha, stats, ctx->d_status, K,   \
+                       cntTail)
+        if (ref && wide)
+            C3D_TAIL(true, false, 4);
+        else if (ref)
+            C3D_TAIL(true, false, 1);
+        else if (mixed && wide)
+            C3D_TAIL(false, true, 4);
+        else if (mixed)
+            C3D_TAIL(false, true, 1);
+        else if (wide)
+            C3D_TAIL(false, false, 4);
+        else
+            C3D_TAIL(false, false, 1);
+#undef C3D_TAIL
+        if (ev)
+            (void)hipEventRecord(ev[1], s);
+        ++ctx->lastTailLaunches;
+        return hipSuccess;
+    };
     int cur = 0;
+    if (ctx->asyncPasses > 0) {
+        // Asynchronous schedule (no host synchronisation; capturable in a graph): a fixed number
+        // of bulk passes, each gated on the device, which stop sweeping once at most tailCells
+        // cells are listed; then one narrow tail launch of tailPasses rounds per cell and one
+        // wide launch for the rest of the pass budget.  A cell's operations are the ones the
+        // adaptive schedule performs, so the results are the same.
+        int *eff = counts + 4;
+        const Lists Le0{Lg[0].Wm, Lg[0].Wf, eff, eff + 1}, Le1{Lg[1].Wm, Lg[1].Wf, eff, eff + 1};
+        const int B = std::min(ctx->asyncPasses, maxPasses);
+        for (int p = 0; p < B; ++p) {
+            const Lists &Lc = Lg[cur], &Ln = Lg[1 - cur];
+            hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, s, Ln.nm, 0);
+            hipLaunchKernelGGL(k_gate, dim3(1), dim3(256), 0, s, Lc, Ln, eff, ctx->tailCells);
+            e = bulk_pass(cur == 0 ? Le0 : Le1, Ln);
+            if (e != hipSuccess)
+                return e;
+            cur = 1 - cur;
+        }
+        int rest = maxPasses - B;
+        const int K1 = std::min(ctx->tailPasses, rest);
+        if (K1 > 0) {
+            hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, s, Lg[1 - cur].nm, 0);
+            e = tail_launch(Lg[cur], Lg[1 - cur], (unsigned)std::max(ctx->tailCells, 1024), K1,
+                            false);
+            if (e != hipSuccess)
+                return e;
+            cur = 1 - cur;
+            rest -= K1;
+        }
+        if (rest > 0) {
+            hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, s, Lg[1 - cur].nm, 0);
+            e = tail_launch(Lg[cur], Lg[1 - cur], (unsigned)std::max(ctx->tailWideCells, 256),
+                            rest, true);
+            if (e != hipSuccess)
+                return e;
+            cur = 1 - cur;
+        }
+        ctx->lastPasses = maxPasses;
+        hipLaunchKernelGGL(k_budget<NM>, dim3(64), dim3(256), 0, s, Lg[cur], stats,
+                           ctx->d_status);
+        return hipGetLastError();
+    }
     int64_t left = (int64_t)n;  // upper bound on the listed cells until the first read-back
     int pass = 0;
     while (pass < maxPasses) {
@@ -2961,72 +3100,15 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
         const bool tail = ctx->tailCells > 0 && left <= ctx->tailCells;
         if (tail) {
             const int K = std::min(ctx->tailPasses, maxPasses - pass);
-            hipEvent_t *ev = prof ? take_events(ctx, 2, 1) : nullptr;
-            if (prof && !ev)
-                return hipErrorOutOfMemory;
-            if (ev)
-                (void)hipEventRecord(ev[0], s);
-#define C3D_TAIL(R, X, TW)                                                                 \
-    hipLaunchKernelGGL((k_tail<NM, R, X, TW>), dim3((unsigned)left), dim3(TW * kWave), 0, s, P, \
-                       ctx->d_M, ctx->d_MT, ctx->d_wpad, S, Lc, Ln, u, alpha, stats,          \
-                       ctx->d_status, K,                                                      \
-                       cntTail)
             // few cells: four waves per cell settle up to four line-search trials a round
-            const bool wide = left <= ctx->tailWideCells;
-            if (ref && wide)
-                C3D_TAIL(true, false, 4);
-            else if (ref)
-                C3D_TAIL(true, false, 1);
-            else if (mixed && wide)
-                C3D_TAIL(false, true, 4);
-            else if (mixed)
-                C3D_TAIL(false, true, 1);
-            else if (wide)
-                C3D_TAIL(false, false, 4);
-            else
-                C3D_TAIL(false, false, 1);
-#undef C3D_TAIL
-            if (ev)
-                (void)hipEventRecord(ev[1], s);
+            e = tail_launch(Lc, Ln, (unsigned)left, K, left <= ctx->tailWideCells);
+            if (e != hipSuccess)
+                return e;
             pass += K;
-            ++ctx->lastTailLaunches;
         } else {
-            hipEvent_t *ev = prof ? take_events(ctx, 4, 0) : nullptr;
-            if (prof && !ev)
-                return hipErrorOutOfMemory;
-            if (ev)
-                (void)hipEventRecord(ev[0], s);
-            if (mixed) {
-                bool two = true;
-                if constexpr (NM <= 16) {
-                    two = ctx->tune.mixedWaves == 2;
-                    if (!two)
-                        hipLaunchKernelGGL((k_sweep_mixed<NM, 3>), dim3(grid_cap(ctx, 8)),
-                                           dim3(kBlock), 0, s, P, ctx->d_M, ctx->d_wpad, S, Lc.Wm,
-                                           Lc.nm);
-                }
-                if (two)
-                    hipLaunchKernelGGL((k_sweep_mixed<NM, 2>), dim3(grid_cap(ctx, 8)), dim3(kBlock),
-                                       0, s, P, ctx->d_M, ctx->d_wpad, S, Lc.Wm, Lc.nm);
-            }
-            if (ev)
-                (void)hipEventRecord(ev[1], s);
-            if (ref)
-                hipLaunchKernelGGL(k_sweep_ref<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s,
-                                   P, ctx->d_M, ctx->d_wpad, S, Lc.Wf, Lc.nf);
-            else
-                hipLaunchKernelGGL(k_sweep<NM>, dim3(grid_cap(ctx, 8)), dim3(kBlock), 0, s, P,
-                                   ctx->d_M, ctx->d_wpad, S, Lc.Wf, Lc.nf);
-            if (ev)
-                (void)hipEventRecord(ev[2], s);
-            if (ref)
-                hipLaunchKernelGGL((k_step<NM, true>), dim3(grid_cap(ctx, 16)), dim3(kBlock), 0,
-                                   s, P, S, Lc, Ln, u, alpha, stats, ctx->d_status, cntBulk);
-            else
-                hipLaunchKernelGGL((k_step<NM, false>), dim3(grid_cap(ctx, 16)), dim3(kBlock),
-                                   0, s, P, S, Lc, Ln, u, alpha, stats, ctx->d_status, cntBulk);
-            if (ev)
-                (void)hipEventRecord(ev[3], s);
+            e = bulk_pass(Lc, Ln);
+            if (e != hipSuccess)
+                return e;
             ++pass;
         }
         ctx->lastPasses = pass;

@@ -259,10 +259,11 @@ int c3d_set_tail(c3d_ctx ctx, int max_cells, int passes_per_launch);
 
 /* Once at most max_cells cells remain, the fused kernel runs four wavefronts per cell that
  * evaluate f at the next steps of the backtracking line search (t, t/2, t/4, ...): one step
- * per wavefront in a search's first round, then 8 (nm <= 9), 2 (nm = 16) or 1 (nm = 25) each,
- * so a round settles up to 4-32 trials of the sequential search, with the same accept/reject
- * decisions and results.  0 keeps one wavefront per cell.  Default 512.  No counterpart in
- * the reference. */
+ * per wavefront in a search's first round, then 8 (nm <= 9), 2 (nm = 16) or 1 (nm = 25) each
+ * (in the mixed phase 4, or 2 at nm = 25: the rows of the exponent products), so a round
+ * settles up to 4-32 trials of the sequential search, with the same accept/reject decisions
+ * and results.  0 keeps one wavefront per cell.  Default 512.  No counterpart in the
+ * reference. */
 int c3d_set_tail_wide(c3d_ctx ctx, int max_cells);
 
 /* Launch and hand-over parameters of the closure kernels, for tuning per (N, Q); results do not
@@ -271,9 +272,24 @@ int c3d_set_tail_wide(c3d_ctx ctx, int max_cells);
  *                                 M_4 sweep always runs at 2)
  *   C3D_LAUNCH_MIXED_FINISH_COND  10 * log10 of the condition indicator above which a mixed
  *                                 cell with ||g|| <= switch_tol continues on FP64 Hessians
- *                                 (default 50, i.e. 1e5); changes which cells take that path */
+ *                                 (default 50, i.e. 1e5); changes which cells take that path
+ *   C3D_LAUNCH_ASYNC              0 (default): the closure adapts its schedule to the number of
+ *                                 unconverged cells, which it reads back on the host now and
+ *                                 then (the call returns once the cells are done).  1..10000:
+ *                                 the asynchronous schedule, with no host synchronisation: that
+ *                                 many bulk passes, each gated on the device (a pass sweeps only
+ *                                 while more than the tail threshold of c3d_set_tail are
+ *                                 listed), then one fused-kernel launch of passes_per_launch
+ *                                 rounds per cell and one four-wave launch for the rest of the
+ *                                 pass budget.  The call returns once the work is queued on the
+ *                                 stream, and c3d_closure, c3d_update and c3d_closure_noalloc can
+ *                                 then be captured in a HIP graph (after one eager call has sized
+ *                                 the buffers).  Per-cell results are the same in both
+ *                                 schedules; cells still listed after the bulk passes go to the
+ *                                 fused kernel whatever their number. */
 #define C3D_LAUNCH_MIXED_WAVES 1
 #define C3D_LAUNCH_MIXED_FINISH_COND 2
+#define C3D_LAUNCH_ASYNC 3
 int c3d_set_launch(c3d_ctx ctx, int param, int value);
 
 /* H(alpha) for ncells cells (solveHMn semantics), device pointers, hs [ncells][nm][nm].

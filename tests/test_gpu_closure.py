@@ -28,12 +28,23 @@ def _tables(O, N, Q):
 
 # worklist strategies of c3d_closure: launch pairs only, the fused tail kernel only (few
 # passes per launch, so cells resume across launches) in its four-wave form (speculative
-# line-search trials) or its one-wave form, and switches part way through
-TAILS = ("bulk", "fused", "fused1", "switch")
+# line-search trials) or its one-wave form, and switches part way through; and the
+# asynchronous schedule, with few bulk passes (most cells finish in the fused kernels, whose
+# blocks then stride over more cells than there are blocks) or with many (the device-side gate
+# turns the later passes off)
+TAILS = ("bulk", "fused", "fused1", "switch", "async", "async_gate")
 
 
 def _set_tail(ctx, tail, n):
-    if tail == "bulk":
+    import sys
+    P = sys.modules["c3d_hip"]
+    if tail == "async":
+        ctx.set_tail(max(1, n // 5), 3, wide_cells=max(1, n // 10))
+        ctx.set_launch(P.LAUNCH_ASYNC, 6)
+    elif tail == "async_gate":
+        ctx.set_tail(max(1, n // 5), 5, wide_cells=max(1, n // 10))
+        ctx.set_launch(P.LAUNCH_ASYNC, 100)
+    elif tail == "bulk":
         ctx.set_tail(0)
     elif tail == "fused":
         ctx.set_tail(1 << 30, 7, wide_cells=1 << 30)
