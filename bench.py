@@ -135,6 +135,88 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _short_kernel(name):
+    import re
+    m = re.search(r"(k_\w+(<[^(]*>)?|__amd_rocclr_\w+)", name)
+    return m.group(1) if m else name[:48]
+
+
+def summarize_profile(paths, steps, warmup, write_traffic=None):
+    """Per-kernel figures of the timed solves of a default closure bench run, from rocprofv3
+    CSV output (--output-format csv): a kernel trace (run_kernel_trace.csv: duration per
+    dispatch) or a counter collection (run_counter_collection.csv: --pmc).  Dispatches are
+    split into solves at each k_init launch (one per closure call): the warm start's training
+    solve, then `warmup` + `steps` solves of the mixed+nn path, then the same of the FP64 path;
+    the timed solves are averaged.  FETCH_SIZE / WRITE_SIZE (KiB) are calibrated on the
+    timed loop's 128 MiB input copies (u.copy_(u0)); with write_traffic the bytes per dispatch
+    go to that JSON file (read by the bench's roofline)."""
+    import csv
+    import collections
+    out, traffic, calib = [], {}, {}
+    for path in paths:
+        for fn in sorted(os.listdir(path)):
+            if not fn.endswith(".csv") or not ("kernel_trace" in fn or "counter_collection" in fn):
+                continue
+            rows = list(csv.DictReader(open(os.path.join(path, fn))))
+            disp = collections.OrderedDict()
+            for r in rows:
+                d = int(r.get("Dispatch_Id") or r.get("Correlation_Id"))
+                e = disp.setdefault(d, {"name": _short_kernel(r["Kernel_Name"]), "c": {}})
+                e["ns"] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+                if "Counter_Name" in r:
+                    e["c"][r["Counter_Name"]] = (e["c"].get(r["Counter_Name"], 0.0)
+                                                 + float(r["Counter_Value"]))
+            ds = [disp[k] for k in sorted(disp)]
+            starts = [i for i, e in enumerate(ds) if e["name"].startswith("k_init<")]
+            segs = [ds[i:(starts[j + 1] if j + 1 < len(starts) else len(ds))]
+                    for j, i in enumerate(starts)]
+            if len(segs) < 1 + 2 * (warmup + steps):
+                raise SystemExit(f"{path}/{fn}: {len(segs)} solves, expected "
+                                 f"{1 + 2 * (warmup + steps)}")
+            timed = {"mixed+nn": segs[1 + warmup:1 + warmup + steps],
+                     "fp64": segs[1 + 2 * warmup + steps:1 + 2 * (warmup + steps)]}
+            for cn in ("FETCH_SIZE", "WRITE_SIZE"):
+                big = [e["c"][cn] for e in ds if e["name"] == "__amd_rocclr_copyBuffer"
+                       and e["c"].get(cn, 0) > 1e4]
+                if big:
+                    calib[cn] = 131072.0 / sorted(big)[len(big) // 2]
+            out.append(f"### {path}/{fn}\n")
+            for label, sel in timed.items():
+                acc = collections.defaultdict(collections.Counter)
+                cnt, ns = collections.Counter(), collections.Counter()
+                for seg in sel:
+                    for e in seg:
+                        cnt[e["name"]] += 1
+                        ns[e["name"]] += e["ns"]
+                        acc[e["name"]].update(e["c"])
+                cols = sorted({c for v in acc.values() for c in v})
+                out.append(f"{label}, {len(sel)} timed solves (per solve):\n")
+                out.append("| kernel | dispatches | ms | avg us per dispatch |"
+                           + "".join(f" {c} |" for c in cols))
+                out.append("|---|---|---|---|" + "---|" * len(cols))
+                for k in sorted(ns, key=lambda k: -ns[k]):
+                    if ns[k] / len(sel) < 1e4 and not acc[k]:
+                        continue
+                    out.append(f"| {k} | {cnt[k] / len(sel):.1f} | {ns[k] / len(sel) / 1e6:.3f} | "
+                               f"{ns[k] / cnt[k] / 1e3:.1f} |"
+                               + "".join(f" {acc[k][c] / len(sel):.4g} |" for c in cols))
+                    for cn, key in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write")):
+                        if cn in acc[k] and cn in calib:
+                            t = traffic.setdefault(label, {}).setdefault(k, {})
+                            t[key + "_bytes_per_dispatch"] = (acc[k][cn] / cnt[k] * calib[cn]
+                                                              * 1024.0)
+                            t["dispatches_per_solve"] = cnt[k] / len(sel)
+                out.append("")
+    if calib:
+        out.append("Calibration on the 128 MiB input copies: " + ", ".join(
+            f"{k} x {v:.4f}" for k, v in calib.items()) + " (KiB per KiB counted)")
+    print("\n".join(out))
+    if write_traffic and traffic:
+        json.dump({"calibration": calib, "paths": traffic,
+                   "command": "python3 bench.py --summarize-profile <dirs> --write-traffic ..."},
+                  open(write_traffic, "w"), indent=1, sort_keys=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,8 +233,16 @@ def main():
                     help="timed solves of the reference-order FP64 path (0: skip it)")
     ap.add_argument("--no-tuning", action="store_true",
                     help="ignore c3d-2015_amd/tuning.json (the library's default launch shapes)")
+    ap.add_argument("--summarize-profile", nargs="*", default=None, metavar="DIR",
+                    help="summarise rocprofv3 CSV output of a default closure bench run (run "
+                         "with the same --steps/--warmup) instead of running; no GPU needed")
+    ap.add_argument("--write-traffic", default=None,
+                    help="with --summarize-profile: write FETCH/WRITE bytes per dispatch here")
     ap.add_argument("--async-passes", type=int, default=0,
                     help="> 0: the closure's asynchronous schedule with this many bulk passes")
+    ap.add_argument("--graph", action="store_true",
+                    help="step workload: capture one SSP-RK2 step in a HIP graph and time its "
+                         "replays (asynchronous schedule; --async-passes, default 48)")
     ap.add_argument("--mixed-waves", type=int, choices=(2, 3), default=None,
                     help="waves per SIMD of the mixed sweep (default: the library's)")
     ap.add_argument("--finish-cond", type=int, default=None,
@@ -178,6 +268,9 @@ def main():
                          "reference-order FP64 one beside the FP64 one from the same state "
                          "and reports both differences from it after every step")
     args = ap.parse_args()
+    if args.summarize_profile is not None:
+        summarize_profile(args.summarize_profile, args.steps, args.warmup, args.write_traffic)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (closure batches split "
               "over cells with no exchange; a time-stepped grid would split into slabs with a "
@@ -326,24 +419,32 @@ def main():
 
     # HBM bytes per dispatch of the sweep kernels, from the committed rocprofv3 PMC summary
     # (FETCH_SIZE / WRITE_SIZE runs calibrated on a copy of known size)
+    # HBM bytes per dispatch of the sweep kernels in the timed solves of each path, from the
+    # committed rocprofv3 PMC summary (FETCH_SIZE / WRITE_SIZE runs calibrated on the timed
+    # loop's input copies)
     traffic = {}
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tpath):
-        for k, v in json.load(open(tpath))["kernels"].items():
-            traffic[k] = round(v["fetch_bytes_per_dispatch"] + v["write_bytes_per_dispatch"])
+        for pth, ks in json.load(open(tpath)).get("paths", {}).items():
+            for k, v in ks.items():
+                if "fetch_bytes_per_dispatch" in v and "write_bytes_per_dispatch" in v:
+                    traffic[(pth, k)] = round(v["fetch_bytes_per_dispatch"]
+                                              + v["write_bytes_per_dispatch"])
 
-    def with_traffic(r):
+    def with_traffic(r, pth):
         key = r["kernel"].split(" (")[0]
-        r["traffic"] = traffic.get(key)
+        r["traffic"] = traffic.get((pth, key))
         if r["traffic"] is not None:
-            r["traffic_source"] = "profiles/closure_C1_pmc_traffic.md (avg bytes per dispatch)"
+            r["traffic_source"] = ("profiles/closure_C1_pmc.md (bytes per dispatch, timed "
+                                   f"{pth} solves)")
         return r
 
     mixed_main = args.path.startswith("mixed")
-    roofline = with_traffic(sweep_roofline(main_r["prof"], mixed_main))
+    roofline = with_traffic(sweep_roofline(main_r["prof"], mixed_main), args.path)
     if mixed_main:
-        roofline["fp64_sweep_in_path"] = with_traffic(sweep_roofline(main_r["prof"], False))
-    roofline["fp64_path_sweep"] = with_traffic(sweep_roofline(base_r["prof"], False))
+        roofline["fp64_sweep_in_path"] = with_traffic(sweep_roofline(main_r["prof"], False),
+                                                      args.path)
+    roofline["fp64_path_sweep"] = with_traffic(sweep_roofline(base_r["prof"], False), "fp64")
 
     def kernel_ms(r):
         p = r["prof"]
@@ -493,14 +594,26 @@ def step_workload(args, P, torch, dev):
 
     mixed = args.step_precision == "mixed"
     errors, fp64_ms = [], None
+    graph = None
+    if args.graph or args.async_passes > 0:
+        ctx.set_launch(P.LAUNCH_ASYNC, args.async_passes or 48)
     if not mixed:
-        for _ in range(args.warmup):
+        for _ in range(max(1, args.warmup)):
             step(u, a)
         rc = ctx.sync()
+        if args.graph:
+            # the eager steps sized the context's buffers; the capture records one step (the
+            # captured work does not run) and each replay advances the state by one step
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                step(u, a)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            step(u, a)
+            if graph is not None:
+                graph.replay()
+            else:
+                step(u, a)
         rc = rc or ctx.sync()
         torch.cuda.synchronize()
         ms = 1e3 * (time.perf_counter() - t0) / args.steps
@@ -622,11 +735,15 @@ def step_workload(args, P, torch, dev):
         "value": round(ni / (ms / 1e3), 1), "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64" if mixed else "f64",
+        "dtype": "f16+f32+f64" if mixed else "f64",
         "data": "synthetic",
         "config": {"workload": f"M_{N} time stepping, {n}^3 periodic grid, Q={Q} ({nq} "
                                "ordinates), pulse sigma 0.05 + 1% noise, theta 1, CFL 0.5, "
                                f"closure tol 1e-8, {prec}",
+                   "step_launch": ("one captured HIP graph per step, replayed" if graph is not None
+                                   else "eager c3d_update calls"),
+                   "closure_schedule": ("asynchronous" if args.graph or args.async_passes > 0
+                                        else "adaptive"),
                    "grid": n, "moment_order": N, "quad_order": Q, "dt": dt, "status": rc,
                    "closure_iters_mean": round(float(st["totalIter"].mean()), 3),
                    "closure_iters_max": int(st["totalIter"].max())},
@@ -654,11 +771,13 @@ def orders_workload(args, P, torch, dev):
     """2^20 independent cells, a quarter each of M_1, M_2, M_3, M_4 (Q = 24), alpha* as in the
     metric's batch.  Each order has its own context and warm-start network (trained on a
     quarter of --train-pairs solver pairs of that order); one step solves the four quarters,
-    mixed+nn, then the same with the plain FP64 path.  With --orders-streams 4 (default) the
-    four solves run concurrently, each on its own stream driven by its own host thread (a
-    closure call waits on its stream between passes, so one host thread would serialise
-    them), so one order's latency-bound tail overlaps the others' bulk passes; with 1 they
-    run back to back on one stream."""
+    mixed+nn, then the same with the plain FP64 path, each order with its tuned launch shapes.
+    With --orders-streams 4 (default) the four solves run concurrently, each on its own
+    stream, so one order's latency-bound tail overlaps the others' bulk passes: with the
+    asynchronous closure schedule (--async-passes > 0) one host thread queues all four (a call
+    returns once its work is queued); otherwise each stream has a host thread of its own (the
+    adaptive schedule waits on its stream between passes).  With 1 they run back to back on
+    one stream."""
     Q, n = 24, args.cells
     opts = P.closure_opts(maxIter=100, tol=1e-8, tolGamma=1.1, condHMax=1e10)
     parts = []
@@ -688,7 +807,9 @@ def orders_workload(args, P, torch, dev):
     concurrent = args.orders_streams > 1
     for p in parts:
         p["stream"] = torch.cuda.Stream() if concurrent else torch.cuda.current_stream()
-    pool = ThreadPoolExecutor(max_workers=len(parts)) if concurrent else None
+        p["ctx"].set_launch(P.LAUNCH_ASYNC, args.async_passes)
+    pool = (ThreadPoolExecutor(max_workers=len(parts))
+            if concurrent and args.async_passes == 0 else None)
 
     def solve(p):
         with torch.cuda.stream(p["stream"]):
@@ -700,6 +821,9 @@ def orders_workload(args, P, torch, dev):
         for p in parts:
             p["ctx"].set_precision(P.PREC_MIXED if path == "mixed+nn" else P.PREC_FP64)
             p["ctx"].set_warmstart(path == "mixed+nn")
+            p["tune"] = None
+            if not args.no_tuning:
+                p["tune"] = p["ctx"].apply_tuning("mixed" if path == "mixed+nn" else "fp64")
 
         def step():
             if pool is None:
@@ -713,10 +837,15 @@ def orders_workload(args, P, torch, dev):
             step()
         for p in parts:
             p["ctx"].sync(p["stream"])
+            p["ctx"].set_profiling(True)
+            p["prof"] = {}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             step()
+            for p in parts:  # HIP-event spans on each order's stream (waits for the stream)
+                for k, v in p["ctx"].closure_profile().items():
+                    p["prof"][k] = p["prof"].get(k, 0) + v / steps
         rc = 0
         for p in parts:
             rc = rc or p["ctx"].sync(p["stream"])
@@ -724,8 +853,19 @@ def orders_workload(args, P, torch, dev):
         ms = 1e3 * (time.perf_counter() - t0) / steps
         per = []
         for p in parts:
+            p["ctx"].set_profiling(False)
             st = P.stats_to_numpy(p["st"])
+            pr = p["prof"]
+            mixed_k = path == "mixed+nn"
+            sweeps = pr["mixed_cell_sweeps"] if mixed_k else pr["cell_sweeps"]
+            kms = pr["mixed_sweep_ms"] if mixed_k else pr["sweep_ms"]
+            nm_, nq_ = p["ctx"].nm, p["ctx"].nq
             per.append({"N": p["N"], "iters_mean": round(float(st["totalIter"].mean()), 3),
+                        "sweep_kernel": (f"k_sweep_mixed<{nm_}, "
+                                         f"{2 if nm_ > 16 else (p['tune'] or {}).get('mixed_waves', 3)}>"
+                                         if mixed_k else f"k_sweep<{nm_}>"),
+                        "sweep_ms": round(kms, 3), "cell_sweeps": int(sweeps),
+                        "flops_per_cell_sweep": 2 * nq_ * nm_ * nm_,
                         "fallback_frac": round(float(((st["flags"] & P.FLAG_FP64_FALLBACK) != 0)
                                                      .mean()), 5),
                         "u": p["u"].cpu().numpy(), "reg": st["regIndex"].copy()})
@@ -734,6 +874,36 @@ def orders_workload(args, P, torch, dev):
 
     ms_m, rc_m, per_m = run("mixed+nn", args.steps, args.warmup)
     ms_f, rc_f, per_f = run("fp64", args.steps, args.warmup)
+    # roofline: the order whose sweep kernel takes the longest, on its matrix-core peak
+    dom = max(per_m, key=lambda r: r["sweep_ms"])
+    ach = dom["flops_per_cell_sweep"] * dom["cell_sweeps"] / (dom["sweep_ms"] / 1e3) / 1e12
+    roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": FP32_MFMA_PEAK,
+            "unit": "TFLOP/s", "frac": round(ach / FP32_MFMA_PEAK, 4), "traffic": None,
+            "kernel": dom["sweep_kernel"] + f" (M_{dom['N']} quarter; split FP16 Hessian, FP64 "
+                      "gradient; peak: the FP32 matrix rate, the precision class of the "
+                      "Hessian; HIP-event spans overlap other orders' kernels)",
+            "per_order": [{k: r[k] for k in ("N", "sweep_kernel", "sweep_ms", "cell_sweeps",
+                                                "flops_per_cell_sweep")} for r in per_m]}
+    # CPU baseline: the oracle on the first cells of each quarter
+    cpu = None
+    if args.cpu_sample > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        info = host_cpu_info()
+        m = max(1, min(args.cpu_sample // 4, n // 4))
+        tot = 0.0
+        for p in parts:
+            T = O.Tables(p["N"], Q)
+            uh = p["u0"][:m].cpu().numpy().copy()
+            ah = p["a0"][:m].cpu().numpy().copy()
+            t0 = time.perf_counter()
+            O.closure(T, uh, ah, O.opts(), info["threads"])
+            tot += time.perf_counter() - t0
+        cpu = {"value": round(4 * m / tot, 1), "unit": "cell-updates/s",
+               "cores": info["threads"], "kind": "port",
+               "sample": f"first {m} cells of each order's quarter: oracle (C, -O3, OpenMP "
+                         f"schedule(dynamic), plain FP64) on {info['threads']} threads: "
+                         f"{tot:.2f} s", **info}
     orders = []
     for pm, pf in zip(per_m, per_f):
         du = np.abs(pm["u"] - pf["u"]).max(axis=1) / np.abs(pf["u"][:, 0])
@@ -745,15 +915,18 @@ def orders_workload(args, P, torch, dev):
         "metric": "M_N cell-updates/s at ||grad h||<=1e-8, orders 1-4 mixed, mixed+NN, one GPU",
         "value": round(n / (ms_m / 1e3), 1), "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_m, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-        "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16+f32+f64", "data": "synthetic",
         "config": {"workload": f"closure-only batch: {n} cells, a quarter each of M_1..M_4, "
                                "Q=24, alpha*~N(0,s^2), s~U[0.5,8]",
-                   "orders_run": "concurrently, one stream and host thread per order"
-                                 if concurrent else "back to back on one stream",
+                   "orders_run": ("back to back on one stream" if not concurrent else
+                                  "concurrently, one stream per order, queued by one host "
+                                  f"thread (asynchronous schedule, {args.async_passes} bulk "
+                                  "passes)" if args.async_passes > 0 else
+                                  "concurrently, one stream and host thread per order"),
                    "status": rc_m, "fp64_ms_per_step": round(ms_f, 3), "fp64_status": rc_f,
                    "speedup_vs_fp64": round(ms_f / ms_m, 3), "orders": orders},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
