@@ -1283,77 +1283,73 @@ __device__ __forceinline__ void exp_arg4(const ExpA<NM> &A, BF b, double (&x)[4]
             x[g] = __builtin_amdgcn_mfma_f64_4x4x4f64(A.a[s], b(g, s), x[g], 0, 0, 0);
 }
 
-// sWH holds, per cell c, the FP16 halves of the scaled weights: hi at [2 c][q], lo at
-// [2 c + 1][q] (kChunk halves each).  Lane L works on cell L >> 4 (rows >= CW compute
-// nothing that is kept) at ordinates 16 g + (L & 15), g = 0..3: fOwn and wmaxOwn are its
-// running sum and maximum of the weights, shiftOwn its cell's shift, sWc the chunk's
-// quadrature weights.
+// Phase 1 of a chunk for ordinate q = 16 g + (lane & 15) of row lane >> 4, given its
+// exponent x: the FP64 weight w_q e^{x - shift} (returned; rows < CW also store it in sWE64 for
+// the gradient products) and the FP16 halves of the scaled weight (the FP32 difference v - hi
+// is exact).  sWH holds, per row c, the halves: hi at [2 c][q], lo at [2 c + 1][q] (kChunk
+// halves each); lanes 2k and 2k + 1 (ordinates q, q + 1 of one row) store them as two 32-bit
+// words, the even lane the hi pair, the odd lane the lo pair (the partner's half by a DPP swap
+// within the pair, so every lane of the wave takes part).  wmaxOwn: the lane's running
+// maximum of its weights.
+template <int CW>
+__device__ __forceinline__ double h16_weight(double x, int q, const double *sWc, double shiftOwn,
+                                             float &wmaxOwn, _Float16 *sWH, double *sWE64,
+                                             int lane)
+{
+    const int row = lane >> 4;
+    const bool odd = lane & 1;
+    const double we = sWc[q] * exp(x - shiftOwn);
+    const float wf = (float)we;
+    wmaxOwn = fmaxf(wmaxOwn, wf);
+    const float v = wf * kWScale;
+    const _Float16 hi = (_Float16)v;
+    const _Float16 lo = (_Float16)(v - (float)hi);
+    const unsigned hb = __builtin_bit_cast(unsigned short, hi);
+    const unsigned lb = __builtin_bit_cast(unsigned short, lo);
+    const unsigned give = odd ? hb : lb;
+    const unsigned got = (unsigned)__builtin_amdgcn_mov_dpp((int)give, 0xB1, 0xF, 0xF, false);
+    const unsigned word = odd ? (got | (lb << 16)) : (hb | (got << 16));
+    if (row < CW) {
+        sWE64[row * kWE64S + q] = we;
+        reinterpret_cast<unsigned *>(sWH)[(2 * row + (odd ? 1 : 0)) * (kChunk / 2) + (q >> 1)] =
+            word;
+    }
+    return we;
+}
+
+// Phase 2: the FP64 gradient moments of a chunk on the FP64 matrix cores, rows = the CW cells
+// (zero rows beyond), 4 ordinates a step, four independent accumulators (step s into
+// accumulator s % 4).
 template <int NM, int CW>
-__device__ __forceinline__ void chunk_h16(
-    const double *sMc, const _Float16 *sBh, const double *sWc, const ExpA<NM> &ax,
-    double shiftOwn, float &wmaxOwn, _Float16 *sWH, double *sWE64,
-    f4 (&acc)[CW][(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
-    double (&gacc)[(NM + 15) / 16][4], double &fOwn, int lane)
+__device__ __forceinline__ void h16_grad(const double *sMc, const double *sWE64,
+                                         double (&gacc)[(NM + 15) / 16][4], int lane)
+{
+    constexpr int NT = (NM + 15) / 16;
+    constexpr int NMS = NM | 1;
+    const int ar = g4_a_row(lane), kk = g4_k(lane), bc = g4_b_col(lane);
+#pragma unroll 4
+    for (int s = 0; s < kChunk / 4; ++s) {
+        const int q = 4 * s + kk;
+        const double a = ar < CW ? sWE64[ar * kWE64S + q] : 0.0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int col = 16 * t + bc;
+            const double b = col < NM ? sMc[q * NMS + col] : 0.0;
+            gacc[t][s & 3] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, gacc[t][s & 3], 0, 0, 0);
+        }
+    }
+}
+
+// Phase 3: the Hessian of a chunk as FP16 split products, 32 ordinates per instruction.  The A
+// operand of tile t (rows 16 t + i16, ordinates q0..q0+7 of the lane) holds the same basis
+// values as the B operand of tile t, so one image serves both.
+template <int NM, int CW>
+__device__ __forceinline__ void h16_hess(const _Float16 *sBh, const _Float16 *sWH,
+                                         f4 (&acc)[CW][(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
+                                         int lane)
 {
     using HC = H16Chunk<NM>;
     constexpr int NT = HC::NT;
-    constexpr int NMS = NM | 1;
-    // 1. per group of 16 ordinates: the exponents on the matrix cores, the FP64 weights (f,
-    // gradient) and the FP16 halves of the scaled weights (the FP32 difference v - hi is
-    // exact); the halves of lanes 2k and 2k + 1 (ordinates q, q + 1 of one cell) go out as two
-    // 32-bit stores, the even lane writing the hi pair, the odd lane the lo pair (the
-    // partner's half by a DPP swap within the pair)
-    const int row = lane >> 4, r16 = lane & 15;
-    const bool odd = lane & 1;
-    unsigned *sWH32 = reinterpret_cast<unsigned *>(sWH);
-    double xg[4];
-    exp_arg4<NM>(ax, [&](int g, int s) {
-        const int m = 4 * s + (lane >> 4);
-        return (NM % 4 == 0 || m < NM) ? sMc[(16 * g + r16) * NMS + m] : 0.0;
-    }, xg);
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-        const int q = 16 * g + r16;
-        const double x = xg[g];
-        const double we = sWc[q] * exp(x - shiftOwn);
-        fOwn = fOwn + we;
-        const float wf = (float)we;
-        wmaxOwn = fmaxf(wmaxOwn, wf);
-        const float v = wf * kWScale;
-        const _Float16 hi = (_Float16)v;
-        const _Float16 lo = (_Float16)(v - (float)hi);
-        const unsigned hb = __builtin_bit_cast(unsigned short, hi);
-        const unsigned lb = __builtin_bit_cast(unsigned short, lo);
-        const unsigned give = odd ? hb : lb;
-        const unsigned got = (unsigned)__builtin_amdgcn_mov_dpp((int)give, 0xB1, 0xF, 0xF, false);
-        const unsigned word = odd ? (got | (lb << 16)) : (hb | (got << 16));
-        if (row < CW) {
-            sWE64[row * kWE64S + q] = we;
-            sWH32[(2 * row + (odd ? 1 : 0)) * (kChunk / 2) + (q >> 1)] = word;
-        }
-    }
-    wave_lds_sync();
-    __builtin_amdgcn_sched_barrier(0);
-    // 2. FP64 gradient moments: rows = the CW cells (zero rows beyond), 4 ordinates a step,
-    // four independent accumulators (step s into accumulator s % 4)
-    {
-        const int ar = g4_a_row(lane), kk = g4_k(lane), bc = g4_b_col(lane);
-#pragma unroll 4
-        for (int s = 0; s < kChunk / 4; ++s) {
-            const int q = 4 * s + kk;
-            const double a = ar < CW ? sWE64[ar * kWE64S + q] : 0.0;
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const int col = 16 * t + bc;
-                const double b = col < NM ? sMc[q * NMS + col] : 0.0;
-                gacc[t][s & 3] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, gacc[t][s & 3], 0, 0, 0);
-            }
-        }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    // 3. the Hessian: FP16 split products, 32 ordinates per instruction.  The A operand of
-    // tile t (rows 16 t + i16, ordinates q0..q0+7 of the lane) holds the same basis values as
-    // the B operand of tile t, so one image serves both.
     const int i16 = lane & 15, g = lane >> 4;
 #pragma unroll 1
     for (int kb = 0; kb < HC::kKB; ++kb) {
@@ -1395,6 +1391,38 @@ __device__ __forceinline__ void chunk_h16(
             }
         }
     }
+}
+
+// One chunk of the mixed sweep for the CW cells of a wave.  Lane L works on cell L >> 4 (rows
+// >= CW compute nothing that is kept) at ordinates 16 g + (L & 15), g = 0..3: fOwn and wmaxOwn
+// are its running sum and maximum of the weights, shiftOwn its cell's shift, sWc the chunk's
+// quadrature weights.
+template <int NM, int CW>
+__device__ __forceinline__ void chunk_h16(
+    const double *sMc, const _Float16 *sBh, const double *sWc, const ExpA<NM> &ax,
+    double shiftOwn, float &wmaxOwn, _Float16 *sWH, double *sWE64,
+    f4 (&acc)[CW][(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
+    double (&gacc)[(NM + 15) / 16][4], double &fOwn, int lane)
+{
+    constexpr int NMS = NM | 1;
+    // 1. per group of 16 ordinates: the exponents on the matrix cores, the weights
+    const int r16 = lane & 15;
+    double xg[4];
+    exp_arg4<NM>(ax, [&](int g, int s) {
+        const int m = 4 * s + (lane >> 4);
+        return (NM % 4 == 0 || m < NM) ? sMc[(16 * g + r16) * NMS + m] : 0.0;
+    }, xg);
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+        fOwn = fOwn + h16_weight<CW>(xg[g], 16 * g + r16, sWc, shiftOwn, wmaxOwn, sWH, sWE64,
+                                     lane);
+    wave_lds_sync();
+    __builtin_amdgcn_sched_barrier(0);
+    // 2. the FP64 gradient moments
+    h16_grad<NM, CW>(sMc, sWE64, gacc, lane);
+    __builtin_amdgcn_sched_barrier(0);
+    // 3. the Hessian
+    h16_hess<NM, CW>(sBh, sWH, acc, lane);
     wave_lds_sync();  // sWH and sWE64 are rewritten by the next chunk
 }
 
@@ -2082,6 +2110,89 @@ __device__ __forceinline__ void f_trials(const KParams &P, const double *__restr
     }
 }
 
+// The mixed sweep of one cell by the four waves of a tail block (256 threads), with exactly the
+// operations a wave of k_sweep_mixed performs for that cell (CW = 1), so the cell's H, f and
+// gradient are the same bits: all waves stage each chunk (PairStager); wave g computes the
+// exponents and weights of ordinate group g (its exponent chain is exp_arg4's chain g); then
+// wave 0 adds the weights into its lanes' running sums in group order and runs the Hessian
+// products while wave 1 runs the gradient products (each accumulator's instruction sequence
+// unchanged).  Results: H (scaled by e^shift, invalid marked as k_sweep_mixed marks it) in sH
+// and the gradient moments in sG; the return value is f (wave 0).  ax: the cell's exponent
+// operand (every wave); sRed: 4 floats.
+template <int NM>
+__device__ __forceinline__ double sweep_mixed_block(const KParams &P, const double *__restrict__ Mg,
+                                                    const double *__restrict__ wg,
+                                                    const ExpA<NM> &ax, double shift, double *sM,
+                                                    _Float16 *sBh, _Float16 *sWH, double *sWE64,
+                                                    double *sW0, double *sH, double *sG,
+                                                    float *sRed, int lane, int wv)
+{
+    using HC = H16Chunk<NM>;
+    constexpr int NT = HC::NT;
+    constexpr int NB = NT * (NT + 1) / 2;
+    constexpr int NMS = NM | 1;
+    const int tid = threadIdx.x, r16 = lane & 15;
+    const double shiftOwn = (lane >> 4) == 0 ? shift : 0.0;
+    f4 acc[1][NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+        acc[0][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    double gacc[NT][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            gacc[t][k] = 0.0;
+    double fOwn = 0.0;
+    float wmaxOwn = 0.0f;
+    const int nch = P.nqPad / kChunk;
+    PairStager<NM> st;
+    st.fetch(Mg, wg, 0, tid);
+    for (int ch = 0; ch < nch; ++ch) {
+        __syncthreads();  // the previous chunk's readers are done
+        st.put(sM, sBh, sW0, tid);
+        __syncthreads();
+        if (ch + 1 < nch)
+            st.fetch(Mg, wg, ch + 1, tid);
+        {
+            const int g = wv;
+            double x = 0.0;
+#pragma unroll
+            for (int s = 0; s < ExpA<NM>::NKS; ++s) {
+                const int m = 4 * s + (lane >> 4);
+                const double b = (NM % 4 == 0 || m < NM) ? sM[(16 * g + r16) * NMS + m] : 0.0;
+                x = __builtin_amdgcn_mfma_f64_4x4x4f64(ax.a[s], b, x, 0, 0, 0);
+            }
+            (void)h16_weight<1>(x, 16 * g + r16, sW0, shiftOwn, wmaxOwn, sWH, sWE64, lane);
+        }
+        __syncthreads();
+        if (wv == 0) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                fOwn = fOwn + sWE64[16 * g + r16];
+            h16_hess<NM, 1>(sBh, sWH, acc, lane);
+        } else if (wv == 1) {
+            h16_grad<NM, 1>(sM, sWE64, gacc, lane);
+        }
+    }
+    const float wRow = row16_max(wmaxOwn);
+    if (lane == 0)
+        sRed[wv] = wRow;
+    __syncthreads();
+    const double es = exp(shift);
+    double fdir = 0.0;
+    if (wv == 0) {
+        const bool ok = h16_range_ok(fmaxf(fmaxf(sRed[0], sRed[1]), fmaxf(sRed[2], sRed[3])));
+        fdir = __shfl(row16_sum(fOwn), 0) * es;
+        store_H32<NM>(acc[0], es / kWScale, sH, lane);
+        if (!ok && lane == 0)
+            sH[0] = __builtin_nan("");
+    } else if (wv == 1) {
+        store_gm<NM>(gacc, 0, es, sG, lane);
+    }
+    return fdir;
+}
+
 // f at up to four trial points in the mixed phase, as chunk_h16 evaluates it: the points are
 // the rows of the exponent products (alpha of point j at sA + j * NM, j < kw), lane L takes
 // point L >> 4 at ordinates 16 g + (L & 15) of each chunk and sums its weights in chunk and
@@ -2195,6 +2306,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     __shared__ __align__(16) _Float16 sWH[MIXED ? 2 * kChunk : 8];
     __shared__ __align__(16) double sWE64[MIXED ? kWE64S : 2];
     __shared__ double sG[NM];
+    __shared__ float sRed[4];
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int nWm = uniform_int(*Lc.nm);
     const int nW = nWm + uniform_int(*Lc.nf);
@@ -2269,7 +2381,27 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 if (!accepted)
                     continue;
             }
-            if (wv == 0) {
+            bool swept = false;
+            if constexpr (MIXED && TW == 4) {
+                static_assert(TW * kWave == kBlock, "the block sweep stages with kBlock threads");
+                if (mx) {  // the four waves share the sweep (same bits as one wave's)
+                    if (wv == 0 && lane < NM)
+                        sA[0][lane] = C.a;
+                    __syncthreads();
+                    ExpA<NM> ax;
+                    ax.load(sA[0], 1, lane);
+                    const double fdir = sweep_mixed_block<NM>(P, Mg, wg, ax, C.shift, sM, sBh, sWH,
+                                                              sWE64, sW[0], sH, sG, sRed, lane,
+                                                              wv);
+                    if (wv == 0) {
+                        if (lane == 0)
+                            sF[0] = fdir;
+                        ++sweeps;
+                    }
+                    swept = true;
+                }
+            }
+            if (wv == 0 && !swept) {
                 double fdir;
                 if (lane < NM)
                     sA[0][lane] = C.a;
