@@ -244,6 +244,9 @@ def main():
                     help="step workload: capture one SSP-RK2 step in a HIP graph and time its "
                          "replays (asynchronous schedule; --async-passes, default 48); orders "
                          "workload: the four orders' closures in one graph, one launch a step")
+    ap.add_argument("--tail", type=int, nargs=3, default=None, metavar=("CELLS", "ROUNDS", "WIDE"),
+                    help="the closure's tail thresholds (c3d_set_tail, c3d_set_tail_wide), "
+                         "overriding the tuning table")
     ap.add_argument("--mixed-waves", type=int, choices=(2, 3), default=None,
                     help="waves per SIMD of the mixed sweep (default: the library's)")
     ap.add_argument("--finish-cond", type=int, default=None,
@@ -337,6 +340,8 @@ def main():
             tuning[path] = ctx.apply_tuning("mixed" if path.startswith("mixed") else "fp64")
         if args.mixed_waves is not None:  # explicit values win over the table
             ctx.set_launch(P.LAUNCH_MIXED_WAVES, args.mixed_waves)
+        if args.tail is not None:
+            ctx.set_tail(args.tail[0], args.tail[1], wide_cells=args.tail[2])
         ctx.set_launch(P.LAUNCH_ASYNC, args.async_passes)
 
         def step():

@@ -2230,18 +2230,35 @@ __device__ __forceinline__ void f_trials_mixed(const KParams &P, const double *_
             fOwn = fOwn + w[g] * exp(xg[g] - shift);
     };
     if constexpr (NKS <= 4) {
+        // two chunks at a time: their eight exponent chains and exps are independent work for
+        // one another's latencies; the lane's sum still takes chunk A's groups, then B's
         double bA[4][NKS], bB[4][NKS], wA[4], wB[4];
         load(bA, wA, 0);
         if (nch > 1)
             load(bB, wB, 1);
         for (int ch = 0; ch < nch; ch += 2) {
-            term(bA, wA);
-            if (ch + 2 < nch)
-                load(bA, wA, ch + 2);
             if (ch + 1 < nch) {
-                term(bB, wB);
+                double xa[4], xb[4];
+                exp_arg4<NM>(ax, [&](int g, int t) { return bA[g][t]; }, xa);
+                exp_arg4<NM>(ax, [&](int g, int t) { return bB[g][t]; }, xb);
+                double ea[4], eb[4];
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    ea[g] = wA[g] * exp(xa[g] - shift);
+                    eb[g] = wB[g] * exp(xb[g] - shift);
+                }
+                if (ch + 2 < nch)
+                    load(bA, wA, ch + 2);
                 if (ch + 3 < nch)
                     load(bB, wB, ch + 3);
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+                    fOwn = fOwn + ea[g];
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+                    fOwn = fOwn + eb[g];
+            } else {
+                term(bA, wA);
             }
         }
     } else {  // nm = 25: one chunk in flight (registers)
@@ -2581,17 +2598,25 @@ __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double 
     for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
         const int grp = blk * kWavesPerBlock + wv;
         int slots[CW], cells[CW];
+        // iso bit c: the caller's alpha of cell c is isotropic (alpha_j = 0 for j >= 1).  Its
+        // gradient there is -u_j for j >= 1 (the basis is orthogonal to the constant m_0 under
+        // the quadrature) and e^{alpha_0 m_0} sum_q w_q m_0 - u_0 = e^{alpha_0 m_0} 4 pi m_0 - u_0
+        // for j = 0, so that point's sums are not evaluated.
+        unsigned iso = 0u;
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
             slots[c] = grp * CW + c;
             cells[c] = (grp < groups && slots[c] < nW) ? list_cell(L, slots[c], nWm) : -1;
+            const bool ok = cells[c] >= 0;
+            double ac = 0.0;
             if (lane < NM) {
-                const bool ok = cells[c] >= 0;
-                sA[wv][(2 * c) * NM + lane] =
-                    ok ? (float)S.vec[(size_t)cells[c] * kVec * NM + kVa * NM + lane] : 0.0f;
+                ac = ok ? S.vec[(size_t)cells[c] * kVec * NM + kVa * NM + lane] : 0.0;
+                sA[wv][(2 * c) * NM + lane] = (float)ac;
                 sA[wv][(2 * c + 1) * NM + lane] =
                     ok ? (float)anet[(size_t)slots[c] * NM + lane] : 0.0f;
             }
+            if (__all(lane < 1 || ac == 0.0) && ok)
+                iso |= 1u << c;
         }
         f4 acc[NT];
 #pragma unroll
@@ -2643,6 +2668,10 @@ __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double 
                 // per SIMD; 192 and two waves unsplit)
 #pragma unroll
                 for (int v = h * NV / 4; v < (h + 1) * NV / 4; ++v) {
+                    if ((v & 1) == 0 && (iso >> (v >> 1)) & 1u) {  // wave-uniform
+                        sWE[wv][v * kChunk + (lane & 3) * 16 + (lane >> 2)] = 0.0f;
+                        continue;
+                    }
                     float x = 0.0f;
 #pragma unroll
                     for (int i = 0; i < NM; ++i)
@@ -2683,13 +2712,17 @@ __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double 
         for (int r = 0; r < 4; ++r) {
             const int v = 4 * gq + r;
             const int c = (v < NV ? v : 0) >> 1;
+            const bool isoRow = v < NV && (v & 1) == 0 && ((iso >> c) & 1u);
             float t = 0.0f;
 #pragma unroll
             for (int bt = 0; bt < NT; ++bt) {
                 const int j = 16 * bt + rq;
                 if (j < NM && cells[c] >= 0) {
                     const float uj = (float)S.vec[(size_t)cells[c] * kVec * NM + kVu * NM + j];
-                    const float d = acc[bt][r] - uj;
+                    // an isotropic row's sums are zero: its gradient is -u_j for j >= 1
+                    const float g0 = __expf(sA[wv][(2 * c) * NM] * (float)P.m0) *
+                                     (float)(4.0 * M_PI * P.m0);
+                    const float d = (isoRow && j == 0) ? g0 - uj : acc[bt][r] - uj;
                     t = fmaf(d, d, t);
                 }
             }

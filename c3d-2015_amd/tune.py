@@ -18,6 +18,10 @@ through it bench.py) reads.  Usage (on the GPU):
 
     python c3d-2015_amd/tune.py [--out c3d-2015_amd/tuning.json] [--cells 262144]
                                 [--shapes 3,24 1,24 2,24 4,24 2,8 4,40] [--precisions fp64 mixed]
+
+The tail thresholds are cell counts, so an entry fits batches of about its "cells" size; the
+closure benchmark's shape (3,24) is tuned at its own batch size (--shapes 3,24 --cells
+1048576, which updates that entry only).
 """
 import argparse
 import importlib.util
