@@ -1602,10 +1602,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 
 // ---------------------------------------------------------------- one cell's state machine
 // Register image of a cell's Newton state over a group of G lanes; vectors are distributed
-// (group lane i < NM holds component i), scalars are uniform over the group.
+// (group lane i < NM holds component i), scalars are uniform over the group.  The ladder's
+// restart point a0, the input alpha and the unscaled moments stay in memory (vb): they are read
+// and written only on restarts and rungs.
 template <int NM, int G = 64>
 struct Cell {
-    double a, a0, aOld, d, g, uu, uOrig, aIn;
+    double a, aOld, d, g, uu;
+    double *vb;  // the cell's vector block in the state
     double u0, f, tls, tprev, maxF, gd, err, gamma, r, rtol, shift;
     int iter, tot, igam, reg, flags, phase, maxIt;
 
@@ -1619,14 +1622,12 @@ struct Cell {
     {
         const bool own = lane < NM;
         const double *v = S.vec + (size_t)c * kVec * NM;
+        vb = S.vec + (size_t)c * kVec * NM;
         a = own ? v[kVa * NM + lane] : 0.0;
-        a0 = own ? v[kVa0 * NM + lane] : 0.0;
         aOld = own ? v[kVaOld * NM + lane] : 0.0;
         d = own ? v[kVd * NM + lane] : 0.0;
         g = own ? v[kVg * NM + lane] : 0.0;
         uu = own ? v[kVu * NM + lane] : 0.0;
-        uOrig = own ? v[kVuOrig * NM + lane] : 0.0;
-        aIn = own ? v[kVaIn * NM + lane] : 0.0;
         const double *sc = S.scal + (size_t)c * kScal;
         u0 = sc[kSu0];
         f = sc[kSf];
@@ -1654,7 +1655,6 @@ struct Cell {
         double *v = S.vec + (size_t)c * kVec * NM;
         if (lane < NM) {
             v[kVa * NM + lane] = a;
-            v[kVa0 * NM + lane] = a0;
             v[kVaOld * NM + lane] = aOld;
             v[kVd * NM + lane] = d;
             v[kVg * NM + lane] = g;
@@ -1687,12 +1687,15 @@ struct Cell {
     // (rtol as for the batched semantics).
     __device__ __forceinline__ void restart_fp64(const KParams &P, int lane)
     {
-        a = aIn;
-        a0 = aIn;
-        aOld = aIn;
+        const bool own = lane < NM;
+        const double in = own ? vb[kVaIn * NM + lane] : 0.0;
+        a = in;
+        if (own)
+            vb[kVa0 * NM + lane] = in;
+        aOld = in;
         d = 0.0;
         g = 0.0;
-        uu = uOrig;
+        uu = own ? vb[kVuOrig * NM + lane] : 0.0;
         f = 0.0;
         tls = 1.0;
         tprev = 1.0;
@@ -1896,8 +1899,11 @@ struct Cell {
                 shift = (lf > -700.0 && lf < 700.0) ? lf : 0.0;
             }
         } else if (tot == 1 && (cholErr != 0 || condest > P.condHMax)) {
-            a0 = (lane == 0) ? P.isoAlpha0 : 0.0;  // isotropic restart, optBatch.cpp:208-218
-            a = a0;
+            // isotropic restart, optBatch.cpp:208-218
+            const double ai = (lane == 0) ? P.isoAlpha0 : 0.0;
+            if (own)
+                vb[kVa0 * NM + lane] = ai;
+            a = ai;
             iter = 0;
             igam = 0;
             tprev = 1.0;
@@ -1909,8 +1915,8 @@ struct Cell {
                 return 2;
             ++reg;
             r = kReg[reg];
-            uu = (lane == 0) ? 1.0 : (own ? uOrig * (1.0 - r) : 0.0);
-            a = a0;
+            uu = (lane == 0) ? 1.0 : (own ? vb[kVuOrig * NM + lane] * (1.0 - r) : 0.0);
+            a = own ? vb[kVa0 * NM + lane] : 0.0;
             igam = 0;
             iter = 0;
             tprev = 1.0;
