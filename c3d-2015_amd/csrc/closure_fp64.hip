@@ -1480,8 +1480,13 @@ struct PairStager {
     }
 };
 
-// H = e^shift * H32, row-major, lower triangle mirrored
-template <int NM>
+// Row i of the packed lower triangle starts at i (i + 1) / 2.
+__host__ __device__ constexpr int tri(int i) { return i * (i + 1) / 2; }
+
+// H = e^shift * H32: PACKED, the lower triangle by rows (entry (i, j <= i) at tri(i) + j; the
+// closure's mixed-phase layout, which its step reads); else row-major, lower triangle
+// mirrored
+template <int NM, bool PACKED = true>
 __device__ __forceinline__ void store_H32(const f4 *acc, double es, double *H, int lane)
 {
     constexpr int NT = (NM + 15) / 16;
@@ -1495,8 +1500,12 @@ __device__ __forceinline__ void store_H32(const f4 *acc, double es, double *H, i
             const int i = 16 * R + 4 * gq + k, j = 16 * Cb + rq;
             if (i < NM && j < NM && j <= i) {
                 const double h = (double)acc[b][k] * es;
-                H[i * NM + j] = h;
-                H[j * NM + i] = h;
+                if constexpr (PACKED) {
+                    H[tri(i) + j] = h;
+                } else {
+                    H[i * NM + j] = h;
+                    H[j * NM + i] = h;
+                }
             }
         }
     }
@@ -1817,16 +1826,22 @@ struct Cell {
             }
         }
         // ================================ one Newton iteration (solveTheRest)
+        const bool mx = mixed(P);
         Dense<NM, EXACT, G> D;
+        if (mx) {  // the mixed sweep's packed lower triangle (all the mixed phase reads)
 #pragma unroll
-        for (int j = 0; j < NM; ++j)
-            D.row[j] = own ? H[lane * NM + j] : 0.0;
+            for (int j = 0; j < NM; ++j)
+                D.row[j] = (own && j <= lane) ? H[tri(lane) + j] : 0.0;
+        } else {
+#pragma unroll
+            for (int j = 0; j < NM; ++j)
+                D.row[j] = own ? H[lane * NM + j] : 0.0;
+        }
         int maxItNow = maxIt;
         if (!P.unbatched && reg == kNumReg - 1)
             maxItNow = 10 * maxIt;
         ++iter;
         ++tot;
-        const bool mx = mixed(P);
         // fobjMnUseH; the mixed phase has the FP64 gradient moments of its sweep
         g = own ? (mx ? gm[lane] : D.row[0] / P.m0) - uu : 0.0;
         if (mx)  // f from the FP64 sum, as the line search evaluates it (fobj.cpp:135-164)
@@ -1846,27 +1861,29 @@ struct Cell {
             f = f - tree_sum(t);
         }
         double norm1 = 0.0;  // utils_norm1: the largest absolute row sum
-        if constexpr (EXACT) {
-            double rs = 0.0;
+        if (!mx) {  // for dpocon (the FP64 path)
+            if constexpr (EXACT) {
+                double rs = 0.0;
 #pragma unroll
-            for (int j = 0; j < NM; ++j)
-                rs = rs + fabs(D.row[j]);
+                for (int j = 0; j < NM; ++j)
+                    rs = rs + fabs(D.row[j]);
 #pragma unroll
-            for (int i = 0; i < NM; ++i) {
-                double vv = Grp<G>::bc(rs, i);
-                if (vv > norm1)
-                    norm1 = vv;
+                for (int i = 0; i < NM; ++i) {
+                    double vv = Grp<G>::bc(rs, i);
+                    if (vv > norm1)
+                        norm1 = vv;
+                }
+            } else {
+                double t[NM];
+#pragma unroll
+                for (int j = 0; j < NM; ++j)
+                    t[j] = fabs(D.row[j]);
+                const double rs = tree_sum(t);
+#pragma unroll
+                for (int i = 0; i < NM; ++i)
+                    t[i] = Grp<G>::bc(rs, i);
+                norm1 = fmax(tree_max(t), 0.0);
             }
-        } else {
-            double t[NM];
-#pragma unroll
-            for (int j = 0; j < NM; ++j)
-                t[j] = fabs(D.row[j]);
-            const double rs = tree_sum(t);
-#pragma unroll
-            for (int i = 0; i < NM; ++i)
-                t[i] = Grp<G>::bc(rs, i);
-            norm1 = fmax(tree_max(t), 0.0);
         }
         const int cholErr = D.cholesky(lane);
         double condest = 0.0;
@@ -2945,7 +2962,7 @@ __global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nq
     }
     if (!have)
         return;
-    store_H32<NM>(acc[0], exp(shift) / kWScale, hs + cell * NM * NM, lane);
+    store_H32<NM, false>(acc[0], exp(shift) / kWScale, hs + cell * NM * NM, lane);
 }
 
 // The dense part of one Newton iteration for a batch of given Hessians (one wave each):
