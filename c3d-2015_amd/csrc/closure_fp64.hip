@@ -1613,7 +1613,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 // Register image of a cell's Newton state over a group of G lanes; vectors are distributed
 // (group lane i < NM holds component i), scalars are uniform over the group.  The ladder's
 // restart point a0, the input alpha and the unscaled moments stay in memory (vb): they are read
-// and written only on restarts and rungs.
+// and written only on restarts and rungs, as are the scaled moments uu (read every step).
 template <int NM, int G = 64>
 struct Cell {
     double a, aOld, d, g, uu;
@@ -1635,7 +1635,8 @@ struct Cell {
         a = own ? v[kVa * NM + lane] : 0.0;
         aOld = own ? v[kVaOld * NM + lane] : 0.0;
         d = own ? v[kVd * NM + lane] : 0.0;
-        g = own ? v[kVg * NM + lane] : 0.0;
+        g = 0.0;  // computed from each sweep before it is used (the g slot carries the sweep's
+                  // gradient moments in the mixed phase)
         uu = own ? v[kVu * NM + lane] : 0.0;
         const double *sc = S.scal + (size_t)c * kScal;
         u0 = sc[kSu0];
@@ -1666,8 +1667,6 @@ struct Cell {
             v[kVa * NM + lane] = a;
             v[kVaOld * NM + lane] = aOld;
             v[kVd * NM + lane] = d;
-            v[kVg * NM + lane] = g;
-            v[kVu * NM + lane] = uu;
         }
         if (lane == 0) {
             double *sc = S.scal + (size_t)c * kScal;
@@ -1705,6 +1704,8 @@ struct Cell {
         d = 0.0;
         g = 0.0;
         uu = own ? vb[kVuOrig * NM + lane] : 0.0;
+        if (own)
+            vb[kVu * NM + lane] = uu;
         f = 0.0;
         tls = 1.0;
         tprev = 1.0;
@@ -1933,6 +1934,8 @@ struct Cell {
             ++reg;
             r = kReg[reg];
             uu = (lane == 0) ? 1.0 : (own ? vb[kVuOrig * NM + lane] * (1.0 - r) : 0.0);
+            if (own)
+                vb[kVu * NM + lane] = uu;
             a = own ? vb[kVa0 * NM + lane] : 0.0;
             igam = 0;
             iter = 0;
