@@ -16,6 +16,30 @@ namespace c3d {
 constexpr int kWave = 64;
 constexpr int kChunk = 64;  // ordinates per LDS chunk in the closure sweeps
 
+// The mixed closure's pair image (orders with nm <= kPairMaxNm): the basis products
+// P_qp = kPairScale m_qi m_qj of the pairs p = i (i + 1) / 2 + j, j <= i (the packed lower
+// triangle), as FP16 halves hi = fp16(fp32 P), lo = fp16(fp32 P - hi), in the B-operand order
+// of V_MFMA_F32_16X16X32_F16: per chunk of kChunk ordinates, per block kb of 32 ordinates,
+// per tile t of 16 pairs and plane (hi, lo), lane 16 g + n holds 8 consecutive halves of pair
+// 16 t + n, element e at ordinate 32 kb + 16 (e >> 2) + g + 4 (e & 3) (the K order in which
+// the column sweep's lanes hold their weights).  Built on the host (capi.hip), read by the
+// mixed sweeps (closure_fp64.hip).
+constexpr double kPairScale = 64.0;  // lifts the small products out of FP16's subnormal range
+constexpr int kPairMaxNm = 16;
+__host__ __device__ constexpr int pair_count(int nm) { return nm * (nm + 1) / 2; }
+__host__ __device__ constexpr int pair_tiles(int nm) { return (pair_count(nm) + 15) / 16; }
+__host__ __device__ constexpr int pair_chunk_halves(int nm)
+{
+    return (kChunk / 32) * pair_tiles(nm) * 2 * 512;
+}
+// element (ordinate qc of its chunk, pair p, plane) within the chunk's image
+__host__ __device__ constexpr int pair_offset(int nm, int qc, int p, int plane)
+{
+    return ((((qc >> 5) * pair_tiles(nm) + (p >> 4)) * 2 + plane) * 64 + (qc & 3) * 16 +
+            (p & 15)) * 8 +
+           4 * ((qc & 31) >> 4) + ((qc >> 2) & 3);
+}
+
 // Host-side tables (init.cpp:423-548, 287-337, 199-234), built by tables.cpp or copied from
 // the caller.
 struct HostTables {
@@ -69,6 +93,7 @@ struct c3d_ctx_s {
     double *d_M = nullptr;     // [nqPad][nm], rows >= nq zero
     double *d_MT = nullptr;    // [nm][nqPad], the transpose of d_M
     double *d_wpad = nullptr;  // [nqPad], entries >= nq zero
+    uint16_t *d_P16 = nullptr;  // the pair image (nm <= kPairMaxNm), nqPad / kChunk chunks
     double *d_M2 = nullptr;    // [nqPad][nm2]
     double *d_beta = nullptr, *d_betaVal = nullptr, *d_pnOps = nullptr;
     int32_t *d_betaColInd = nullptr, *d_betaRowPtr = nullptr;

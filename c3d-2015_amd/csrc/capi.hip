@@ -38,6 +38,27 @@ hipError_t upload(T **dst, const T *src, size_t n)
     return hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice);
 }
 
+// The mixed closure's pair image of the tables (layout and scale in c3d_internal.h); ordinates
+// >= nq and pairs >= nm (nm + 1) / 2 are zero.
+void pair_image(const HostTables &t, int nqPad, std::vector<uint16_t> &img)
+{
+    const int nm = t.nm, ch = pair_chunk_halves(nm);
+    img.assign((size_t)nqPad / kChunk * ch, 0);
+    for (int q = 0; q < t.nq; ++q) {
+        const double *m = &t.M[(size_t)q * nm];
+        uint16_t *dst = img.data() + (size_t)(q / kChunk) * ch;
+        for (int i = 0; i < nm; ++i)
+            for (int j = 0; j <= i; ++j) {
+                const int p = i * (i + 1) / 2 + j;
+                const float v = (float)(kPairScale * (m[i] * m[j]));
+                const _Float16 hi = (_Float16)v;
+                const _Float16 lo = (_Float16)(v - (float)hi);
+                dst[pair_offset(nm, q % kChunk, p, 0)] = __builtin_bit_cast(uint16_t, hi);
+                dst[pair_offset(nm, q % kChunk, p, 1)] = __builtin_bit_cast(uint16_t, lo);
+            }
+    }
+}
+
 int finish_ctx(c3d_ctx_s *ctx)
 {
     HostTables &t = ctx->t;
@@ -75,6 +96,11 @@ int finish_ctx(c3d_ctx_s *ctx)
         UP(d_MT, MTp);
     }
     UP(d_wpad, wp);
+    if (t.nm <= kPairMaxNm) {
+        std::vector<uint16_t> img;
+        pair_image(t, ctx->nqPad, img);
+        UP(d_P16, img);
+    }
     if (!t.M2.empty()) {
         std::vector<double> M2p((size_t)ctx->nqPad * t.nm2, 0.0);
         std::memcpy(M2p.data(), t.M2.data(), t.M2.size() * sizeof(double));
@@ -229,7 +255,7 @@ int c3d_destroy(c3d_ctx ctx)
     if (!ctx)
         return C3D_OK;
     void *ptrs[] = {ctx->d_w, ctx->d_xi, ctx->d_eta, ctx->d_mu, ctx->d_M, ctx->d_MT,
-                    ctx->d_wpad, ctx->d_M2, ctx->d_beta, ctx->d_betaVal, ctx->d_pnOps, ctx->d_betaColInd,
+                    ctx->d_wpad, ctx->d_P16, ctx->d_M2, ctx->d_beta, ctx->d_betaVal, ctx->d_pnOps, ctx->d_betaColInd,
                     ctx->d_betaRowPtr, ctx->d_sweepCount, ctx->d_status, ctx->ansatz.p,
                     ctx->uOld.p, ctx->flux.p, ctx->closureState.p};
     for (void *p : ptrs)

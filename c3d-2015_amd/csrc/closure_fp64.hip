@@ -1164,6 +1164,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 // register i % 4.
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 
 // Row stride (doubles) of the FP64 weights a wave keeps per cell for the gradient products:
 // the four cells' rows start in different LDS banks.
@@ -1505,6 +1506,418 @@ __device__ __forceinline__ void store_H32(const f4 *acc, double es, double *H, i
                 } else {
                     H[i * NM + j] = h;
                     H[j * NM + i] = h;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- column sweep (nm <= 16)
+// The mixed sweep for nm <= kPairMaxNm with the cells as the columns of its matrix products,
+// 16 cells a wave, cell c in lanes l with l & 15 = c:
+//   exponents  X = M alpha^T on V_MFMA_F64_16X16X4_F64 per block of 16 ordinates (A = the M
+//              rows of the block, B = the cells' alpha, kept in registers);
+//   gradient   G = W M on the same instruction, A = the weights exactly as the exponent products
+//              leave them in the lanes (accumulator register r of lane l holds row (l >> 4) + 4 r,
+//              column l & 15), so no transposition;
+//   Hessian    the pair products H_p = sum_q v_q P_qp over the pairs p of the packed lower
+//              triangle on V_MFMA_F32_16X16X32_F16, A = the FP16 halves of the scaled weights,
+//              again from the lanes' own registers (the pair image's K order follows them,
+//              c3d_internal.h), B = the pair image (16 pairs a tile), three products per 32
+//              ordinates, hi hi + hi lo + lo hi.
+// Each output element of a matrix product depends on its own row and column only, so a cell's
+// results have the same bits in whichever column of whichever wave they are computed.
+template <int NM>
+struct PairHess {
+    static constexpr int NP = pair_count(NM);
+    static constexpr int PT = pair_tiles(NM);
+    static constexpr int kHalves = pair_chunk_halves(NM);  // one chunk's image
+    static constexpr int kUnits = kHalves / 512;            // its 1 KB units
+};
+
+// H = scale * acc with the cell's e^shift: the weights carry 2^12, the image kPairScale
+__device__ __forceinline__ double pair_scale(double es) { return es / (kWScale * kPairScale); }
+
+// The exponent products' B operand for K step s: alpha[4 s + (lane >> 4)] of the lane's cell
+// (zero beyond NM and where alpha is null).
+template <int NM>
+struct ExpB {
+    static constexpr int NKS = (NM + 3) / 4;
+    double a[NKS];
+    __device__ __forceinline__ void load(const double *alpha, int lane)
+    {
+#pragma unroll
+        for (int s = 0; s < NKS; ++s) {
+            const int m = 4 * s + (lane >> 4);
+            a[s] = (alpha != nullptr && m < NM) ? alpha[m] : 0.0;
+        }
+    }
+};
+
+// Exponents of the four 16-ordinate blocks of a chunk: x[b][r] = alpha . m_q of the lane's
+// cell at ordinate q = 16 b + (lane >> 4) + 4 r; am(b, s) returns M[16 b + (lane & 15)][4 s +
+// (lane >> 4)] (zero for moments >= NM).  The four K-chains are interleaved (K step outer).
+template <int NM, typename AF>
+__device__ __forceinline__ void col_exponents(const ExpB<NM> &aT, AF am, d4 (&x)[4])
+{
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        x[b] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < ExpB<NM>::NKS; ++s)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            x[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(am(b, s), aT.a[s], x[b], 0, 0, 0);
+}
+
+// The weight of one ordinate from its exponent: w_q e^{x - shift} (FP64), the running block sum
+// fb, the running maximum, and the FP16 halves of the weight scaled by 2^12 (the FP32
+// difference v - hi is exact).
+__device__ __forceinline__ double col_weight(double x, double wq, double shiftOwn, double &fb,
+                                             float &wmaxOwn, _Float16 &hi, _Float16 &lo)
+{
+    const double we = wq * exp(x - shiftOwn);
+    fb = fb + we;
+    const float wf = (float)we;
+    wmaxOwn = fmaxf(wmaxOwn, wf);
+    const float v = wf * kWScale;
+    hi = (_Float16)v;
+    lo = (_Float16)(v - (float)hi);
+    return we;
+}
+
+// The gradient products of block b: gacc[t][b] += we[r] M[q][16 t + (lane & 15)] for
+// q = 16 b + (lane >> 4) + 4 r, r ascending (accumulator b, one per block).
+template <int NM>
+__device__ __forceinline__ void col_grad_block(const double *sMc, const d4 &we, int b,
+                                               d4 (&gacc)[(NM + 15) / 16][4], int lane)
+{
+    constexpr int NMS = NM | 1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int q = 16 * b + (lane >> 4) + 4 * r;
+#pragma unroll
+        for (int t = 0; t < (NM + 15) / 16; ++t) {
+            const int col = 16 * t + (lane & 15);
+            const double bm = col < NM ? sMc[q * NMS + col] : 0.0;
+            gacc[t][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(we[r], bm, gacc[t][b], 0, 0, 0);
+        }
+    }
+}
+
+// The pair products of a chunk into acc[i] for the tiles t = t0 + ts i < PT: ah/al the A
+// operands of the two 32-ordinate blocks (the halves of blocks 2 kb and 2 kb + 1), sP the
+// chunk's pair image.
+template <int NM, int NA>
+__device__ __forceinline__ void pair_hess(const _Float16 *sP, const h8 (&ah)[2], const h8 (&al)[2],
+                                          f4 (&acc)[NA], int t0, int ts, int lane)
+{
+    constexpr int PT = PairHess<NM>::PT;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            const int t = t0 + ts * i;
+            if (t < PT) {
+                const _Float16 *bp = &sP[(kb * PT + t) * 1024 + lane * 8];
+                const h8 bh = *reinterpret_cast<const h8 *>(bp);
+                const h8 bl = *reinterpret_cast<const h8 *>(bp + 512);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[kb], bh, acc[i], 0, 0, 0);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[kb], bl, acc[i], 0, 0, 0);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[kb], bh, acc[i], 0, 0, 0);
+            }
+        }
+    }
+}
+
+// One chunk of the column sweep for the 16 cells of a wave (cell lane & 15): exponents,
+// weights (block sums fb, maximum wmaxOwn), gradient products (accumulator per block) and pair
+// products; sMc the chunk's FP64 rows (leading dimension NM | 1), sWc its weights, sPc its pair
+// image, complete once image_ready() returns (called before the pair products).
+template <int NM, typename RF>
+__device__ __forceinline__ void chunk_cols(const double *sMc, const double *sWc,
+                                           const _Float16 *sPc, const ExpB<NM> &aT,
+                                           double shiftOwn, float &wmaxOwn, double (&fb)[4],
+                                           d4 (&gacc)[(NM + 15) / 16][4],
+                                           f4 (&acc)[PairHess<NM>::PT], int lane, RF image_ready)
+{
+    constexpr int NMS = NM | 1;
+    const int i16 = lane & 15, g = lane >> 4;
+    d4 x[4];
+    col_exponents<NM>(aT, [&](int b, int s) {
+        const int m = 4 * s + g;
+        return (NM % 4 == 0 || m < NM) ? sMc[(16 * b + i16) * NMS + m] : 0.0;
+    }, x);
+    h8 ah[2], al[2];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            _Float16 hi, lo;
+            x[b][r] = col_weight(x[b][r], sWc[16 * b + g + 4 * r], shiftOwn, fb[b], wmaxOwn, hi,
+                                 lo);
+            ah[b >> 1][4 * (b & 1) + r] = hi;
+            al[b >> 1][4 * (b & 1) + r] = lo;
+        }
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        col_grad_block<NM>(sMc, x[b], b, gacc, lane);
+    image_ready();
+    pair_hess<NM, PairHess<NM>::PT>(sPc, ah, al, acc, 0, 1, lane);
+}
+
+// f of the lane's cell from its block sums: the blocks, then the four row groups (l >> 4) of
+// its column, in a fixed order (every lane of the column gets the same bits).
+__device__ __forceinline__ double col_f(const double (&fb)[4])
+{
+    double f = (fb[0] + fb[1]) + (fb[2] + fb[3]);
+    f = f + __shfl_xor(f, 16);
+    return f + __shfl_xor(f, 32);
+}
+__device__ __forceinline__ float col_max(float v)
+{
+    v = fmaxf(v, __shfl_xor(v, 16));
+    return fmaxf(v, __shfl_xor(v, 32));
+}
+// the gradient moments from the per-block accumulators (register r: cell (l >> 4) + 4 r)
+__device__ __forceinline__ double col_g(const d4 (&ga)[4], int r)
+{
+    return (ga[0][r] + ga[1][r]) + (ga[2][r] + ga[3][r]);
+}
+
+// Chunk stager of the FP64 rows (sM, leading dimension NM | 1) and the weights for NTH
+// threads: fetch() loads a chunk into registers, put() writes it to LDS.
+template <int NM, int NTH>
+struct RowStager {
+    static constexpr int NMS = NM | 1;
+    static constexpr int kItems = kChunk * NM;
+    static constexpr int kPer = (kItems + NTH - 1) / NTH;
+    double pre[kPer];
+    double prew;
+    __device__ __forceinline__ void fetch(const double *__restrict__ Mg,
+                                          const double *__restrict__ wg, int ch, int tid)
+    {
+        const double *src = Mg + (size_t)ch * kItems;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int e = tid + NTH * k;
+            pre[k] = (kItems % NTH == 0 || e < kItems) ? src[e] : 0.0;
+        }
+        prew = tid < kChunk ? wg[(size_t)ch * kChunk + tid] : 0.0;
+    }
+    __device__ __forceinline__ void put(double *sM, double *sW, int tid) const
+    {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int e = tid + NTH * k;
+            if (kItems % NTH == 0 || e < kItems) {
+                const int r = e / NM;
+                sM[r * NMS + (e - r * NM)] = pre[k];
+            }
+        }
+        if (tid < kChunk)
+            sW[tid] = prew;
+    }
+};
+
+// The LDS DMA (global_load_lds_dwordx4): lane i of the issuing wave copies the 16 bytes at
+// src + 16 i to dst + 16 i.  dma_units copies `units` 1 KB units, unit j by the wave j % nw of
+// the caller's nw waves (wv its index); the copy completes asynchronously, and each wave waits
+// for its own (dma_wait: vmcnt 0) before a barrier publishes it.  lds_reads_done waits for the
+// wave's outstanding LDS reads, before a copy overwrites what they read.
+__device__ __forceinline__ void dma_units(const _Float16 *src, _Float16 *dst, int units, int wv,
+                                          int nw, int lane)
+{
+    for (int j = wv; j < units; j += nw)
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(src + j * 512 + lane * 8),
+            (__attribute__((address_space(3))) void *)(dst + j * 512), 16, 0, 0);
+}
+__device__ __forceinline__ void dma_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+__device__ __forceinline__ void lds_reads_done() { __builtin_amdgcn_s_waitcnt(0xC07F); }
+
+// One cell's column sweep on one wave, the cell in column 0 (the other columns compute nothing
+// that is kept): the chunks staged by the wave itself, the pair image by LDS DMA into sP
+// (single-buffered: a chunk's copy is issued once the previous chunk's pair products have read
+// sP and is waited for before the chunk's own pair products).  Results as the bulk sweep
+// leaves them for column 0: H (packed lower triangle) in sH, its first entry NaN where the
+// weights left the FP16 range if mark, the gradient moments in sG; returns f.  sA: the cell's
+// alpha.
+template <int NM>
+__device__ __forceinline__ double sweep_cols_wave(int nqPad, const double *__restrict__ Mg,
+                                                  const double *__restrict__ wg,
+                                                  const _Float16 *__restrict__ Pimg,
+                                                  const double *sA, double shift, double *sM,
+                                                  double *sW0, _Float16 *sP, double *sH,
+                                                  double *sG, bool mark, int lane)
+{
+    using PH = PairHess<NM>;
+    constexpr int NT = (NM + 15) / 16;
+    constexpr int PT = PH::PT;
+    const bool own = (lane & 15) == 0;
+    ExpB<NM> aT;
+    aT.load(own ? sA : nullptr, lane);
+    const double shiftOwn = own ? shift : 0.0;
+    float wmaxOwn = 0.0f;
+    double fb[4] = {0.0, 0.0, 0.0, 0.0};
+    d4 gacc[NT][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            gacc[t][b] = d4{0.0, 0.0, 0.0, 0.0};
+    f4 acc[PT];
+#pragma unroll
+    for (int t = 0; t < PT; ++t)
+        acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int nch = nqPad / kChunk;
+    RowStager<NM, kWave> st;
+    st.fetch(Mg, wg, 0, lane);
+    wave_lds_sync();
+    lds_reads_done();  // earlier readers of sP are done
+    dma_units(Pimg, sP, PH::kUnits, 0, 1, lane);
+    for (int ch = 0; ch < nch; ++ch) {
+        wave_lds_sync();  // the previous chunk's readers of sM and sW0 are done
+        st.put(sM, sW0, lane);
+        if (ch + 1 < nch)
+            st.fetch(Mg, wg, ch + 1, lane);
+        wave_lds_sync();
+        chunk_cols<NM>(sM, sW0, sP, aT, shiftOwn, wmaxOwn, fb, gacc, acc, lane, [] {
+            dma_wait();
+            wave_lds_sync();
+        });
+        if (ch + 1 < nch) {
+            lds_reads_done();  // the pair products have read sP
+            dma_units(Pimg + (size_t)(ch + 1) * PH::kHalves, sP, PH::kUnits, 0, 1, lane);
+        }
+    }
+    const double f = col_f(fb);
+    const float wm = col_max(wmaxOwn);
+    const double es = exp(shift);
+    const double sc = pair_scale(es);
+    const bool ok = !mark || h16_range_ok(__shfl(wm, 0));
+    if (lane < 16) {
+#pragma unroll
+        for (int t = 0; t < PT; ++t) {
+            const int p = 16 * t + lane;
+            if (p < PH::NP)
+                sH[p] = (p == 0 && !ok) ? __builtin_nan("") : (double)acc[t][0] * sc;
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int col = 16 * t + lane;
+            if (col < NM)
+                sG[col] = col_g(gacc[t], 0) * es;
+        }
+    }
+    return __shfl(f, 0) * es;
+}
+
+constexpr int kCWaves = 8;  // waves per block of the column sweep
+constexpr int kCBlock = kCWaves * kWave;
+constexpr int kCCells = 16;  // cells per wave
+
+// Sweep pass of the mixed phase for nm <= kPairMaxNm (the column sweep): 16 cells per wave, 128
+// per block sharing each chunk; the chunk's FP64 rows and weights staged through registers, its
+// pair image by LDS DMA, both double-buffered (LDS 90 KB at nm = 16).  Writes H (packed lower
+// triangle, first entry NaN where the weights left the FP16 range), sum_q w e (fdir) and the
+// FP64 gradient moments (g slot).
+template <int NM>
+__global__ __launch_bounds__(kCBlock) void k_sweep_cols(KParams P, const double *__restrict__ Mg,
+                                                        const double *__restrict__ wg,
+                                                        const _Float16 *__restrict__ Pimg,
+                                                        State S, const int *__restrict__ W,
+                                                        const int *__restrict__ nWp)
+{
+    using PH = PairHess<NM>;
+    constexpr int NT = (NM + 15) / 16;
+    constexpr int PT = PH::PT;
+    constexpr int NMS = NM | 1;
+    __shared__ __align__(16) _Float16 sP[2][PH::kHalves];
+    __shared__ double sM[2][kChunk * NMS];
+    __shared__ double sW[2][kChunk];
+    const int tid = threadIdx.x, lane = tid & 63, wv = uniform_int(tid >> 6);
+    const int i16 = lane & 15, g = lane >> 4;
+    const int nch = P.nqPad / kChunk;
+    const int nW = uniform_int(*nWp);
+    const int groups = (nW + kCCells - 1) / kCCells;
+    const int nBlocksNeeded = (groups + kCWaves - 1) / kCWaves;
+    for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
+        const int grp = blk * kCWaves + wv;
+        const bool any = grp < groups;
+        const int slot = grp * kCCells + i16;
+        const int cell = (any && slot < nW) ? W[slot] : -1;  // the lane's cell
+        const double shift = cell >= 0 ? S.scal[(size_t)cell * kScal + kSshift] : 0.0;
+        ExpB<NM> aT;
+        aT.load(cell >= 0 ? S.vec + (size_t)cell * kVec * NM + kVa * NM : nullptr, lane);
+        float wmaxOwn = 0.0f;
+        double fb[4] = {0.0, 0.0, 0.0, 0.0};
+        d4 gacc[NT][4];
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                gacc[t][b] = d4{0.0, 0.0, 0.0, 0.0};
+        f4 acc[PT];
+#pragma unroll
+        for (int t = 0; t < PT; ++t)
+            acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        RowStager<NM, kCBlock> st;
+        st.fetch(Mg, wg, 0, tid);
+        __syncthreads();  // the previous cells' last chunk is consumed
+        dma_units(Pimg, sP[0], PH::kUnits, wv, kCWaves, lane);
+        for (int ch = 0; ch < nch; ++ch) {
+            const int b = ch & 1;
+            st.put(sM[b], sW[b], tid);
+            dma_wait();
+            __syncthreads();
+            if (ch + 1 < nch) {
+                dma_units(Pimg + (size_t)(ch + 1) * PH::kHalves, sP[b ^ 1], PH::kUnits, wv,
+                          kCWaves, lane);
+                st.fetch(Mg, wg, ch + 1, tid);
+            }
+            if (any)
+                chunk_cols<NM>(sM[b], sW[b], sP[b], aT, shift, wmaxOwn, fb, gacc, acc, lane,
+                               [] {});
+        }
+        if (!any)
+            continue;
+        const double f = col_f(fb);
+        const float wm = col_max(wmaxOwn);
+        const double es = exp(shift);
+        if (g == 0 && cell >= 0)
+            S.fdir[cell] = f * es;
+        // H: accumulator k of lane l is the cell of column 4 (l >> 4) + k, pair 16 t + (l & 15)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int src = 4 * g + k;
+            const int ck = __shfl(cell, src);
+            const double sc = pair_scale(__shfl(es, src));
+            const bool ok = h16_range_ok(__shfl(wm, src));
+            if (ck >= 0) {
+                double *H = S.H + (size_t)ck * NM * NM;
+#pragma unroll
+                for (int t = 0; t < PT; ++t) {
+                    const int p = 16 * t + i16;
+                    if (p < PH::NP)
+                        H[p] = (p == 0 && !ok) ? __builtin_nan("") : (double)acc[t][k] * sc;
+                }
+            }
+        }
+        // gradient: register r of lane l is the cell of column (l >> 4) + 4 r, moment
+        // 16 t + (l & 15)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int src = g + 4 * r;
+            const int cr = __shfl(cell, src);
+            const double esr = __shfl(es, src);
+            if (cr >= 0) {
+                double *gm = S.vec + (size_t)cr * kVec * NM + kVg * NM;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int col = 16 * t + i16;
+                    if (col < NM)
+                        gm[col] = col_g(gacc[t], r) * esr;
                 }
             }
         }
@@ -2219,6 +2632,180 @@ __device__ __forceinline__ double sweep_mixed_block(const KParams &P, const doub
     return fdir;
 }
 
+// One cell's column sweep on the four waves of a tail block, with the bulk's bits for column 0:
+// wave w takes ordinate block w of every chunk (its exponent chain, weights, running block sum
+// and gradient accumulator w), the waves exchange the FP16 halves through sXH for the pair
+// products (wave w: tiles w, w + 4, ..), and f, the largest weight and the gradient are
+// combined in the bulk's order at the end (sXF, sXG, sRed).  Results as sweep_cols_wave's.
+template <int NM>
+__device__ __forceinline__ double sweep_cols_block(const KParams &P, const double *__restrict__ Mg,
+                                                   const double *__restrict__ wg,
+                                                   const _Float16 *__restrict__ Pimg,
+                                                   const double *sA, double shift, double *sM,
+                                                   double *sW0, _Float16 *sP, _Float16 *sXH,
+                                                   double *sXF, double *sXG, float *sRed,
+                                                   double *sH, double *sG, int lane, int wv)
+{
+    using PH = PairHess<NM>;
+    constexpr int NT = (NM + 15) / 16;
+    constexpr int NA = (PH::PT + 3) / 4;
+    constexpr int NMS = NM | 1;
+    const int tid = threadIdx.x, i16 = lane & 15, g = lane >> 4;
+    const bool own = i16 == 0;
+    ExpB<NM> aT;
+    aT.load(own ? sA : nullptr, lane);
+    const double shiftOwn = own ? shift : 0.0;
+    float wmaxOwn = 0.0f;
+    double fbw = 0.0;
+    d4 ga[NT][4];  // only [t][wv] is used: col_grad_block's accumulator of block wv
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            ga[t][b] = d4{0.0, 0.0, 0.0, 0.0};
+    f4 acc[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+        acc[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int nch = P.nqPad / kChunk;
+    RowStager<NM, kBlock> st;
+    st.fetch(Mg, wg, 0, tid);
+    for (int ch = 0; ch < nch; ++ch) {
+        __syncthreads();  // the previous chunk's readers are done
+        st.put(sM, sW0, tid);
+        dma_units(Pimg + (size_t)ch * PH::kHalves, sP, PH::kUnits, wv, 4, lane);
+        __syncthreads();
+        if (ch + 1 < nch)
+            st.fetch(Mg, wg, ch + 1, tid);
+        const int b = wv;
+        d4 x = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < ExpB<NM>::NKS; ++s) {
+            const int m = 4 * s + g;
+            const double am = (NM % 4 == 0 || m < NM) ? sM[(16 * b + i16) * NMS + m] : 0.0;
+            x = __builtin_amdgcn_mfma_f64_16x16x4f64(am, aT.a[s], x, 0, 0, 0);
+        }
+        _Float16 hv[2][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            x[r] = col_weight(x[r], sW0[16 * b + g + 4 * r], shiftOwn, fbw, wmaxOwn, hv[0][r],
+                              hv[1][r]);
+#pragma unroll
+        for (int pl = 0; pl < 2; ++pl) {
+            *reinterpret_cast<h4 *>(&sXH[((wv * 2 + pl) * kWave + lane) * 4]) =
+                h4{hv[pl][0], hv[pl][1], hv[pl][2], hv[pl][3]};
+        }
+        col_grad_block<NM>(sM, x, wv, ga, lane);
+        dma_wait();
+        __syncthreads();  // halves and pair image
+        h8 ah[2], al[2];
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int src = 2 * kb + (e >> 2);
+                ah[kb][e] = sXH[((src * 2 + 0) * kWave + lane) * 4 + (e & 3)];
+                al[kb][e] = sXH[((src * 2 + 1) * kWave + lane) * 4 + (e & 3)];
+            }
+        pair_hess<NM, NA>(sP, ah, al, acc, wv, 4, lane);
+    }
+    // the bulk's combination order: block sums per lane, then the column's row groups
+    sXF[wv * kWave + lane] = fbw;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            sXG[((wv * NT + t) * kWave + lane) * 4 + r] = ga[t][wv][r];
+    const float wm = col_max(wmaxOwn);
+    if (lane == 0)
+        sRed[wv] = wm;
+    __syncthreads();
+    const double es = exp(shift);
+    const bool ok = h16_range_ok(fmaxf(fmaxf(sRed[0], sRed[1]), fmaxf(sRed[2], sRed[3])));
+    const double sc = pair_scale(es);
+    if (lane < 16) {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            const int t = wv + 4 * i, p = 16 * t + lane;
+            if (t < PH::PT && p < PH::NP)
+                sH[p] = (p == 0 && !ok) ? __builtin_nan("") : (double)acc[i][0] * sc;
+        }
+    }
+    double fdir = 0.0;
+    if (wv == 0) {
+        const double fb[4] = {sXF[lane], sXF[kWave + lane], sXF[2 * kWave + lane],
+                              sXF[3 * kWave + lane]};
+        fdir = __shfl(col_f(fb), 0) * es;
+    } else if (wv == 1 && lane < 16) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int col = 16 * t + lane;
+            if (col < NM) {
+                d4 gb[4];
+#pragma unroll
+                for (int bb = 0; bb < 4; ++bb)
+                    gb[bb][0] = sXG[((bb * NT + t) * kWave + lane) * 4];
+                sG[col] = col_g(gb, 0) * es;
+            }
+        }
+    }
+    return fdir;
+}
+
+// f at up to 16 trial points in the mixed phase for nm <= kPairMaxNm, as the column sweep
+// computes it: point j (alpha at sA + j * NM, j < kw) in column j, each lane's weights summed
+// per ordinate block in chunk and row order, then col_f; M is read from its transposed copy
+// MT[m][q] (leading dimension nqPad), the next chunk's operands in flight.  Returns in lane l
+// the f of point l & 15.
+template <int NM>
+__device__ __forceinline__ double f_trials_cols(const KParams &P, const double *__restrict__ MT,
+                                                const double *__restrict__ wg, const double *sA,
+                                                int kw, double shift, int lane)
+{
+    constexpr int NKS = ExpB<NM>::NKS;
+    const int nch = P.nqPad / kChunk;
+    const size_t ld = (size_t)P.nqPad;
+    const int i16 = lane & 15, g = lane >> 4;
+    ExpB<NM> aT;
+    aT.load(i16 < kw ? sA + i16 * NM : nullptr, lane);
+    auto load = [&](double(&am)[4][NKS], double(&w)[4][4], int ch) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const size_t q0 = (size_t)ch * kChunk + 16 * b;
+#pragma unroll
+            for (int s = 0; s < NKS; ++s) {
+                const int m = 4 * s + g;
+                am[b][s] = (NM % 4 == 0 || m < NM) ? MT[m * ld + q0 + i16] : 0.0;
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                w[b][r] = wg[q0 + g + 4 * r];
+        }
+    };
+    double fb[4] = {0.0, 0.0, 0.0, 0.0};
+    auto term = [&](const double(&am)[4][NKS], const double(&w)[4][4]) {
+        d4 x[4];
+        col_exponents<NM>(aT, [&](int b, int s) { return am[b][s]; }, x);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                fb[b] = fb[b] + w[b][r] * exp(x[b][r] - shift);
+    };
+    double amA[4][NKS], wA[4][4], amB[4][NKS], wB[4][4];
+    load(amA, wA, 0);
+    for (int ch = 0; ch < nch; ch += 2) {
+        if (ch + 1 < nch)
+            load(amB, wB, ch + 1);
+        term(amA, wA);
+        if (ch + 2 < nch)
+            load(amA, wA, ch + 2);
+        if (ch + 1 < nch)
+            term(amB, wB);
+    }
+    return col_f(fb) * exp(shift);
+}
+
 // f at up to four trial points in the mixed phase, as chunk_h16 evaluates it: the points are
 // the rows of the exponent products (alpha of point j at sA + j * NM, j < kw), lane L takes
 // point L >> 4 at ordinates 16 g + (L & 15) of each chunk and sums its weights in chunk and
@@ -2316,7 +2903,8 @@ __device__ __forceinline__ void f_trials_mixed(const KParams &P, const double *_
 template <int NM, bool REF, bool MIXED, int TW>
 __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__restrict__ Mg,
                                                       const double *__restrict__ MT,
-                                                      const double *__restrict__ wg, State S,
+                                                      const double *__restrict__ wg,
+                                                      const _Float16 *__restrict__ Pimg, State S,
                                                       Lists Lc, Lists Ln, double *u_g,
                                                       double *alpha_g, c3d_cell_stats *stats,
                                                       int *status, int passes,
@@ -2344,10 +2932,18 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     __shared__ double sA[TW][KWA * NM];
     __shared__ int sOk[TW * KWA];       // Armijo verdict at each trial point
     __shared__ double sF[1];            // f of the sweep
-    // the mixed sweep's images (H16Chunk), weights and FP64 gradient moments
-    __shared__ __align__(16) _Float16 sBh[MIXED ? H16Chunk<NM>::kBHalves : 8];
-    __shared__ __align__(16) _Float16 sWH[MIXED ? 2 * kChunk : 8];
-    __shared__ __align__(16) double sWE64[MIXED ? kWE64S : 2];
+    // the mixed sweep's images: for nm <= kPairMaxNm (the column sweep) the pair image and,
+    // with four waves, the exchanged halves, block sums and gradient accumulators; else
+    // H16Chunk's image, the weights' halves and FP64 weights
+    constexpr bool COLS = MIXED && NM <= kPairMaxNm;
+    constexpr bool COLS4 = COLS && TW == 4;
+    __shared__ __align__(16) _Float16 sBh[MIXED && !COLS ? H16Chunk<NM>::kBHalves : 8];
+    __shared__ __align__(16) _Float16 sWH[MIXED && !COLS ? 2 * kChunk : 8];
+    __shared__ __align__(16) double sWE64[MIXED && !COLS ? kWE64S : 2];
+    __shared__ __align__(16) _Float16 sP[COLS ? PairHess<NM>::kHalves : 8];
+    __shared__ __align__(16) _Float16 sXH[COLS4 ? 4 * 2 * kWave * 4 : 8];
+    __shared__ double sXF[COLS4 ? 4 * kWave : 1];
+    __shared__ double sXG[COLS4 ? 4 * NT * kWave * 4 : 1];
     __shared__ double sG[NM];
     __shared__ float sRed[4];
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
@@ -2356,7 +2952,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     if ((int)blockIdx.x >= nW)
         return;
     const int li = lane & (G - 1), row = lane / G;
-    if constexpr (MIXED) {
+    if constexpr (MIXED && !COLS) {
         H16Chunk<NM>::clear_pad(sBh, threadIdx.x, TW * kWave);
         __syncthreads();
     }
@@ -2387,11 +2983,18 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 wave_lds_sync();
                 double ft[KWA];
                 if (mx) {
-                    double f4p[4];
-                    f_trials_mixed<NM>(P, MT, wg, sA[wv], kw, C.shift, lane, f4p);
+                    if constexpr (COLS) {
+                        const double fl = f_trials_cols<NM>(P, MT, wg, sA[wv], kw, C.shift, lane);
 #pragma unroll
-                    for (int j = 0; j < KWA; ++j)
-                        ft[j] = j < 4 ? f4p[j] : 0.0;
+                        for (int j = 0; j < KWA; ++j)
+                            ft[j] = __shfl(fl, j);
+                    } else {
+                        double f4p[4];
+                        f_trials_mixed<NM>(P, MT, wg, sA[wv], kw, C.shift, lane, f4p);
+#pragma unroll
+                        for (int j = 0; j < KWA; ++j)
+                            ft[j] = j < 4 ? f4p[j] : 0.0;
+                    }
                 } else {
                     double fk[KW];
                     f_trials<NM, REF ? kFRef : kFp64, KW>(P, MT, wg, sA[wv], kw, 0.0, sW[wv],
@@ -2431,11 +3034,16 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                     if (wv == 0 && lane < NM)
                         sA[0][lane] = C.a;
                     __syncthreads();
-                    ExpA<NM> ax;
-                    ax.load(sA[0], 1, lane);
-                    const double fdir = sweep_mixed_block<NM>(P, Mg, wg, ax, C.shift, sM, sBh, sWH,
-                                                              sWE64, sW[0], sH, sG, sRed, lane,
-                                                              wv);
+                    double fdir;
+                    if constexpr (COLS) {
+                        fdir = sweep_cols_block<NM>(P, Mg, wg, Pimg, sA[0], C.shift, sM, sW[0], sP,
+                                                    sXH, sXF, sXG, sRed, sH, sG, lane, wv);
+                    } else {
+                        ExpA<NM> ax;
+                        ax.load(sA[0], 1, lane);
+                        fdir = sweep_mixed_block<NM>(P, Mg, wg, ax, C.shift, sM, sBh, sWH, sWE64,
+                                                     sW[0], sH, sG, sRed, lane, wv);
+                    }
                     if (wv == 0) {
                         if (lane == 0)
                             sF[0] = fdir;
@@ -2448,7 +3056,17 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 double fdir;
                 if (lane < NM)
                     sA[0][lane] = C.a;
-                if constexpr (REF) {
+                bool colSwept = false;
+                if constexpr (COLS) {
+                    if (mx) {
+                        wave_lds_sync();  // sA[0]
+                        fdir = sweep_cols_wave<NM>(P.nqPad, Mg, wg, Pimg, sA[0], C.shift, sM,
+                                                   sW[0], sP, sH, sG, true, lane);
+                        colSwept = true;
+                    }
+                }
+                if (colSwept) {
+                } else if constexpr (REF) {
                     const RefEntries<NM> E(lane);
                     double h[RefEntries<NM>::NS];
 #pragma unroll
@@ -2505,7 +3123,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                             const int e = lane + kWave * k;
                             const int rr = e / NM, cc = e - rr * NM;
                             sM[rr * NMS + cc] = pre[k];
-                            if constexpr (MIXED)
+                            if constexpr (MIXED && !COLS)
                                 if (mx)
                                     H16Chunk<NM>::put(sBh, rr, cc, pre[k]);
                         }
@@ -2520,10 +3138,12 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                             for (int k = 0; k < NM; ++k)
                                 pre[k] = Mg[base * NM + lane + kWave * k];
                         }
-                        if (mx)
-                            chunk_h16<NM, 1>(sM, sBh, sW[0], ax, shiftOwn, wmaxOwn, sWH, sWE64,
-                                             acc32, gacc, fOwn, lane);
-                        else
+                        if constexpr (MIXED && !COLS) {
+                            if (mx)
+                                chunk_h16<NM, 1>(sM, sBh, sW[0], ax, shiftOwn, wmaxOwn, sWH,
+                                                 sWE64, acc32, gacc, fOwn, lane);
+                        }
+                        if (!mx)
                             chunk_mfma<NM, 1>(sM, wl, sA[0], acc, fpart, lane);
                     }
                     wave_lds_sync();
@@ -2882,6 +3502,68 @@ __global__ __launch_bounds__(kBlock) void k_hessian_fp64(int64_t ncells, int nqP
     }
 }
 
+// H only, mixed mode, nm <= kPairMaxNm: the column sweep's Hessian under the cell's own shift
+// (the largest exponent and quadrature weight, so that every weight is at most one and the
+// largest at least min w / max w), one wave per cell in column 0; H row-major, both triangles.
+template <int NM>
+__global__ __launch_bounds__(kWave) void k_hessian_cols(int64_t ncells, int nqPad,
+                                                        const double *__restrict__ Mg,
+                                                        const double *__restrict__ wg,
+                                                        const _Float16 *__restrict__ Pimg,
+                                                        const double *__restrict__ alpha_g,
+                                                        double *hs)
+{
+    constexpr int NMS = NM | 1;
+    __shared__ double sM[kChunk * NMS];
+    __shared__ double sW[kChunk];
+    __shared__ __align__(16) _Float16 sP[PairHess<NM>::kHalves];
+    __shared__ double sA[NM];
+    __shared__ double sH[PairHess<NM>::NP];
+    __shared__ double sG[NM];
+    const int lane = threadIdx.x;
+    const int64_t cell = blockIdx.x;
+    if (cell >= ncells)
+        return;
+    if (lane < NM)
+        sA[lane] = alpha_g[cell * NM + lane];
+    double xmax = -DBL_MAX, wtop = 0.0;
+    for (int c = 0; c < nqPad / kChunk; ++c) {
+        wave_lds_sync();
+        for (int e = lane; e < kChunk * NM; e += kWave) {
+            const int rr = e / NM, cc = e - rr * NM;
+            sM[rr * NMS + cc] = Mg[(size_t)c * kChunk * NM + e];
+        }
+        sW[lane] = wg[(size_t)c * kChunk + lane];
+        wave_lds_sync();
+        if (sW[lane] > 0.0) {
+            double x = 0.0;
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                x = fma(sA[i], sM[lane * NMS + i], x);
+            xmax = fmax(xmax, x);
+            wtop = fmax(wtop, sW[lane]);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        xmax = fmax(xmax, __shfl_xor(xmax, o));
+        wtop = fmax(wtop, __shfl_xor(wtop, o));
+    }
+    const double shift = xmax + log(wtop);
+    wave_lds_sync();
+    (void)sweep_cols_wave<NM>(nqPad, Mg, wg, Pimg, sA, shift, sM, sW, sP, sH, sG, false, lane);
+    wave_lds_sync();
+    double *H = hs + cell * NM * NM;
+    for (int p = lane; p < PairHess<NM>::NP; p += kWave) {
+        int i = 0;
+        while (tri(i + 1) <= p)
+            ++i;
+        const int j = p - tri(i);
+        H[i * NM + j] = sH[p];
+        H[j * NM + i] = sH[p];
+    }
+}
+
 // H only, mixed mode (FP32 matrix cores, zero shift): one wave per cell.
 template <int NM>
 __global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nqPad,
@@ -3177,15 +3859,11 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
         if (ev)
             (void)hipEventRecord(ev[0], s);
         if (mixed) {
-            bool two = true;
-            if constexpr (NM <= 16) {
-                two = ctx->tune.mixedWaves == 2;
-                if (!two)
-                    hipLaunchKernelGGL((k_sweep_mixed<NM, 3>), dim3(grid_cap(ctx, 8)),
-                                       dim3(kBlock), 0, s, P, ctx->d_M, ctx->d_wpad, S, Lc.Wm,
-                                       Lc.nm);
-            }
-            if (two)
+            if constexpr (NM <= kPairMaxNm)
+                hipLaunchKernelGGL(k_sweep_cols<NM>, dim3(grid_cap(ctx, 1)), dim3(kCBlock), 0, s,
+                                   P, ctx->d_M, ctx->d_wpad, (const _Float16 *)ctx->d_P16, S,
+                                   Lc.Wm, Lc.nm);
+            else
                 hipLaunchKernelGGL((k_sweep_mixed<NM, 2>), dim3(grid_cap(ctx, 8)), dim3(kBlock),
                                    0, s, P, ctx->d_M, ctx->d_wpad, S, Lc.Wm, Lc.nm);
         }
@@ -3220,8 +3898,8 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
             (void)hipEventRecord(ev[0], s);
 #define C3D_TAIL(R, X, TW)                                                                   \
     hipLaunchKernelGGL((k_tail<NM, R, X, TW>), dim3(grid), dim3(TW * kWave), 0, s, P, ctx->d_M,  \
-                       ctx->d_MT, ctx->d_wpad, S, Lc, Ln, u, alpha, stats, ctx->d_status, K,   \
-                       cntTail)
+                       ctx->d_MT, ctx->d_wpad, (const _Float16 *)ctx->d_P16, S, Lc, Ln, u,     \
+                       alpha, stats, ctx->d_status, K, cntTail)
         if (ref && wide)
             C3D_TAIL(true, false, 4);
         else if (ref)
@@ -3382,15 +4060,18 @@ hipError_t launch_hessian_fp64(c3d_ctx_s *ctx, int64_t ncells, const double *alp
         return hipSuccess;
     dim3 grid((unsigned)((ncells + kWavesPerBlock - 1) / kWavesPerBlock));
     if (ctx->precision == C3D_PREC_MIXED) {
-#define C3D_HM(NMV) hipLaunchKernelGGL(k_hessian_mixed<NMV>, grid, dim3(kBlock), 0, s, ncells, ctx->nqPad, ctx->d_M, ctx->d_wpad, alpha, hs)
+#define C3D_HC(NMV) hipLaunchKernelGGL(k_hessian_cols<NMV>, dim3((unsigned)ncells), dim3(kWave), 0, s, ncells, ctx->nqPad, ctx->d_M, ctx->d_wpad, (const _Float16 *)ctx->d_P16, alpha, hs)
         switch (ctx->t.nm) {
-        case 4: C3D_HM(4); break;
-        case 9: C3D_HM(9); break;
-        case 16: C3D_HM(16); break;
-        case 25: C3D_HM(25); break;
+        case 4: C3D_HC(4); break;
+        case 9: C3D_HC(9); break;
+        case 16: C3D_HC(16); break;
+        case 25:
+            hipLaunchKernelGGL(k_hessian_mixed<25>, grid, dim3(kBlock), 0, s, ncells, ctx->nqPad,
+                               ctx->d_M, ctx->d_wpad, alpha, hs);
+            break;
         default: return hipErrorInvalidValue;
         }
-#undef C3D_HM
+#undef C3D_HC
         return hipGetLastError();
     }
     switch (ctx->t.nm) {
