@@ -40,6 +40,11 @@ __host__ __device__ constexpr int pair_offset(int nm, int qc, int p, int plane)
            4 * ((qc & 31) >> 4) + ((qc >> 2) & 3);
 }
 
+// The column sweep's row image (nm <= kPairMaxNm): per chunk the kChunk rows of M, leading
+// dimension nm | 1 (the pad column zero), then the chunk's kChunk weights, in FP64 -- the LDS
+// layout of the sweeps, copied whole by LDS DMA ((nm | 1) + 1) / 2 KB a chunk.
+__host__ __device__ constexpr int row_chunk_doubles(int nm) { return ((nm | 1) + 1) * kChunk; }
+
 // Host-side tables (init.cpp:423-548, 287-337, 199-234), built by tables.cpp or copied from
 // the caller.
 struct HostTables {
@@ -94,6 +99,7 @@ struct c3d_ctx_s {
     double *d_MT = nullptr;    // [nm][nqPad], the transpose of d_M
     double *d_wpad = nullptr;  // [nqPad], entries >= nq zero
     uint16_t *d_P16 = nullptr;  // the pair image (nm <= kPairMaxNm), nqPad / kChunk chunks
+    double *d_R = nullptr;      // the row image (nm <= kPairMaxNm), nqPad / kChunk chunks
     double *d_M2 = nullptr;    // [nqPad][nm2]
     double *d_beta = nullptr, *d_betaVal = nullptr, *d_pnOps = nullptr;
     int32_t *d_betaColInd = nullptr, *d_betaRowPtr = nullptr;

@@ -100,6 +100,15 @@ int finish_ctx(c3d_ctx_s *ctx)
         std::vector<uint16_t> img;
         pair_image(t, ctx->nqPad, img);
         UP(d_P16, img);
+        const int nms = t.nm | 1, rc = row_chunk_doubles(t.nm);
+        std::vector<double> rows((size_t)ctx->nqPad / kChunk * rc, 0.0);
+        for (int q = 0; q < t.nq; ++q) {
+            double *dst = rows.data() + (size_t)(q / kChunk) * rc;
+            for (int i = 0; i < t.nm; ++i)
+                dst[(q % kChunk) * nms + i] = t.M[(size_t)q * t.nm + i];
+            dst[kChunk * nms + q % kChunk] = t.w[q];
+        }
+        UP(d_R, rows);
     }
     if (!t.M2.empty()) {
         std::vector<double> M2p((size_t)ctx->nqPad * t.nm2, 0.0);
@@ -255,7 +264,7 @@ int c3d_destroy(c3d_ctx ctx)
     if (!ctx)
         return C3D_OK;
     void *ptrs[] = {ctx->d_w, ctx->d_xi, ctx->d_eta, ctx->d_mu, ctx->d_M, ctx->d_MT,
-                    ctx->d_wpad, ctx->d_P16, ctx->d_M2, ctx->d_beta, ctx->d_betaVal, ctx->d_pnOps, ctx->d_betaColInd,
+                    ctx->d_wpad, ctx->d_P16, ctx->d_R, ctx->d_M2, ctx->d_beta, ctx->d_betaVal, ctx->d_pnOps, ctx->d_betaColInd,
                     ctx->d_betaRowPtr, ctx->d_sweepCount, ctx->d_status, ctx->ansatz.p,
                     ctx->uOld.p, ctx->flux.p, ctx->closureState.p};
     for (void *p : ptrs)

@@ -1685,73 +1685,56 @@ __device__ __forceinline__ double col_g(const d4 (&ga)[4], int r)
     return (ga[0][r] + ga[1][r]) + (ga[2][r] + ga[3][r]);
 }
 
-// Chunk stager of the FP64 rows (sM, leading dimension NM | 1) and the weights for NTH
-// threads: fetch() loads a chunk into registers, put() writes it to LDS.
-template <int NM, int NTH>
-struct RowStager {
-    static constexpr int NMS = NM | 1;
-    static constexpr int kItems = kChunk * NM;
-    static constexpr int kPer = (kItems + NTH - 1) / NTH;
-    double pre[kPer];
-    double prew;
-    __device__ __forceinline__ void fetch(const double *__restrict__ Mg,
-                                          const double *__restrict__ wg, int ch, int tid)
-    {
-        const double *src = Mg + (size_t)ch * kItems;
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int e = tid + NTH * k;
-            pre[k] = (kItems % NTH == 0 || e < kItems) ? src[e] : 0.0;
-        }
-        prew = tid < kChunk ? wg[(size_t)ch * kChunk + tid] : 0.0;
-    }
-    __device__ __forceinline__ void put(double *sM, double *sW, int tid) const
-    {
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int e = tid + NTH * k;
-            if (kItems % NTH == 0 || e < kItems) {
-                const int r = e / NM;
-                sM[r * NMS + (e - r * NM)] = pre[k];
-            }
-        }
-        if (tid < kChunk)
-            sW[tid] = prew;
-    }
-};
-
 // The LDS DMA (global_load_lds_dwordx4): lane i of the issuing wave copies the 16 bytes at
 // src + 16 i to dst + 16 i.  dma_units copies `units` 1 KB units, unit j by the wave j % nw of
-// the caller's nw waves (wv its index); the copy completes asynchronously, and each wave waits
-// for its own (dma_wait: vmcnt 0) before a barrier publishes it.  lds_reads_done waits for the
-// wave's outstanding LDS reads, before a copy overwrites what they read.
-__device__ __forceinline__ void dma_units(const _Float16 *src, _Float16 *dst, int units, int wv,
-                                          int nw, int lane)
+// the caller's nw waves (wv its index), one instruction per unit; the copy completes
+// asynchronously, and each wave waits for its own (dma_wait: vmcnt 0; dma_wait_n<N>: all but
+// its last N vector-memory instructions) before a barrier publishes it.  lds_reads_done waits
+// for the wave's outstanding LDS reads, before a copy overwrites what they read.
+__device__ __forceinline__ void dma_units(const void *src, void *dst, int units, int wv, int nw,
+                                          int lane)
 {
     for (int j = wv; j < units; j += nw)
         __builtin_amdgcn_global_load_lds(
-            (const void *)(src + j * 512 + lane * 8),
-            (__attribute__((address_space(3))) void *)(dst + j * 512), 16, 0, 0);
+            (const void *)((const char *)src + j * 1024 + lane * 16),
+            (__attribute__((address_space(3))) void *)((char *)dst + j * 1024), 16, 0, 0);
 }
 __device__ __forceinline__ void dma_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+template <int N>
+__device__ __forceinline__ void dma_wait_n()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt holds 6 bits");
+    __builtin_amdgcn_s_waitcnt(0x0F70 | (N & 15) | ((N >> 4) << 14));
+}
 __device__ __forceinline__ void lds_reads_done() { __builtin_amdgcn_s_waitcnt(0xC07F); }
 
-// One cell's column sweep on one wave, the cell in column 0 (the other columns compute nothing
-// that is kept): the chunks staged by the wave itself, the pair image by LDS DMA into sP
-// (single-buffered: a chunk's copy is issued once the previous chunk's pair products have read
-// sP and is waited for before the chunk's own pair products).  Results as the bulk sweep
-// leaves them for column 0: H (packed lower triangle) in sH, its first entry NaN where the
-// weights left the FP16 range if mark, the gradient moments in sG; returns f.  sA: the cell's
-// alpha.
+// The row image's geometry (c3d_internal.h): a chunk's rows at 0, leading dimension NM | 1, its
+// weights after them
 template <int NM>
-__device__ __forceinline__ double sweep_cols_wave(int nqPad, const double *__restrict__ Mg,
-                                                  const double *__restrict__ wg,
+struct RowImg {
+    static constexpr int NMS = NM | 1;
+    static constexpr int kDoubles = row_chunk_doubles(NM);
+    static constexpr int kUnits = kDoubles * 8 / 1024;
+    static constexpr int kW = kChunk * NMS;  // offset of the weights
+};
+
+// One cell's column sweep on one wave, the cell in column 0 (the other columns compute nothing
+// that is kept), its chunks staged by LDS DMA from the row image (double-buffered, sR0/sR1)
+// and the pair image (single-buffered, sP: a chunk's copy is issued once the previous chunk's
+// pair products have read sP and is waited for before its own).  The waits count the wave's
+// copies: oldest first, the outstanding ones are a chunk's rows, its pair image, the next
+// chunk's rows.  Results as the bulk sweep leaves them for column 0: H (packed lower triangle)
+// in sH, its first entry NaN where the weights left the FP16 range if mark, the gradient
+// moments in sG; returns f.  sA: the cell's alpha.
+template <int NM>
+__device__ __forceinline__ double sweep_cols_wave(int nqPad, const double *__restrict__ Rimg,
                                                   const _Float16 *__restrict__ Pimg,
-                                                  const double *sA, double shift, double *sM,
-                                                  double *sW0, _Float16 *sP, double *sH,
+                                                  const double *sA, double shift, double *sR0,
+                                                  double *sR1, _Float16 *sP, double *sH,
                                                   double *sG, bool mark, int lane)
 {
     using PH = PairHess<NM>;
+    using RI = RowImg<NM>;
     constexpr int NT = (NM + 15) / 16;
     constexpr int PT = PH::PT;
     const bool own = (lane & 15) == 0;
@@ -1771,23 +1754,27 @@ __device__ __forceinline__ double sweep_cols_wave(int nqPad, const double *__res
     for (int t = 0; t < PT; ++t)
         acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
     const int nch = nqPad / kChunk;
-    RowStager<NM, kWave> st;
-    st.fetch(Mg, wg, 0, lane);
     wave_lds_sync();
-    lds_reads_done();  // earlier readers of sP are done
+    lds_reads_done();  // earlier readers of sR0, sR1 and sP are done
+    dma_units(Rimg, sR0, RI::kUnits, 0, 1, lane);
     dma_units(Pimg, sP, PH::kUnits, 0, 1, lane);
     for (int ch = 0; ch < nch; ++ch) {
-        wave_lds_sync();  // the previous chunk's readers of sM and sW0 are done
-        st.put(sM, sW0, lane);
-        if (ch + 1 < nch)
-            st.fetch(Mg, wg, ch + 1, lane);
+        double *cur = (ch & 1) ? sR1 : sR0;
+        double *nxt = (ch & 1) ? sR0 : sR1;
+        dma_wait_n<PH::kUnits>();  // this chunk's rows (its pair image may still be in flight)
         wave_lds_sync();
-        chunk_cols<NM>(sM, sW0, sP, aT, shiftOwn, wmaxOwn, fb, gacc, acc, lane, [] {
-            dma_wait();
+        const bool more = ch + 1 < nch;
+        if (more)  // nxt was last read by the previous chunk, whose reads are done
+            dma_units(Rimg + (size_t)(ch + 1) * RI::kDoubles, nxt, RI::kUnits, 0, 1, lane);
+        chunk_cols<NM>(cur, cur + RI::kW, sP, aT, shiftOwn, wmaxOwn, fb, gacc, acc, lane, [&] {
+            if (more)
+                dma_wait_n<RI::kUnits>();  // the pair image (the next rows may be in flight)
+            else
+                dma_wait();
             wave_lds_sync();
         });
-        if (ch + 1 < nch) {
-            lds_reads_done();  // the pair products have read sP
+        if (more) {
+            lds_reads_done();  // the chunk's reads of cur and sP are done
             dma_units(Pimg + (size_t)(ch + 1) * PH::kHalves, sP, PH::kUnits, 0, 1, lane);
         }
     }
@@ -1818,25 +1805,23 @@ constexpr int kCBlock = kCWaves * kWave;
 constexpr int kCCells = 16;  // cells per wave
 
 // Sweep pass of the mixed phase for nm <= kPairMaxNm (the column sweep): 16 cells per wave, 128
-// per block sharing each chunk; the chunk's FP64 rows and weights staged through registers, its
-// pair image by LDS DMA, both double-buffered (LDS 90 KB at nm = 16).  Writes H (packed lower
+// per block sharing each chunk; the chunk's row image and pair image staged by LDS DMA, both
+// double-buffered (LDS 90 KB at nm = 16).  Writes H (packed lower
 // triangle, first entry NaN where the weights left the FP16 range), sum_q w e (fdir) and the
 // FP64 gradient moments (g slot).
 template <int NM>
-__global__ __launch_bounds__(kCBlock) void k_sweep_cols(KParams P, const double *__restrict__ Mg,
-                                                        const double *__restrict__ wg,
+__global__ __launch_bounds__(kCBlock) void k_sweep_cols(KParams P, const double *__restrict__ Rimg,
                                                         const _Float16 *__restrict__ Pimg,
                                                         State S, const int *__restrict__ W,
                                                         const int *__restrict__ nWp)
 {
     using PH = PairHess<NM>;
+    using RI = RowImg<NM>;
     constexpr int NT = (NM + 15) / 16;
     constexpr int PT = PH::PT;
-    constexpr int NMS = NM | 1;
     __shared__ __align__(16) _Float16 sP[2][PH::kHalves];
-    __shared__ double sM[2][kChunk * NMS];
-    __shared__ double sW[2][kChunk];
-    const int tid = threadIdx.x, lane = tid & 63, wv = uniform_int(tid >> 6);
+    __shared__ __align__(16) double sR[2][RI::kDoubles];
+    const int lane = threadIdx.x & 63, wv = uniform_int(threadIdx.x >> 6);
     const int i16 = lane & 15, g = lane >> 4;
     const int nch = P.nqPad / kChunk;
     const int nW = uniform_int(*nWp);
@@ -1862,23 +1847,22 @@ __global__ __launch_bounds__(kCBlock) void k_sweep_cols(KParams P, const double 
 #pragma unroll
         for (int t = 0; t < PT; ++t)
             acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-        RowStager<NM, kCBlock> st;
-        st.fetch(Mg, wg, 0, tid);
         __syncthreads();  // the previous cells' last chunk is consumed
-        dma_units(Pimg, sP[0], PH::kUnits, wv, kCWaves, lane);
+        dma_units(Rimg, sR[0], RI::kUnits, wv, kCWaves, lane);
+        dma_units(Pimg, sP[0], PH::kUnits, (wv + RI::kUnits) % kCWaves, kCWaves, lane);
         for (int ch = 0; ch < nch; ++ch) {
             const int b = ch & 1;
-            st.put(sM[b], sW[b], tid);
             dma_wait();
             __syncthreads();
             if (ch + 1 < nch) {
-                dma_units(Pimg + (size_t)(ch + 1) * PH::kHalves, sP[b ^ 1], PH::kUnits, wv,
+                dma_units(Rimg + (size_t)(ch + 1) * RI::kDoubles, sR[b ^ 1], RI::kUnits, wv,
                           kCWaves, lane);
-                st.fetch(Mg, wg, ch + 1, tid);
+                dma_units(Pimg + (size_t)(ch + 1) * PH::kHalves, sP[b ^ 1], PH::kUnits,
+                          (wv + RI::kUnits) % kCWaves, kCWaves, lane);
             }
             if (any)
-                chunk_cols<NM>(sM[b], sW[b], sP[b], aT, shift, wmaxOwn, fb, gacc, acc, lane,
-                               [] {});
+                chunk_cols<NM>(sR[b], sR[b] + RI::kW, sP[b], aT, shift, wmaxOwn, fb, gacc, acc,
+                               lane, [] {});
         }
         if (!any)
             continue;
@@ -2636,21 +2620,23 @@ __device__ __forceinline__ double sweep_mixed_block(const KParams &P, const doub
 // wave w takes ordinate block w of every chunk (its exponent chain, weights, running block sum
 // and gradient accumulator w), the waves exchange the FP16 halves through sXH for the pair
 // products (wave w: tiles w, w + 4, ..), and f, the largest weight and the gradient are
-// combined in the bulk's order at the end (sXF, sXG, sRed).  Results as sweep_cols_wave's.
+// combined in the bulk's order at the end (sXF, sXG, sRed).  The row and pair images are staged
+// by LDS DMA, double-buffered (sR0/sR1, sP0/sP1).  Results as sweep_cols_wave's.
 template <int NM>
-__device__ __forceinline__ double sweep_cols_block(const KParams &P, const double *__restrict__ Mg,
-                                                   const double *__restrict__ wg,
+__device__ __forceinline__ double sweep_cols_block(const KParams &P, const double *__restrict__ Rimg,
                                                    const _Float16 *__restrict__ Pimg,
-                                                   const double *sA, double shift, double *sM,
-                                                   double *sW0, _Float16 *sP, _Float16 *sXH,
-                                                   double *sXF, double *sXG, float *sRed,
-                                                   double *sH, double *sG, int lane, int wv)
+                                                   const double *sA, double shift, double *sR0,
+                                                   double *sR1, _Float16 *sP0, _Float16 *sP1,
+                                                   _Float16 *sXH, double *sXF, double *sXG,
+                                                   float *sRed, double *sH, double *sG, int lane,
+                                                   int wv)
 {
     using PH = PairHess<NM>;
+    using RI = RowImg<NM>;
     constexpr int NT = (NM + 15) / 16;
     constexpr int NA = (PH::PT + 3) / 4;
     constexpr int NMS = NM | 1;
-    const int tid = threadIdx.x, i16 = lane & 15, g = lane >> 4;
+    const int i16 = lane & 15, g = lane >> 4;
     const bool own = i16 == 0;
     ExpB<NM> aT;
     aT.load(own ? sA : nullptr, lane);
@@ -2668,15 +2654,20 @@ __device__ __forceinline__ double sweep_cols_block(const KParams &P, const doubl
     for (int i = 0; i < NA; ++i)
         acc[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
     const int nch = P.nqPad / kChunk;
-    RowStager<NM, kBlock> st;
-    st.fetch(Mg, wg, 0, tid);
+    __syncthreads();  // earlier readers of the buffers are done
+    dma_units(Rimg, sR0, RI::kUnits, wv, 4, lane);
+    dma_units(Pimg, sP0, PH::kUnits, wv, 4, lane);
     for (int ch = 0; ch < nch; ++ch) {
-        __syncthreads();  // the previous chunk's readers are done
-        st.put(sM, sW0, tid);
-        dma_units(Pimg + (size_t)ch * PH::kHalves, sP, PH::kUnits, wv, 4, lane);
-        __syncthreads();
-        if (ch + 1 < nch)
-            st.fetch(Mg, wg, ch + 1, tid);
+        const double *sM = (ch & 1) ? sR1 : sR0;
+        const _Float16 *sP = (ch & 1) ? sP1 : sP0;
+        dma_wait();
+        __syncthreads();  // the chunk's images; the previous chunk's readers are done
+        if (ch + 1 < nch) {
+            dma_units(Rimg + (size_t)(ch + 1) * RI::kDoubles, (ch & 1) ? sR0 : sR1, RI::kUnits,
+                      wv, 4, lane);
+            dma_units(Pimg + (size_t)(ch + 1) * PH::kHalves, (ch & 1) ? sP0 : sP1, PH::kUnits,
+                      wv, 4, lane);
+        }
         const int b = wv;
         d4 x = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -2688,16 +2679,14 @@ __device__ __forceinline__ double sweep_cols_block(const KParams &P, const doubl
         _Float16 hv[2][4];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-            x[r] = col_weight(x[r], sW0[16 * b + g + 4 * r], shiftOwn, fbw, wmaxOwn, hv[0][r],
-                              hv[1][r]);
+            x[r] = col_weight(x[r], sM[RI::kW + 16 * b + g + 4 * r], shiftOwn, fbw, wmaxOwn,
+                              hv[0][r], hv[1][r]);
 #pragma unroll
-        for (int pl = 0; pl < 2; ++pl) {
+        for (int pl = 0; pl < 2; ++pl)
             *reinterpret_cast<h4 *>(&sXH[((wv * 2 + pl) * kWave + lane) * 4]) =
                 h4{hv[pl][0], hv[pl][1], hv[pl][2], hv[pl][3]};
-        }
         col_grad_block<NM>(sM, x, wv, ga, lane);
-        dma_wait();
-        __syncthreads();  // halves and pair image
+        __syncthreads();  // the halves
         h8 ah[2], al[2];
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb)
@@ -2754,54 +2743,63 @@ __device__ __forceinline__ double sweep_cols_block(const KParams &P, const doubl
 
 // f at up to 16 trial points in the mixed phase for nm <= kPairMaxNm, as the column sweep
 // computes it: point j (alpha at sA + j * NM, j < kw) in column j, each lane's weights summed
-// per ordinate block in chunk and row order, then col_f; M is read from its transposed copy
-// MT[m][q] (leading dimension nqPad), the next chunk's operands in flight.  Returns in lane l
-// the f of point l & 15.
+// per ordinate block in chunk and row order, then col_f.  M is read from its transposed copy
+// MT[m][q] (leading dimension nqPad) half a chunk (two blocks) at a time, the next half's
+// operands in flight.  Returns in lane l the f of point l & 15.
 template <int NM>
 __device__ __forceinline__ double f_trials_cols(const KParams &P, const double *__restrict__ MT,
                                                 const double *__restrict__ wg, const double *sA,
                                                 int kw, double shift, int lane)
 {
     constexpr int NKS = ExpB<NM>::NKS;
-    const int nch = P.nqPad / kChunk;
+    const int nh = 2 * (P.nqPad / kChunk);
     const size_t ld = (size_t)P.nqPad;
     const int i16 = lane & 15, g = lane >> 4;
     ExpB<NM> aT;
     aT.load(i16 < kw ? sA + i16 * NM : nullptr, lane);
-    auto load = [&](double(&am)[4][NKS], double(&w)[4][4], int ch) {
+    // half hc: blocks 2 (hc & 1) and 2 (hc & 1) + 1 of chunk hc >> 1
+    auto load = [&](double(&am)[2][NKS], double(&w)[2][4], int hc) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const size_t q0 = (size_t)ch * kChunk + 16 * b;
+        for (int bb = 0; bb < 2; ++bb) {
+            const size_t q0 = (size_t)(hc >> 1) * kChunk + 16 * (2 * (hc & 1) + bb);
 #pragma unroll
             for (int s = 0; s < NKS; ++s) {
                 const int m = 4 * s + g;
-                am[b][s] = (NM % 4 == 0 || m < NM) ? MT[m * ld + q0 + i16] : 0.0;
+                am[bb][s] = (NM % 4 == 0 || m < NM) ? MT[m * ld + q0 + i16] : 0.0;
             }
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                w[b][r] = wg[q0 + g + 4 * r];
+                w[bb][r] = wg[q0 + g + 4 * r];
         }
     };
     double fb[4] = {0.0, 0.0, 0.0, 0.0};
-    auto term = [&](const double(&am)[4][NKS], const double(&w)[4][4]) {
-        d4 x[4];
-        col_exponents<NM>(aT, [&](int b, int s) { return am[b][s]; }, x);
+    auto term = [&](const double(&am)[2][NKS], const double(&w)[2][4], int hc) {
+        d4 x[2] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int s = 0; s < NKS; ++s)
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                fb[b] = fb[b] + w[b][r] * exp(x[b][r] - shift);
+            for (int bb = 0; bb < 2; ++bb)
+                x[bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[bb][s], aT.a[s], x[bb], 0, 0, 0);
+        const int b0 = 2 * (hc & 1);
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double we = w[bb][r] * exp(x[bb][r] - shift);
+                if (b0 == 0)
+                    fb[bb] = fb[bb] + we;
+                else
+                    fb[2 + bb] = fb[2 + bb] + we;
+            }
     };
-    double amA[4][NKS], wA[4][4], amB[4][NKS], wB[4][4];
+    double amA[2][NKS], wA[2][4], amB[2][NKS], wB[2][4];
     load(amA, wA, 0);
-    for (int ch = 0; ch < nch; ch += 2) {
-        if (ch + 1 < nch)
-            load(amB, wB, ch + 1);
-        term(amA, wA);
-        if (ch + 2 < nch)
-            load(amA, wA, ch + 2);
-        if (ch + 1 < nch)
-            term(amB, wB);
+    for (int hc = 0; hc < nh; hc += 2) {
+        load(amB, wB, hc + 1);
+        term(amA, wA, hc);
+        if (hc + 2 < nh)
+            load(amA, wA, hc + 2);
+        term(amB, wB, hc + 1);
     }
     return col_f(fb) * exp(shift);
 }
@@ -2904,7 +2902,8 @@ template <int NM, bool REF, bool MIXED, int TW>
 __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__restrict__ Mg,
                                                       const double *__restrict__ MT,
                                                       const double *__restrict__ wg,
-                                                      const _Float16 *__restrict__ Pimg, State S,
+                                                      const _Float16 *__restrict__ Pimg,
+                                                      const double *__restrict__ Rimg, State S,
                                                       Lists Lc, Lists Ln, double *u_g,
                                                       double *alpha_g, c3d_cell_stats *stats,
                                                       int *status, int passes,
@@ -2917,10 +2916,20 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     // evaluation's (f costs nq (NM + exp) per point)
     constexpr int KW = REF ? 1 : (NM <= 9 ? 8 : (NM <= 16 ? 2 : 1));
     // in the mixed phase the points are the rows of the exponent products: four per wave
-    constexpr int KWM = NM <= 16 ? 4 : 2;
-    constexpr int KWA = MIXED && KWM > KW ? KWM : KW;
+    // (the column sweep's trials: the 16 columns of its products, as cheap as four)
+    constexpr int KWM = NM <= kPairMaxNm ? 16 : 2;
+    constexpr int KWA = MIXED && NM > kPairMaxNm && KWM > KW ? KWM : KW;  // register arrays
+    constexpr int KWS = MIXED && KWM > KW ? KWM : KW;                     // LDS slots
     static_assert(!(REF && MIXED), "one arithmetic mode");
-    __shared__ double sM[kChunk * NMS];
+    // the column sweep's images (nm <= kPairMaxNm, mixed): rows double-buffered, the pair image
+    // double-buffered with four waves, else single; the first rows buffer also serves as the
+    // other paths' chunk rows (sM)
+    constexpr bool COLS = MIXED && NM <= kPairMaxNm;
+    constexpr bool COLS4 = COLS && TW == 4;
+    __shared__ __align__(16) double sR[COLS ? 2 : 1][COLS ? RowImg<NM>::kDoubles : 1];
+    __shared__ __align__(16) _Float16 sP[COLS4 ? 2 : 1][COLS ? PairHess<NM>::kHalves : 8];
+    __shared__ double sMo[COLS ? 1 : kChunk * NMS];
+    double *const sM = COLS ? &sR[0][0] : sMo;
     __shared__ double sH[NM * NM];
     // the state machine runs on 16-lane rows for NM <= 16 (DPP broadcasts; the four rows of a
     // wave hold identical copies), else on the whole wave
@@ -2929,18 +2938,15 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     __shared__ double sT[TW][RW][NM * (NM + 1)];
     __shared__ double sW[TW][kChunk];
     __shared__ double sE[TW][kChunk];
-    __shared__ double sA[TW][KWA * NM];
-    __shared__ int sOk[TW * KWA];       // Armijo verdict at each trial point
+    __shared__ double sA[TW][KWS * NM];
+    __shared__ int sOk[TW * KWS];       // Armijo verdict at each trial point
     __shared__ double sF[1];            // f of the sweep
-    // the mixed sweep's images: for nm <= kPairMaxNm (the column sweep) the pair image and,
-    // with four waves, the exchanged halves, block sums and gradient accumulators; else
-    // H16Chunk's image, the weights' halves and FP64 weights
-    constexpr bool COLS = MIXED && NM <= kPairMaxNm;
-    constexpr bool COLS4 = COLS && TW == 4;
+    // the mixed sweep's images: for the column sweep with four waves the exchanged halves,
+    // block sums and gradient accumulators; else H16Chunk's image, the weights' halves and
+    // FP64 weights
     __shared__ __align__(16) _Float16 sBh[MIXED && !COLS ? H16Chunk<NM>::kBHalves : 8];
     __shared__ __align__(16) _Float16 sWH[MIXED && !COLS ? 2 * kChunk : 8];
     __shared__ __align__(16) double sWE64[MIXED && !COLS ? kWE64S : 2];
-    __shared__ __align__(16) _Float16 sP[COLS ? PairHess<NM>::kHalves : 8];
     __shared__ __align__(16) _Float16 sXH[COLS4 ? 4 * 2 * kWave * 4 : 8];
     __shared__ double sXF[COLS4 ? 4 * kWave : 1];
     __shared__ double sXG[COLS4 ? 4 * NT * kWave * 4 : 1];
@@ -2971,45 +2977,68 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 double tk = C.tls;
                 for (int k = 0; k < wv * kw; ++k)
                     tk = 0.5 * tk;
-                double aj[KWA], tj[KWA];
-#pragma unroll
-                for (int j = 0; j < KWA; ++j) {
-                    tj[j] = tk;
-                    aj[j] = (wv == 0 && j == 0) ? C.a : (li < NM ? C.aOld + tk * C.d : 0.0);
-                    if (j < kw && lane < NM)
-                        sA[wv][j * NM + lane] = aj[j];
-                    tk = 0.5 * tk;
-                }
-                wave_lds_sync();
-                double ft[KWA];
-                if (mx) {
-                    if constexpr (COLS) {
+                auto point = [&](int j, double t) {
+                    return (wv == 0 && j == 0) ? C.a : (li < NM ? C.aOld + t * C.d : 0.0);
+                };
+                bool judged = false;
+                if constexpr (MIXED && NM <= kPairMaxNm) {
+                    if (mx) {  // the column sweep's trials: no per-point registers
+                        const double t0 = tk;
+#pragma unroll 1
+                        for (int j = 0; j < kw; ++j) {
+                            if (lane < NM)
+                                sA[wv][j * NM + lane] = point(j, tk);
+                            tk = 0.5 * tk;
+                        }
+                        wave_lds_sync();
                         const double fl = f_trials_cols<NM>(P, MT, wg, sA[wv], kw, C.shift, lane);
+                        tk = t0;
+#pragma unroll 1
+                        for (int j = 0; j < kw; ++j) {
+                            const bool ok =
+                                C.template accepts_at<REF>(__shfl(fl, j), point(j, tk), tk);
+                            if (lane == 0)
+                                sOk[wv * kw + j] = ok;
+                            tk = 0.5 * tk;
+                        }
+                        judged = true;
+                    }
+                }
+                if (!judged) {
+                    double aj[KWA], tj[KWA];
 #pragma unroll
-                        for (int j = 0; j < KWA; ++j)
-                            ft[j] = __shfl(fl, j);
-                    } else {
+                    for (int j = 0; j < KWA; ++j) {
+                        tj[j] = tk;
+                        aj[j] = point(j, tk);
+                        if (j < kw && lane < NM)
+                            sA[wv][j * NM + lane] = aj[j];
+                        tk = 0.5 * tk;
+                    }
+                    wave_lds_sync();
+                    double ft[KWA];
+                    if (mx && NM > kPairMaxNm) {
                         double f4p[4];
                         f_trials_mixed<NM>(P, MT, wg, sA[wv], kw, C.shift, lane, f4p);
 #pragma unroll
                         for (int j = 0; j < KWA; ++j)
                             ft[j] = j < 4 ? f4p[j] : 0.0;
+                    } else {
+                        double fk[KW];
+                        f_trials<NM, REF ? kFRef : kFp64, KW>(P, MT, wg, sA[wv], kw, 0.0, sW[wv],
+                                                              sE[wv], lane, fk);
+#pragma unroll
+                        for (int j = 0; j < KWA; ++j)
+                            ft[j] = j < KW ? fk[j] : 0.0;
                     }
-                } else {
-                    double fk[KW];
-                    f_trials<NM, REF ? kFRef : kFp64, KW>(P, MT, wg, sA[wv], kw, 0.0, sW[wv],
-                                                          sE[wv], lane, fk);
+                    // each wave judges its own points; the search then walks the verdicts in
+                    // order
 #pragma unroll
-                    for (int j = 0; j < KWA; ++j)
-                        ft[j] = j < KW ? fk[j] : 0.0;
-                }
-                // each wave judges its own points; the search then walks the verdicts in order
-#pragma unroll
-                for (int j = 0; j < KWA; ++j) {
-                    if (j < kw) {
-                        const bool ok = C.template accepts_at<REF>(ft[j], aj[j], tj[j]);
-                        if (lane == 0)
-                            sOk[wv * kw + j] = ok;
+                    for (int j = 0; j < KWA; ++j) {
+                        if (j < kw) {
+                            const bool ok = C.template accepts_at<REF>(ft[j], aj[j], tj[j]);
+                            if (lane == 0)
+                                sOk[wv * kw + j] = ok;
+                        }
                     }
                 }
                 __syncthreads();
@@ -3036,7 +3065,8 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                     __syncthreads();
                     double fdir;
                     if constexpr (COLS) {
-                        fdir = sweep_cols_block<NM>(P, Mg, wg, Pimg, sA[0], C.shift, sM, sW[0], sP,
+                        fdir = sweep_cols_block<NM>(P, Rimg, Pimg, sA[0], C.shift, sR[0],
+                                                    sR[COLS ? 1 : 0], sP[0], sP[COLS4 ? 1 : 0],
                                                     sXH, sXF, sXG, sRed, sH, sG, lane, wv);
                     } else {
                         ExpA<NM> ax;
@@ -3060,8 +3090,8 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 if constexpr (COLS) {
                     if (mx) {
                         wave_lds_sync();  // sA[0]
-                        fdir = sweep_cols_wave<NM>(P.nqPad, Mg, wg, Pimg, sA[0], C.shift, sM,
-                                                   sW[0], sP, sH, sG, true, lane);
+                        fdir = sweep_cols_wave<NM>(P.nqPad, Rimg, Pimg, sA[0], C.shift, sR[0],
+                                                   sR[COLS ? 1 : 0], sP[0], sH, sG, true, lane);
                         colSwept = true;
                     }
                 }
@@ -3510,11 +3540,13 @@ __global__ __launch_bounds__(kWave) void k_hessian_cols(int64_t ncells, int nqPa
                                                         const double *__restrict__ Mg,
                                                         const double *__restrict__ wg,
                                                         const _Float16 *__restrict__ Pimg,
+                                                        const double *__restrict__ Rimg,
                                                         const double *__restrict__ alpha_g,
                                                         double *hs)
 {
     constexpr int NMS = NM | 1;
-    __shared__ double sM[kChunk * NMS];
+    __shared__ __align__(16) double sR[2][RowImg<NM>::kDoubles];
+    double *const sM = sR[0];
     __shared__ double sW[kChunk];
     __shared__ __align__(16) _Float16 sP[PairHess<NM>::kHalves];
     __shared__ double sA[NM];
@@ -3551,7 +3583,8 @@ __global__ __launch_bounds__(kWave) void k_hessian_cols(int64_t ncells, int nqPa
     }
     const double shift = xmax + log(wtop);
     wave_lds_sync();
-    (void)sweep_cols_wave<NM>(nqPad, Mg, wg, Pimg, sA, shift, sM, sW, sP, sH, sG, false, lane);
+    (void)sweep_cols_wave<NM>(nqPad, Rimg, Pimg, sA, shift, sR[0], sR[1], sP, sH, sG, false,
+                              lane);
     wave_lds_sync();
     double *H = hs + cell * NM * NM;
     for (int p = lane; p < PairHess<NM>::NP; p += kWave) {
@@ -3861,8 +3894,7 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
         if (mixed) {
             if constexpr (NM <= kPairMaxNm)
                 hipLaunchKernelGGL(k_sweep_cols<NM>, dim3(grid_cap(ctx, 1)), dim3(kCBlock), 0, s,
-                                   P, ctx->d_M, ctx->d_wpad, (const _Float16 *)ctx->d_P16, S,
-                                   Lc.Wm, Lc.nm);
+                                   P, ctx->d_R, (const _Float16 *)ctx->d_P16, S, Lc.Wm, Lc.nm);
             else
                 hipLaunchKernelGGL((k_sweep_mixed<NM, 2>), dim3(grid_cap(ctx, 8)), dim3(kBlock),
                                    0, s, P, ctx->d_M, ctx->d_wpad, S, Lc.Wm, Lc.nm);
@@ -3898,8 +3930,8 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
             (void)hipEventRecord(ev[0], s);
 #define C3D_TAIL(R, X, TW)                                                                   \
     hipLaunchKernelGGL((k_tail<NM, R, X, TW>), dim3(grid), dim3(TW * kWave), 0, s, P, ctx->d_M,  \
-                       ctx->d_MT, ctx->d_wpad, (const _Float16 *)ctx->d_P16, S, Lc, Ln, u,     \
-                       alpha, stats, ctx->d_status, K, cntTail)
+                       ctx->d_MT, ctx->d_wpad, (const _Float16 *)ctx->d_P16, ctx->d_R, S, Lc, \
+                       Ln, u, alpha, stats, ctx->d_status, K, cntTail)
         if (ref && wide)
             C3D_TAIL(true, false, 4);
         else if (ref)
@@ -4060,7 +4092,7 @@ hipError_t launch_hessian_fp64(c3d_ctx_s *ctx, int64_t ncells, const double *alp
         return hipSuccess;
     dim3 grid((unsigned)((ncells + kWavesPerBlock - 1) / kWavesPerBlock));
     if (ctx->precision == C3D_PREC_MIXED) {
-#define C3D_HC(NMV) hipLaunchKernelGGL(k_hessian_cols<NMV>, dim3((unsigned)ncells), dim3(kWave), 0, s, ncells, ctx->nqPad, ctx->d_M, ctx->d_wpad, (const _Float16 *)ctx->d_P16, alpha, hs)
+#define C3D_HC(NMV) hipLaunchKernelGGL(k_hessian_cols<NMV>, dim3((unsigned)ncells), dim3(kWave), 0, s, ncells, ctx->nqPad, ctx->d_M, ctx->d_wpad, (const _Float16 *)ctx->d_P16, ctx->d_R, alpha, hs)
         switch (ctx->t.nm) {
         case 4: C3D_HC(4); break;
         case 9: C3D_HC(9); break;

@@ -11,7 +11,7 @@ runs in the benchmark) under each candidate of a small grid of:
   tail_passes  rounds per tail launch (c3d_set_tail)
   tail_wide    cells left when the tail evaluates four line-search trials per round
                (c3d_set_tail_wide)
-  mixed_waves  waves per SIMD of the mixed sweep (c3d_set_launch, mixed only)
+  mixed_waves  waves per SIMD of the mixed sweep (c3d_set_launch, mixed with nm > 16)
 
 and writes the fastest to a JSON table keyed "N,Q,precision", which Ctx.apply_tuning() (and
 through it bench.py) reads.  Usage (on the GPU):
@@ -45,10 +45,10 @@ def front_end():
     return mod
 
 
-def candidates(precision):
-    grid = {"tail_cells": (4096, 8192, 32768), "tail_passes": (32, 128),
-            "tail_wide": (512, 2048, 8192)}
-    if precision == "mixed":
+def candidates(precision, nm):
+    grid = {"tail_cells": (1024, 2048, 4096, 8192, 32768), "tail_passes": (32, 128),
+            "tail_wide": (256, 512, 2048, 8192)}
+    if precision == "mixed" and nm > 16:  # the column sweep (nm <= 16) has one launch shape
         grid["mixed_waves"] = (2, 3)
     keys = list(grid)
     for vals in itertools.product(*(grid[k] for k in keys)):
@@ -96,7 +96,7 @@ def tune_shape(P, torch, dev, N, Q, precision, cells, seed, reps, log):
         return best, rc
 
     results = []
-    for c in candidates(precision):
+    for c in candidates(precision, nm):
         t, rc = run(c)
         results.append((t, c))
         log(f"  N={N} Q={Q} {precision} {c}: {1e3 * t:.2f} ms (status {rc})")

@@ -61,8 +61,9 @@ def test_deck_reader_rules(tmp_path):
 
 def test_tuning_table_lookup_and_candidates(tmp_path):
     """The tuner's candidate grid keeps the four-wave threshold at or below the tail switch
-    and covers the mixed sweep's wave counts; the front end reads entries by (N, Q,
-    precision) and reports absent ones as None."""
+    and covers the wave counts of the mixed sweep of order 4 (the column sweep of orders 1-3
+    has one launch shape); the front end reads entries by (N, Q, precision) and reports
+    absent ones as None."""
     import importlib.util
     import json
     import os
@@ -71,11 +72,12 @@ def test_tuning_table_lookup_and_candidates(tmp_path):
         "c3d_tune", os.path.join(ROOT, "c3d-2015_amd", "tune.py"))
     tune = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(tune)
-    mixed = list(tune.candidates("mixed"))
-    fp64 = list(tune.candidates("fp64"))
-    assert all(c["tail_wide"] <= c["tail_cells"] for c in mixed + fp64)
+    mixed = list(tune.candidates("mixed", 25))
+    cols = list(tune.candidates("mixed", 16))
+    fp64 = list(tune.candidates("fp64", 16))
+    assert all(c["tail_wide"] <= c["tail_cells"] for c in mixed + cols + fp64)
     assert {c["mixed_waves"] for c in mixed} == {2, 3}
-    assert all("mixed_waves" not in c for c in fp64)
+    assert all("mixed_waves" not in c for c in cols + fp64)
     P = product()
     path = tmp_path / "t.json"
     path.write_text(json.dumps({"3,24,mixed": {"tail_cells": 8192, "tail_passes": 32,
