@@ -1570,14 +1570,14 @@ __device__ __forceinline__ void col_exponents(const ExpB<NM> &aT, AF am, d4 (&x)
             x[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(am(b, s), aT.a[s], x[b], 0, 0, 0);
 }
 
-// The weight of one ordinate from its exponent: w_q e^{x - shift} (FP64), the running block sum
-// fb, the running maximum, and the FP16 halves of the weight scaled by 2^12 (the FP32
-// difference v - hi is exact).
-__device__ __forceinline__ double col_weight(double x, double wq, double shiftOwn, double &fb,
+// The weight of one ordinate from its exponent: w_q e^{x - shift} (FP64), the running sum fo
+// it belongs to, the running maximum, and the FP16 halves of the weight scaled by 2^12 (the
+// FP32 difference v - hi is exact).
+__device__ __forceinline__ double col_weight(double x, double wq, double shiftOwn, double &fo,
                                              float &wmaxOwn, _Float16 &hi, _Float16 &lo)
 {
     const double we = wq * exp(x - shiftOwn);
-    fb = fb + we;
+    fo = fo + we;
     const float wf = (float)we;
     wmaxOwn = fmaxf(wmaxOwn, wf);
     const float v = wf * kWScale;
@@ -1631,13 +1631,14 @@ __device__ __forceinline__ void pair_hess(const _Float16 *sP, const h8 (&ah)[2],
 }
 
 // One chunk of the column sweep for the 16 cells of a wave (cell lane & 15): exponents,
-// weights (block sums fb, maximum wmaxOwn), gradient products (accumulator per block) and pair
-// products; sMc the chunk's FP64 rows (leading dimension NM | 1), sWc its weights, sPc its pair
-// image, complete once image_ready() returns (called before the pair products).
+// weights (running sums fo[r] of the ordinates o = (lane >> 4) + 4 r of every block, over
+// blocks and chunks in order; maximum wmaxOwn), gradient products (accumulator per block) and
+// pair products; sMc the chunk's FP64 rows (leading dimension NM | 1), sWc its weights, sPc its
+// pair image, complete once image_ready() returns (called before the pair products).
 template <int NM, typename RF>
 __device__ __forceinline__ void chunk_cols(const double *sMc, const double *sWc,
                                            const _Float16 *sPc, const ExpB<NM> &aT,
-                                           double shiftOwn, float &wmaxOwn, double (&fb)[4],
+                                           double shiftOwn, float &wmaxOwn, double (&fo)[4],
                                            d4 (&gacc)[(NM + 15) / 16][4],
                                            f4 (&acc)[PairHess<NM>::PT], int lane, RF image_ready)
 {
@@ -1654,7 +1655,7 @@ __device__ __forceinline__ void chunk_cols(const double *sMc, const double *sWc,
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             _Float16 hi, lo;
-            x[b][r] = col_weight(x[b][r], sWc[16 * b + g + 4 * r], shiftOwn, fb[b], wmaxOwn, hi,
+            x[b][r] = col_weight(x[b][r], sWc[16 * b + g + 4 * r], shiftOwn, fo[r], wmaxOwn, hi,
                                  lo);
             ah[b >> 1][4 * (b & 1) + r] = hi;
             al[b >> 1][4 * (b & 1) + r] = lo;
@@ -1666,13 +1667,22 @@ __device__ __forceinline__ void chunk_cols(const double *sMc, const double *sWc,
     pair_hess<NM, PairHess<NM>::PT>(sPc, ah, al, acc, 0, 1, lane);
 }
 
-// f of the lane's cell from its block sums: the blocks, then the four row groups (l >> 4) of
-// its column, in a fixed order (every lane of the column gets the same bits).
-__device__ __forceinline__ double col_f(const double (&fb)[4])
+// f of the lane's cell from the running sums of its 16 ordinate slots o = (lane >> 4) + 4 r:
+// the butterfly of row16_sum over o (partners o ^ 8, o ^ 4, o ^ 2, o ^ 1: registers r ^ 2,
+// r ^ 1, then lanes ^ 32, ^ 16), so that a lone cell whose slot o sits in lane o of a 16-lane
+// row, summed by row16_sum, gets the same bits; every lane of the column gets them.
+__device__ __forceinline__ double col_f(const double (&fo)[4])
 {
-    double f = (fb[0] + fb[1]) + (fb[2] + fb[3]);
-    f = f + __shfl_xor(f, 16);
-    return f + __shfl_xor(f, 32);
+    double s1[4], s2[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        s1[r] = fo[r] + fo[r ^ 2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        s2[r] = s1[r] + s1[r ^ 1];
+    double f = s2[0];
+    f = f + __shfl_xor(f, 32);
+    return f + __shfl_xor(f, 16);
 }
 __device__ __forceinline__ float col_max(float v)
 {
@@ -1718,86 +1728,180 @@ struct RowImg {
     static constexpr int kW = kChunk * NMS;  // offset of the weights
 };
 
-// One cell's column sweep on one wave, the cell in column 0 (the other columns compute nothing
-// that is kept), its chunks staged by LDS DMA from the row image (double-buffered, sR0/sR1)
-// and the pair image (single-buffered, sP: a chunk's copy is issued once the previous chunk's
-// pair products have read sP and is waited for before its own).  The waits count the wave's
-// copies: oldest first, the outstanding ones are a chunk's rows, its pair image, the next
-// chunk's rows.  Results as the bulk sweep leaves them for column 0: H (packed lower triangle)
-// in sH, its first entry NaN where the weights left the FP16 range if mark, the gradient
-// moments in sG; returns f.  sA: the cell's alpha.
-template <int NM>
-__device__ __forceinline__ double sweep_cols_wave(int nqPad, const double *__restrict__ Rimg,
-                                                  const _Float16 *__restrict__ Pimg,
-                                                  const double *sA, double shift, double *sR0,
-                                                  double *sR1, _Float16 *sP, double *sH,
-                                                  double *sG, bool mark, int lane)
+// One cell's mixed sweep for nm <= kPairMaxNm on NWV = 1 or 4 waves, with the bulk column
+// sweep's bits for that cell.  FP64 matrix products round as sequential fused multiply-adds
+// over K in order (both V_MFMA_F64_16X16X4_F64 and V_MFMA_F64_4X4X4_4B_F64), so the cell's
+// exponents come from the cheaper four-row products (the cell in row 0: lanes 0..15 hold its
+// ordinates 16 b + lane) and its gradient from vector FMA chains in the bulk's order (lane
+// 16 b + j: moment j of block b's accumulator, K order r then k); the Hessian's pair products
+// run on the FP16 matrix cores with the cell in row 0 (A rows 1..15 zero).  With four waves,
+// wave w computes the exponents of block w and the pair products of tiles w, w + 4, ..;
+// wave 1 the gradient.  f sums the weights per ordinate slot o = lane (blocks and chunks in
+// order) and reduces by row16_sum, the bulk's order.  Staging by LDS DMA: one wave
+// single-buffers (the next chunk's rows copied after the gradient has read the rows, its pair
+// image after the pair products; the waits count the wave's copies), four waves double-buffer
+// (sR1, sP1).  sWE: 64 doubles; sRed: 4 floats.  Results: H (packed lower triangle, first entry
+// NaN where the weights left the FP16 range, if mark) in sH, the gradient moments in sG; f
+// returned (wave 0).
+template <int NM, int NWV>
+__device__ __forceinline__ double sweep_lone(int nqPad, const double *__restrict__ Rimg,
+                                             const _Float16 *__restrict__ Pimg, const double *sA,
+                                             double shift, double *sR0, double *sR1,
+                                             _Float16 *sP0, _Float16 *sP1, double *sWE,
+                                             float *sRed, double *sH, double *sG, bool mark,
+                                             int lane, int wv)
 {
     using PH = PairHess<NM>;
     using RI = RowImg<NM>;
-    constexpr int NT = (NM + 15) / 16;
-    constexpr int PT = PH::PT;
-    const bool own = (lane & 15) == 0;
-    ExpB<NM> aT;
-    aT.load(own ? sA : nullptr, lane);
-    const double shiftOwn = own ? shift : 0.0;
+    constexpr int NMS = NM | 1;
+    constexpr int NA = (PH::PT + NWV - 1) / NWV;
+    static_assert(NWV == 1 || NWV == 4, "one wave or four");
+    const int o = lane & 15;
+    ExpA<NM> ax;
+    ax.load(sA, 1, lane);
+    const double shiftOwn = (lane >> 4) == 0 ? shift : 0.0;
     float wmaxOwn = 0.0f;
-    double fb[4] = {0.0, 0.0, 0.0, 0.0};
-    d4 gacc[NT][4];
+    double fo = 0.0, gl = 0.0;
+    f4 acc[NA];
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-            gacc[t][b] = d4{0.0, 0.0, 0.0, 0.0};
-    f4 acc[PT];
-#pragma unroll
-    for (int t = 0; t < PT; ++t)
-        acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    const int nch = nqPad / kChunk;
-    wave_lds_sync();
-    lds_reads_done();  // earlier readers of sR0, sR1 and sP are done
-    dma_units(Rimg, sR0, RI::kUnits, 0, 1, lane);
-    dma_units(Pimg, sP, PH::kUnits, 0, 1, lane);
-    for (int ch = 0; ch < nch; ++ch) {
-        double *cur = (ch & 1) ? sR1 : sR0;
-        double *nxt = (ch & 1) ? sR0 : sR1;
-        dma_wait_n<PH::kUnits>();  // this chunk's rows (its pair image may still be in flight)
-        wave_lds_sync();
-        const bool more = ch + 1 < nch;
-        if (more)  // nxt was last read by the previous chunk, whose reads are done
-            dma_units(Rimg + (size_t)(ch + 1) * RI::kDoubles, nxt, RI::kUnits, 0, 1, lane);
-        chunk_cols<NM>(cur, cur + RI::kW, sP, aT, shiftOwn, wmaxOwn, fb, gacc, acc, lane, [&] {
-            if (more)
-                dma_wait_n<RI::kUnits>();  // the pair image (the next rows may be in flight)
-            else
-                dma_wait();
+    for (int i = 0; i < NA; ++i)
+        acc[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    auto sync = [] {
+        if constexpr (NWV == 1)
             wave_lds_sync();
-        });
-        if (more) {
-            lds_reads_done();  // the chunk's reads of cur and sP are done
-            dma_units(Pimg + (size_t)(ch + 1) * PH::kHalves, sP, PH::kUnits, 0, 1, lane);
+        else
+            __syncthreads();
+    };
+    // one chunk from its rows sM (weights after them), its pair image sP (complete once
+    // image_ready returns)
+    auto chunk = [&](const double *sM, const _Float16 *sP, auto image_ready) {
+        const double *sW = sM + RI::kW;
+        if constexpr (NWV == 1) {
+            double x[4];
+            exp_arg4<NM>(ax, [&](int b, int s) {
+                const int m = 4 * s + (lane >> 4);
+                return (NM % 4 == 0 || m < NM) ? sM[(16 * b + o) * NMS + m] : 0.0;
+            }, x);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const double we = sW[16 * b + o] * exp(x[b] - shiftOwn);
+                wmaxOwn = fmaxf(wmaxOwn, (float)we);
+                if (lane < 16)
+                    sWE[16 * b + o] = we;
+            }
+        } else {
+            const int b = wv;
+            double x = 0.0;
+#pragma unroll
+            for (int s = 0; s < ExpA<NM>::NKS; ++s) {
+                const int m = 4 * s + (lane >> 4);
+                const double bm = (NM % 4 == 0 || m < NM) ? sM[(16 * b + o) * NMS + m] : 0.0;
+                x = __builtin_amdgcn_mfma_f64_4x4x4f64(ax.a[s], bm, x, 0, 0, 0);
+            }
+            const double we = sW[16 * b + o] * exp(x - shiftOwn);
+            wmaxOwn = fmaxf(wmaxOwn, (float)we);
+            if (lane < 16)
+                sWE[16 * b + o] = we;
+        }
+        sync();
+        if (wv == 0 && lane < 16) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                fo = fo + sWE[16 * b + o];
+        }
+        if (wv == (NWV == 1 ? 0 : 1) && o < NM) {
+            const int bg = lane >> 4;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int q = 16 * bg + k + 4 * r;
+                    gl = fma(sWE[q], sM[q * NMS + o], gl);
+                }
+        }
+        h8 ah[2], al[2];
+        const bool arow = o == 0;
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int q = 32 * kb + 16 * (e >> 2) + (lane >> 4) + 4 * (e & 3);
+                const float v = arow ? (float)sWE[q] * kWScale : 0.0f;
+                const _Float16 hi = (_Float16)v;
+                ah[kb][e] = hi;
+                al[kb][e] = (_Float16)(v - (float)hi);
+            }
+        image_ready();
+        pair_hess<NM, NA>(sP, ah, al, acc, wv, NWV, lane);
+    };
+    const int nch = nqPad / kChunk;
+    if constexpr (NWV == 1) {
+        wave_lds_sync();
+        lds_reads_done();  // earlier readers of sR0 and sP0 are done
+        dma_units(Rimg, sR0, RI::kUnits, 0, 1, lane);
+        dma_units(Pimg, sP0, PH::kUnits, 0, 1, lane);
+        for (int ch = 0; ch < nch; ++ch) {
+            const bool more = ch + 1 < nch;
+            dma_wait_n<PH::kUnits>();  // the chunk's rows (its pair image may be in flight)
+            wave_lds_sync();
+            chunk(sR0, sP0, [&] {
+                lds_reads_done();  // the rows are read
+                if (more) {
+                    dma_units(Rimg + (size_t)(ch + 1) * RI::kDoubles, sR0, RI::kUnits, 0, 1,
+                              lane);
+                    dma_wait_n<RI::kUnits>();  // the pair image (the next rows may be in flight)
+                } else {
+                    dma_wait();
+                }
+                wave_lds_sync();
+            });
+            if (more) {
+                lds_reads_done();  // the pair products have read sP0
+                dma_units(Pimg + (size_t)(ch + 1) * PH::kHalves, sP0, PH::kUnits, 0, 1, lane);
+            }
+        }
+    } else {
+        __syncthreads();  // earlier readers of the buffers are done
+        dma_units(Rimg, sR0, RI::kUnits, wv, 4, lane);
+        dma_units(Pimg, sP0, PH::kUnits, wv, 4, lane);
+        for (int ch = 0; ch < nch; ++ch) {
+            dma_wait();
+            __syncthreads();  // the chunk's images; the previous chunk's readers are done
+            if (ch + 1 < nch) {
+                dma_units(Rimg + (size_t)(ch + 1) * RI::kDoubles, (ch & 1) ? sR0 : sR1,
+                          RI::kUnits, wv, 4, lane);
+                dma_units(Pimg + (size_t)(ch + 1) * PH::kHalves, (ch & 1) ? sP0 : sP1,
+                          PH::kUnits, wv, 4, lane);
+            }
+            chunk((ch & 1) ? sR1 : sR0, (ch & 1) ? sP1 : sP0, [] {});
         }
     }
-    const double f = col_f(fb);
-    const float wm = col_max(wmaxOwn);
+    const float wRow = row16_max(wmaxOwn);
+    float wm = __shfl(wRow, 0);
+    if constexpr (NWV == 4) {
+        if (lane == 0)
+            sRed[wv] = wm;
+        __syncthreads();
+        wm = fmaxf(fmaxf(sRed[0], sRed[1]), fmaxf(sRed[2], sRed[3]));
+    }
     const double es = exp(shift);
     const double sc = pair_scale(es);
-    const bool ok = !mark || h16_range_ok(__shfl(wm, 0));
+    const bool ok = !mark || h16_range_ok(wm);
     if (lane < 16) {
 #pragma unroll
-        for (int t = 0; t < PT; ++t) {
-            const int p = 16 * t + lane;
-            if (p < PH::NP)
-                sH[p] = (p == 0 && !ok) ? __builtin_nan("") : (double)acc[t][0] * sc;
-        }
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = 16 * t + lane;
-            if (col < NM)
-                sG[col] = col_g(gacc[t], 0) * es;
+        for (int i = 0; i < NA; ++i) {
+            const int t = wv + NWV * i, p = 16 * t + lane;
+            if (t < PH::PT && p < PH::NP)
+                sH[p] = (p == 0 && !ok) ? __builtin_nan("") : (double)acc[i][0] * sc;
         }
     }
-    return __shfl(f, 0) * es;
+    if (wv == (NWV == 1 ? 0 : 1)) {
+        const double g0 = __shfl(gl, o), g1 = __shfl(gl, 16 + o);
+        const double g2 = __shfl(gl, 32 + o), g3 = __shfl(gl, 48 + o);
+        if (lane < NM)
+            sG[lane] = ((g0 + g1) + (g2 + g3)) * es;
+    }
+    return wv == 0 ? __shfl(row16_sum(fo), 0) * es : 0.0;
 }
 
 constexpr int kCWaves = 8;  // waves per block of the column sweep
@@ -2616,194 +2720,6 @@ __device__ __forceinline__ double sweep_mixed_block(const KParams &P, const doub
     return fdir;
 }
 
-// One cell's column sweep on the four waves of a tail block, with the bulk's bits for column 0:
-// wave w takes ordinate block w of every chunk (its exponent chain, weights, running block sum
-// and gradient accumulator w), the waves exchange the FP16 halves through sXH for the pair
-// products (wave w: tiles w, w + 4, ..), and f, the largest weight and the gradient are
-// combined in the bulk's order at the end (sXF, sXG, sRed).  The row and pair images are staged
-// by LDS DMA, double-buffered (sR0/sR1, sP0/sP1).  Results as sweep_cols_wave's.
-template <int NM>
-__device__ __forceinline__ double sweep_cols_block(const KParams &P, const double *__restrict__ Rimg,
-                                                   const _Float16 *__restrict__ Pimg,
-                                                   const double *sA, double shift, double *sR0,
-                                                   double *sR1, _Float16 *sP0, _Float16 *sP1,
-                                                   _Float16 *sXH, double *sXF, double *sXG,
-                                                   float *sRed, double *sH, double *sG, int lane,
-                                                   int wv)
-{
-    using PH = PairHess<NM>;
-    using RI = RowImg<NM>;
-    constexpr int NT = (NM + 15) / 16;
-    constexpr int NA = (PH::PT + 3) / 4;
-    constexpr int NMS = NM | 1;
-    const int i16 = lane & 15, g = lane >> 4;
-    const bool own = i16 == 0;
-    ExpB<NM> aT;
-    aT.load(own ? sA : nullptr, lane);
-    const double shiftOwn = own ? shift : 0.0;
-    float wmaxOwn = 0.0f;
-    double fbw = 0.0;
-    d4 ga[NT][4];  // only [t][wv] is used: col_grad_block's accumulator of block wv
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-            ga[t][b] = d4{0.0, 0.0, 0.0, 0.0};
-    f4 acc[NA];
-#pragma unroll
-    for (int i = 0; i < NA; ++i)
-        acc[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    const int nch = P.nqPad / kChunk;
-    __syncthreads();  // earlier readers of the buffers are done
-    dma_units(Rimg, sR0, RI::kUnits, wv, 4, lane);
-    dma_units(Pimg, sP0, PH::kUnits, wv, 4, lane);
-    for (int ch = 0; ch < nch; ++ch) {
-        const double *sM = (ch & 1) ? sR1 : sR0;
-        const _Float16 *sP = (ch & 1) ? sP1 : sP0;
-        dma_wait();
-        __syncthreads();  // the chunk's images; the previous chunk's readers are done
-        if (ch + 1 < nch) {
-            dma_units(Rimg + (size_t)(ch + 1) * RI::kDoubles, (ch & 1) ? sR0 : sR1, RI::kUnits,
-                      wv, 4, lane);
-            dma_units(Pimg + (size_t)(ch + 1) * PH::kHalves, (ch & 1) ? sP0 : sP1, PH::kUnits,
-                      wv, 4, lane);
-        }
-        const int b = wv;
-        d4 x = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int s = 0; s < ExpB<NM>::NKS; ++s) {
-            const int m = 4 * s + g;
-            const double am = (NM % 4 == 0 || m < NM) ? sM[(16 * b + i16) * NMS + m] : 0.0;
-            x = __builtin_amdgcn_mfma_f64_16x16x4f64(am, aT.a[s], x, 0, 0, 0);
-        }
-        _Float16 hv[2][4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-            x[r] = col_weight(x[r], sM[RI::kW + 16 * b + g + 4 * r], shiftOwn, fbw, wmaxOwn,
-                              hv[0][r], hv[1][r]);
-#pragma unroll
-        for (int pl = 0; pl < 2; ++pl)
-            *reinterpret_cast<h4 *>(&sXH[((wv * 2 + pl) * kWave + lane) * 4]) =
-                h4{hv[pl][0], hv[pl][1], hv[pl][2], hv[pl][3]};
-        col_grad_block<NM>(sM, x, wv, ga, lane);
-        __syncthreads();  // the halves
-        h8 ah[2], al[2];
-#pragma unroll
-        for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int src = 2 * kb + (e >> 2);
-                ah[kb][e] = sXH[((src * 2 + 0) * kWave + lane) * 4 + (e & 3)];
-                al[kb][e] = sXH[((src * 2 + 1) * kWave + lane) * 4 + (e & 3)];
-            }
-        pair_hess<NM, NA>(sP, ah, al, acc, wv, 4, lane);
-    }
-    // the bulk's combination order: block sums per lane, then the column's row groups
-    sXF[wv * kWave + lane] = fbw;
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-            sXG[((wv * NT + t) * kWave + lane) * 4 + r] = ga[t][wv][r];
-    const float wm = col_max(wmaxOwn);
-    if (lane == 0)
-        sRed[wv] = wm;
-    __syncthreads();
-    const double es = exp(shift);
-    const bool ok = h16_range_ok(fmaxf(fmaxf(sRed[0], sRed[1]), fmaxf(sRed[2], sRed[3])));
-    const double sc = pair_scale(es);
-    if (lane < 16) {
-#pragma unroll
-        for (int i = 0; i < NA; ++i) {
-            const int t = wv + 4 * i, p = 16 * t + lane;
-            if (t < PH::PT && p < PH::NP)
-                sH[p] = (p == 0 && !ok) ? __builtin_nan("") : (double)acc[i][0] * sc;
-        }
-    }
-    double fdir = 0.0;
-    if (wv == 0) {
-        const double fb[4] = {sXF[lane], sXF[kWave + lane], sXF[2 * kWave + lane],
-                              sXF[3 * kWave + lane]};
-        fdir = __shfl(col_f(fb), 0) * es;
-    } else if (wv == 1 && lane < 16) {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = 16 * t + lane;
-            if (col < NM) {
-                d4 gb[4];
-#pragma unroll
-                for (int bb = 0; bb < 4; ++bb)
-                    gb[bb][0] = sXG[((bb * NT + t) * kWave + lane) * 4];
-                sG[col] = col_g(gb, 0) * es;
-            }
-        }
-    }
-    return fdir;
-}
-
-// f at up to 16 trial points in the mixed phase for nm <= kPairMaxNm, as the column sweep
-// computes it: point j (alpha at sA + j * NM, j < kw) in column j, each lane's weights summed
-// per ordinate block in chunk and row order, then col_f.  M is read from its transposed copy
-// MT[m][q] (leading dimension nqPad) half a chunk (two blocks) at a time, the next half's
-// operands in flight.  Returns in lane l the f of point l & 15.
-template <int NM>
-__device__ __forceinline__ double f_trials_cols(const KParams &P, const double *__restrict__ MT,
-                                                const double *__restrict__ wg, const double *sA,
-                                                int kw, double shift, int lane)
-{
-    constexpr int NKS = ExpB<NM>::NKS;
-    const int nh = 2 * (P.nqPad / kChunk);
-    const size_t ld = (size_t)P.nqPad;
-    const int i16 = lane & 15, g = lane >> 4;
-    ExpB<NM> aT;
-    aT.load(i16 < kw ? sA + i16 * NM : nullptr, lane);
-    // half hc: blocks 2 (hc & 1) and 2 (hc & 1) + 1 of chunk hc >> 1
-    auto load = [&](double(&am)[2][NKS], double(&w)[2][4], int hc) {
-#pragma unroll
-        for (int bb = 0; bb < 2; ++bb) {
-            const size_t q0 = (size_t)(hc >> 1) * kChunk + 16 * (2 * (hc & 1) + bb);
-#pragma unroll
-            for (int s = 0; s < NKS; ++s) {
-                const int m = 4 * s + g;
-                am[bb][s] = (NM % 4 == 0 || m < NM) ? MT[m * ld + q0 + i16] : 0.0;
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                w[bb][r] = wg[q0 + g + 4 * r];
-        }
-    };
-    double fb[4] = {0.0, 0.0, 0.0, 0.0};
-    auto term = [&](const double(&am)[2][NKS], const double(&w)[2][4], int hc) {
-        d4 x[2] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
-#pragma unroll
-        for (int s = 0; s < NKS; ++s)
-#pragma unroll
-            for (int bb = 0; bb < 2; ++bb)
-                x[bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[bb][s], aT.a[s], x[bb], 0, 0, 0);
-        const int b0 = 2 * (hc & 1);
-#pragma unroll
-        for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const double we = w[bb][r] * exp(x[bb][r] - shift);
-                if (b0 == 0)
-                    fb[bb] = fb[bb] + we;
-                else
-                    fb[2 + bb] = fb[2 + bb] + we;
-            }
-    };
-    double amA[2][NKS], wA[2][4], amB[2][NKS], wB[2][4];
-    load(amA, wA, 0);
-    for (int hc = 0; hc < nh; hc += 2) {
-        load(amB, wB, hc + 1);
-        term(amA, wA, hc);
-        if (hc + 2 < nh)
-            load(amA, wA, hc + 2);
-        term(amB, wB, hc + 1);
-    }
-    return col_f(fb) * exp(shift);
-}
-
 // f at up to four trial points in the mixed phase, as chunk_h16 evaluates it: the points are
 // the rows of the exponent products (alpha of point j at sA + j * NM, j < kw), lane L takes
 // point L >> 4 at ordinates 16 g + (L & 15) of each chunk and sums its weights in chunk and
@@ -2916,17 +2832,14 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     // evaluation's (f costs nq (NM + exp) per point)
     constexpr int KW = REF ? 1 : (NM <= 9 ? 8 : (NM <= 16 ? 2 : 1));
     // in the mixed phase the points are the rows of the exponent products: four per wave
-    // (the column sweep's trials: the 16 columns of its products, as cheap as four)
-    constexpr int KWM = NM <= kPairMaxNm ? 16 : 2;
-    constexpr int KWA = MIXED && NM > kPairMaxNm && KWM > KW ? KWM : KW;  // register arrays
-    constexpr int KWS = MIXED && KWM > KW ? KWM : KW;                     // LDS slots
+    constexpr int KWM = NM <= 16 ? 4 : 2;
+    constexpr int KWA = MIXED && KWM > KW ? KWM : KW;
     static_assert(!(REF && MIXED), "one arithmetic mode");
-    // the column sweep's images (nm <= kPairMaxNm, mixed): rows double-buffered, the pair image
-    // double-buffered with four waves, else single; the first rows buffer also serves as the
-    // other paths' chunk rows (sM)
+    // the column sweep's images (nm <= kPairMaxNm, mixed): double-buffered with four waves,
+    // else single; the first rows buffer also serves as the other paths' chunk rows (sM)
     constexpr bool COLS = MIXED && NM <= kPairMaxNm;
     constexpr bool COLS4 = COLS && TW == 4;
-    __shared__ __align__(16) double sR[COLS ? 2 : 1][COLS ? RowImg<NM>::kDoubles : 1];
+    __shared__ __align__(16) double sR[COLS4 ? 2 : 1][COLS ? RowImg<NM>::kDoubles : 1];
     __shared__ __align__(16) _Float16 sP[COLS4 ? 2 : 1][COLS ? PairHess<NM>::kHalves : 8];
     __shared__ double sMo[COLS ? 1 : kChunk * NMS];
     double *const sM = COLS ? &sR[0][0] : sMo;
@@ -2934,22 +2847,20 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     // the state machine runs on 16-lane rows for NM <= 16 (DPP broadcasts; the four rows of a
     // wave hold identical copies), else on the whole wave
     constexpr int G = NM <= 16 ? 16 : kWave;
-    constexpr int RW = kWave / G;
-    __shared__ double sT[TW][RW][NM * (NM + 1)];
+    // the rows of a wave run identical copies of the state machine and write identical values:
+    // one factor workspace per wave serves them all
+    __shared__ double sT[TW][NM * (NM + 1)];
     __shared__ double sW[TW][kChunk];
     __shared__ double sE[TW][kChunk];
-    __shared__ double sA[TW][KWS * NM];
-    __shared__ int sOk[TW * KWS];       // Armijo verdict at each trial point
+    __shared__ double sA[TW][KWA * NM];
+    __shared__ int sOk[TW * KWA];       // Armijo verdict at each trial point
     __shared__ double sF[1];            // f of the sweep
-    // the mixed sweep's images: for the column sweep with four waves the exchanged halves,
-    // block sums and gradient accumulators; else H16Chunk's image, the weights' halves and
+    // the mixed sweep's images for nm > kPairMaxNm: H16Chunk's image, the weights' halves and
     // FP64 weights
     __shared__ __align__(16) _Float16 sBh[MIXED && !COLS ? H16Chunk<NM>::kBHalves : 8];
     __shared__ __align__(16) _Float16 sWH[MIXED && !COLS ? 2 * kChunk : 8];
     __shared__ __align__(16) double sWE64[MIXED && !COLS ? kWE64S : 2];
-    __shared__ __align__(16) _Float16 sXH[COLS4 ? 4 * 2 * kWave * 4 : 8];
-    __shared__ double sXF[COLS4 ? 4 * kWave : 1];
-    __shared__ double sXG[COLS4 ? 4 * NT * kWave * 4 : 1];
+    __shared__ double sWEl[COLS ? kChunk : 1];  // the lone-cell sweep's weights
     __shared__ double sG[NM];
     __shared__ float sRed[4];
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
@@ -2980,31 +2891,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 auto point = [&](int j, double t) {
                     return (wv == 0 && j == 0) ? C.a : (li < NM ? C.aOld + t * C.d : 0.0);
                 };
-                bool judged = false;
-                if constexpr (MIXED && NM <= kPairMaxNm) {
-                    if (mx) {  // the column sweep's trials: no per-point registers
-                        const double t0 = tk;
-#pragma unroll 1
-                        for (int j = 0; j < kw; ++j) {
-                            if (lane < NM)
-                                sA[wv][j * NM + lane] = point(j, tk);
-                            tk = 0.5 * tk;
-                        }
-                        wave_lds_sync();
-                        const double fl = f_trials_cols<NM>(P, MT, wg, sA[wv], kw, C.shift, lane);
-                        tk = t0;
-#pragma unroll 1
-                        for (int j = 0; j < kw; ++j) {
-                            const bool ok =
-                                C.template accepts_at<REF>(__shfl(fl, j), point(j, tk), tk);
-                            if (lane == 0)
-                                sOk[wv * kw + j] = ok;
-                            tk = 0.5 * tk;
-                        }
-                        judged = true;
-                    }
-                }
-                if (!judged) {
+                {
                     double aj[KWA], tj[KWA];
 #pragma unroll
                     for (int j = 0; j < KWA; ++j) {
@@ -3016,7 +2903,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                     }
                     wave_lds_sync();
                     double ft[KWA];
-                    if (mx && NM > kPairMaxNm) {
+                    if (mx) {  // the mixed sweeps' f: the same bits as the sweep's
                         double f4p[4];
                         f_trials_mixed<NM>(P, MT, wg, sA[wv], kw, C.shift, lane, f4p);
 #pragma unroll
@@ -3065,9 +2952,9 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                     __syncthreads();
                     double fdir;
                     if constexpr (COLS) {
-                        fdir = sweep_cols_block<NM>(P, Rimg, Pimg, sA[0], C.shift, sR[0],
-                                                    sR[COLS ? 1 : 0], sP[0], sP[COLS4 ? 1 : 0],
-                                                    sXH, sXF, sXG, sRed, sH, sG, lane, wv);
+                        fdir = sweep_lone<NM, 4>(P.nqPad, Rimg, Pimg, sA[0], C.shift, sR[0],
+                                                 sR[COLS4 ? 1 : 0], sP[0], sP[COLS4 ? 1 : 0], sWEl,
+                                                 sRed, sH, sG, true, lane, wv);
                     } else {
                         ExpA<NM> ax;
                         ax.load(sA[0], 1, lane);
@@ -3090,8 +2977,9 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 if constexpr (COLS) {
                     if (mx) {
                         wave_lds_sync();  // sA[0]
-                        fdir = sweep_cols_wave<NM>(P.nqPad, Rimg, Pimg, sA[0], C.shift, sR[0],
-                                                   sR[COLS ? 1 : 0], sP[0], sH, sG, true, lane);
+                        fdir = sweep_lone<NM, 1>(P.nqPad, Rimg, Pimg, sA[0], C.shift, sR[0],
+                                                 sR[0], sP[0], sP[0], sWEl, sRed, sH, sG, true,
+                                                 lane, 0);
                         colSwept = true;
                     }
                 }
@@ -3195,7 +3083,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 ++sweeps;
             }
             __syncthreads();
-            done = C.template advance<REF>(P, sH, sF[0], MIXED ? sG : nullptr, sT[wv][row], li);
+            done = C.template advance<REF>(P, sH, sF[0], MIXED ? sG : nullptr, sT[wv], li);
             __syncthreads();  // sH and sF are rewritten by the next round
             if (done)
                 break;
@@ -3545,13 +3433,14 @@ __global__ __launch_bounds__(kWave) void k_hessian_cols(int64_t ncells, int nqPa
                                                         double *hs)
 {
     constexpr int NMS = NM | 1;
-    __shared__ __align__(16) double sR[2][RowImg<NM>::kDoubles];
-    double *const sM = sR[0];
+    __shared__ __align__(16) double sR[RowImg<NM>::kDoubles];
+    double *const sM = sR;
     __shared__ double sW[kChunk];
     __shared__ __align__(16) _Float16 sP[PairHess<NM>::kHalves];
     __shared__ double sA[NM];
     __shared__ double sH[PairHess<NM>::NP];
     __shared__ double sG[NM];
+    __shared__ double sWE[kChunk];
     const int lane = threadIdx.x;
     const int64_t cell = blockIdx.x;
     if (cell >= ncells)
@@ -3583,8 +3472,8 @@ __global__ __launch_bounds__(kWave) void k_hessian_cols(int64_t ncells, int nqPa
     }
     const double shift = xmax + log(wtop);
     wave_lds_sync();
-    (void)sweep_cols_wave<NM>(nqPad, Rimg, Pimg, sA, shift, sR[0], sR[1], sP, sH, sG, false,
-                              lane);
+    (void)sweep_lone<NM, 1>(nqPad, Rimg, Pimg, sA, shift, sR, sR, sP, sP, sWE, nullptr, sH, sG,
+                            false, lane, 0);
     wave_lds_sync();
     double *H = hs + cell * NM * NM;
     for (int p = lane; p < PairHess<NM>::NP; p += kWave) {
