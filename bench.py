@@ -9,8 +9,10 @@ generated on the device once and stay in HBM; a step restores u and alpha from p
 device copies and solves the whole batch.
 
 Paths (--path): "fp64" plain FP64 (matrix-core Hessian sums); "mixed" the Hessian as split
-FP16 matrix-core products under a log-sum-exp shift with the gradient, f, Cholesky and the
-stopping test in FP64 (ill-conditioned cells continue on FP64 Hessians); "+nn" adds the
+FP16 matrix-core products under a log-sum-exp shift (for M_1..M_3 the column sweep: cells as
+the columns of the products, Hessian as pair products) with the exponents, gradient, f,
+Cholesky and the stopping test in FP64 (ill-conditioned cells continue on FP64 Hessians);
+"+nn" adds the
 learned warm start, trained first (not timed) on --train-pairs (u, alpha) pairs that the FP64
 closure solves from a different seed.  The plain FP64 path and the reference-order FP64 path
 (bit-identical to the oracle) are timed beside the chosen one (same inputs, same process);
@@ -38,7 +40,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = ("M_N cell-updates/s at ‖∇h‖≤1e-8: batched mixed+NN vs plain FP64, "
           "one GPU")
 FP64_MFMA_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix
-FP32_MFMA_PEAK = 157.3  # TFLOP/s, MI355X dense FP32 matrix
 FP16_MFMA_PEAK = 2500.0  # TFLOP/s, MI355X dense FP16 matrix
 
 
@@ -136,9 +137,31 @@ def log(msg):
 
 
 def _short_kernel(name):
+    """k_name<args> from a demangled or an Itanium-mangled kernel name (rocprofv3 leaves some
+    templates mangled: k_tailILi16ELb0ELb1ELi4EE... -> k_tail<16, false, true, 4>)."""
     import re
+    m = re.search(r"(k_[a-z0-9_]+?)I((?:L[bi]\d+E)+)E", name)
+    if m:
+        args = [("true" if v == "1" else "false") if t == "b" else v
+                for t, v in re.findall(r"L([bi])(\d+)E", m.group(2))]
+        return f"{m.group(1)}<{', '.join(args)}>"
     m = re.search(r"(k_\w+(<[^(]*>)?|__amd_rocclr_\w+)", name)
     return m.group(1) if m else name[:48]
+
+
+def mixed_sweep_work(nm, nq):
+    """Algorithmic matrix work of a mixed-phase sweep per cell: the exponents alpha . m_q and
+    the gradient moments sum_q w e m_q on the FP64 matrix cores (2 nq nm flops each), the
+    Hessian's nm (nm + 1) / 2 distinct entries as three FP16 products (hi hi, hi lo, lo hi;
+    6 nq nm (nm + 1) / 2 flops); in FP64-equivalent flops, the FP16 part weighted by the ratio
+    of the two matrix peaks (time at peak rate on the FP64 matrix cores)."""
+    f64 = 4.0 * nq * nm
+    f16 = 6.0 * nq * nm * (nm + 1) / 2
+    return {"fp64": f64, "fp16": f16, "fp64_equivalent": f64 + f16 * FP64_MFMA_PEAK / FP16_MFMA_PEAK}
+
+
+def mixed_sweep_kernel(nm):
+    return (f"k_sweep_cols<{nm}>" if nm <= 16 else "k_sweep_mixed<25, 2>")
 
 
 def summarize_profile(paths, steps, warmup, write_traffic=None):
@@ -392,35 +415,36 @@ def main():
     agree = {"moments_within_1e-6": round(float((du <= 1e-6).mean()), 6),
              "rung_agreement": round(float((main_r["reg"] == base_r["reg"]).mean()), 6)}
 
-    # roofline of the path's matrix-core sweep kernel.  Algorithmic work: the Hessian
-    # H = M^T diag(w e) M, 2 nq nm^2 flops per cell-sweep.  The mixed sweep computes it to FP32
-    # accuracy (about 22-bit split products, FP32 accumulation) as three FP16 matrix-core
-    # products, with the gradient on the FP64 matrix cores; its line is against the FP32 matrix
-    # peak (the accuracy class delivered), with the executed FP16 products against the FP16
-    # peak beside it.  The FP64 path's sweep runs on the FP64 matrix cores.
-    flops_per_sweep = 2.0 * nq * nm * nm
-    mixed_waves = (tuning.get(args.path) or {}).get("mixed_waves", 3) if args.mixed_waves is None \
-        else args.mixed_waves
+    # roofline of the path's matrix-core sweep kernel.  The FP64 path's sweep: the Hessian
+    # H = M^T diag(w e) M on the FP64 matrix cores, 2 nq nm^2 flops per cell-sweep, against the
+    # FP64 matrix peak.  The mixed sweep (mixed_sweep_work): FP64 exponent and gradient
+    # products plus the split FP16 pair products of the Hessian, in FP64-equivalent flops
+    # against the FP64 matrix peak, i.e. the fraction of the kernel's time its matrix work
+    # would take at the published peaks; the rest is its FP64 vector work (exp) and stalls.
 
     def sweep_roofline(prof, mixed_kernel):
         cs = prof["mixed_cell_sweeps"] if mixed_kernel else prof["cell_sweeps"]
         ms = prof["mixed_sweep_ms"] if mixed_kernel else prof["sweep_ms"]
-        peak = FP32_MFMA_PEAK if mixed_kernel else FP64_MFMA_PEAK
-        ach = flops_per_sweep * cs / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
-        r = {"bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
-             "frac": round(ach / peak, 4), "traffic": None,
-             "kernel": (f"k_sweep_mixed<{nm}, {mixed_waves}> (Hessian as split FP16 matrix-core "
-                        f"products, FP64 matrix-core gradient, mixed phase)" if mixed_kernel
-                        else f"k_sweep<{nm}> (FP64 MFMA Hessian)"),
-             "flops_per_cell_sweep": flops_per_sweep, "cell_sweeps_per_step": int(cs),
+        secs = ms * 1e-3
+        if mixed_kernel:
+            w = mixed_sweep_work(nm, nq)
+            fl = w["fp64_equivalent"]
+            kern = (f"{mixed_sweep_kernel(nm)} (mixed phase: FP64 matrix-core exponents and "
+                    "gradient, Hessian as split FP16 pair products)")
+        else:
+            fl = 2.0 * nq * nm * nm
+            kern = f"k_sweep<{nm}> (FP64 MFMA Hessian)"
+        ach = fl * cs / secs / 1e12 if ms > 0 else 0.0
+        r = {"bound": "mfma", "achieved": round(ach, 3), "peak": FP64_MFMA_PEAK,
+             "unit": "TFLOP/s", "frac": round(ach / FP64_MFMA_PEAK, 4), "traffic": None,
+             "kernel": kern, "flops_per_cell_sweep": fl, "cell_sweeps_per_step": int(cs),
              "kernel_ms_per_step": round(ms, 3)}
         if mixed_kernel:
-            ex = 3.0 * ach
-            r["fp16_matrix_executed"] = {"achieved": round(ex, 3), "peak": FP16_MFMA_PEAK,
-                                         "frac": round(ex / FP16_MFMA_PEAK, 4),
-                                         "note": "three FP16 products per Hessian entry; the "
-                                                 "kernel is bound by its FP64 vector work "
-                                                 "(exponent, exp) and the operand split"}
+            r["basis"] = ("FP64-equivalent flops: FP64 matrix work plus FP16 matrix work x "
+                          f"{FP64_MFMA_PEAK} / {FP16_MFMA_PEAK}")
+            r["work_per_cell_sweep"] = {"fp64_matrix": w["fp64"], "fp16_matrix": w["fp16"]}
+            r["fp64_matrix_achieved"] = round(w["fp64"] * cs / secs / 1e12, 3) if ms > 0 else 0.0
+            r["fp16_matrix_achieved"] = round(w["fp16"] * cs / secs / 1e12, 3) if ms > 0 else 0.0
         return r
 
     # HBM bytes per dispatch of the sweep kernels, from the committed rocprofv3 PMC summary
@@ -898,11 +922,11 @@ def orders_workload(args, P, torch, dev):
             kms = pr["mixed_sweep_ms"] if mixed_k else pr["sweep_ms"]
             nm_, nq_ = p["ctx"].nm, p["ctx"].nq
             per.append({"N": p["N"], "iters_mean": round(float(st["totalIter"].mean()), 3),
-                        "sweep_kernel": (f"k_sweep_mixed<{nm_}, "
-                                         f"{2 if nm_ > 16 else (p['tune'] or {}).get('mixed_waves', 3)}>"
-                                         if mixed_k else f"k_sweep<{nm_}>"),
+                        "sweep_kernel": (mixed_sweep_kernel(nm_) if mixed_k
+                                         else f"k_sweep<{nm_}>"),
                         "sweep_ms": round(kms, 3), "cell_sweeps": int(sweeps),
-                        "flops_per_cell_sweep": 2 * nq_ * nm_ * nm_,
+                        "flops_per_cell_sweep": (mixed_sweep_work(nm_, nq_)["fp64_equivalent"]
+                                                 if mixed_k else 2 * nq_ * nm_ * nm_),
                         "fallback_frac": round(float(((st["flags"] & P.FLAG_FP64_FALLBACK) != 0)
                                                      .mean()), 5),
                         "u": p["u"].cpu().numpy(), "reg": st["regIndex"].copy()})
@@ -914,11 +938,11 @@ def orders_workload(args, P, torch, dev):
     # roofline: the order whose sweep kernel takes the longest, on its matrix-core peak
     dom = max(per_m, key=lambda r: r["sweep_ms"])
     ach = dom["flops_per_cell_sweep"] * dom["cell_sweeps"] / (dom["sweep_ms"] / 1e3) / 1e12
-    roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": FP32_MFMA_PEAK,
-            "unit": "TFLOP/s", "frac": round(ach / FP32_MFMA_PEAK, 4), "traffic": None,
-            "kernel": dom["sweep_kernel"] + f" (M_{dom['N']} quarter; split FP16 Hessian, FP64 "
-                      "gradient; peak: the FP32 matrix rate, the precision class of the "
-                      "Hessian; HIP-event spans overlap other orders' kernels)",
+    roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": FP64_MFMA_PEAK,
+            "unit": "TFLOP/s", "frac": round(ach / FP64_MFMA_PEAK, 4), "traffic": None,
+            "kernel": dom["sweep_kernel"] + f" (M_{dom['N']} quarter; FP64 exponents and "
+                      "gradient, split FP16 Hessian, in FP64-equivalent flops as the closure "
+                      "line counts them; HIP-event spans overlap other orders' kernels)",
             "per_order": [{k: r[k] for k in ("N", "sweep_kernel", "sweep_ms", "cell_sweeps",
                                                 "flops_per_cell_sweep")} for r in per_m]}
     # CPU baseline: the oracle on the first cells of each quarter

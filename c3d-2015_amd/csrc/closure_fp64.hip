@@ -1228,6 +1228,50 @@ __device__ __forceinline__ int g4_b_col(int lane) { return lane & 15; }
 __device__ __forceinline__ int g4_d_row(int lane) { return lane >> 4; }
 __device__ __forceinline__ int g4_d_col(int lane) { return lane & 15; }
 
+// e^x for the weights of the mixed sweeps (every mixed path computes them with it, so that they
+// agree bit for bit): x = k ln2 / 64 + r by Cody-Waite reduction with fused multiply-adds
+// (|r| <= ln2 / 128), e^x = 2^(k >> 6) 2^((k & 63) / 64) e^r with the middle factor from a
+// 64-entry table (correctly rounded, 512 bytes that stay in the vector caches) and e^r by its
+// degree-5 Taylor polynomial (truncation below 4e-17): relative error within 2 ulp for
+// |x| < 700, NaN propagates.
+__constant__ double kExp2Tab[64] = {
+    0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0,
+    0x1.0874518759bc8p+0, 0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0,
+    0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0, 0x1.172b83c7d517bp+0,
+    0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
+    0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0,
+    0x1.2d285a6e4030bp+0, 0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0,
+    0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0, 0x1.3dea64c123422p+0,
+    0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
+    0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0,
+    0x1.56f4736b527dap+0, 0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0,
+    0x1.6247eb03a5585p+0, 0x1.6623882552225p+0, 0x1.6a09e667f3bcdp+0,
+    0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
+    0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0,
+    0x1.868d99b4492edp+0, 0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0,
+    0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0, 0x1.9c49182a3f090p+0,
+    0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0,
+    0x1.bcc1e904bc1d2p+0, 0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0,
+    0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0, 0x1.d5818dcfba487p+0,
+    0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
+    0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0,
+    0x1.fa7c1819e90d8p+0,
+};
+__device__ __forceinline__ double mexp(double x)
+{
+    const double kf = __builtin_rint(x * 0x1.71547652b82fep+6);  // 64 / ln2
+    double r = fma(kf, -0x1.62e42fefa39efp-7, x);                // ln2 / 64, high part
+    r = fma(kf, -0x1.abc9e3b39803fp-62, r);                      // and the rest
+    const int k = (int)kf;
+    double p = fma(r, 1.0 / 120.0, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(kExp2Tab[k & 63] * p, k >> 6);
+}
+
 // The FP16 operands carry the weights scaled by 2^12.  At the shift's own iterate the weights
 // sum to one, so the largest lies in [1/nq, 1] and every operand in FP16's range with full
 // precision for the terms that matter; each lane keeps a running maximum of its weights, and a
@@ -1299,7 +1343,7 @@ __device__ __forceinline__ double h16_weight(double x, int q, const double *sWc,
 {
     const int row = lane >> 4;
     const bool odd = lane & 1;
-    const double we = sWc[q] * exp(x - shiftOwn);
+    const double we = sWc[q] * mexp(x - shiftOwn);
     const float wf = (float)we;
     wmaxOwn = fmaxf(wmaxOwn, wf);
     const float v = wf * kWScale;
@@ -1576,7 +1620,7 @@ __device__ __forceinline__ void col_exponents(const ExpB<NM> &aT, AF am, d4 (&x)
 __device__ __forceinline__ double col_weight(double x, double wq, double shiftOwn, double &fo,
                                              float &wmaxOwn, _Float16 &hi, _Float16 &lo)
 {
-    const double we = wq * exp(x - shiftOwn);
+    const double we = wq * mexp(x - shiftOwn);
     fo = fo + we;
     const float wf = (float)we;
     wmaxOwn = fmaxf(wmaxOwn, wf);
@@ -1704,10 +1748,12 @@ __device__ __forceinline__ double col_g(const d4 (&ga)[4], int r)
 __device__ __forceinline__ void dma_units(const void *src, void *dst, int units, int wv, int nw,
                                           int lane)
 {
+    // dst points into LDS: its low 32 bits are the LDS address (no aperture test needed)
+    const uint32_t lds = (uint32_t)(uintptr_t)dst;
     for (int j = wv; j < units; j += nw)
         __builtin_amdgcn_global_load_lds(
             (const void *)((const char *)src + j * 1024 + lane * 16),
-            (__attribute__((address_space(3))) void *)((char *)dst + j * 1024), 16, 0, 0);
+            (__attribute__((address_space(3))) void *)(uintptr_t)(lds + j * 1024), 16, 0, 0);
 }
 __device__ __forceinline__ void dma_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 template <int N>
@@ -1766,7 +1812,7 @@ __device__ __forceinline__ double sweep_lone(int nqPad, const double *__restrict
 #pragma unroll
     for (int i = 0; i < NA; ++i)
         acc[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    auto sync = [] {
+    auto sync = []() __attribute__((always_inline)) {
         if constexpr (NWV == 1)
             wave_lds_sync();
         else
@@ -1774,23 +1820,12 @@ __device__ __forceinline__ double sweep_lone(int nqPad, const double *__restrict
     };
     // one chunk from its rows sM (weights after them), its pair image sP (complete once
     // image_ready returns)
-    auto chunk = [&](const double *sM, const _Float16 *sP, auto image_ready) {
+    auto chunk = [&](const double *sM, const _Float16 *sP,
+                     auto image_ready) __attribute__((always_inline)) {
         const double *sW = sM + RI::kW;
-        if constexpr (NWV == 1) {
-            double x[4];
-            exp_arg4<NM>(ax, [&](int b, int s) {
-                const int m = 4 * s + (lane >> 4);
-                return (NM % 4 == 0 || m < NM) ? sM[(16 * b + o) * NMS + m] : 0.0;
-            }, x);
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const double we = sW[16 * b + o] * exp(x[b] - shiftOwn);
-                wmaxOwn = fmaxf(wmaxOwn, (float)we);
-                if (lane < 16)
-                    sWE[16 * b + o] = we;
-            }
-        } else {
-            const int b = wv;
+        for (int bb = 0; bb < 4 / NWV; ++bb) {  // the blocks of this wave, one chain each
+            const int b = NWV == 1 ? bb : wv;
             double x = 0.0;
 #pragma unroll
             for (int s = 0; s < ExpA<NM>::NKS; ++s) {
@@ -1798,7 +1833,7 @@ __device__ __forceinline__ double sweep_lone(int nqPad, const double *__restrict
                 const double bm = (NM % 4 == 0 || m < NM) ? sM[(16 * b + o) * NMS + m] : 0.0;
                 x = __builtin_amdgcn_mfma_f64_4x4x4f64(ax.a[s], bm, x, 0, 0, 0);
             }
-            const double we = sW[16 * b + o] * exp(x - shiftOwn);
+            const double we = sW[16 * b + o] * mexp(x - shiftOwn);
             wmaxOwn = fmaxf(wmaxOwn, (float)we);
             if (lane < 16)
                 sWE[16 * b + o] = we;
@@ -2754,7 +2789,7 @@ __device__ __forceinline__ void f_trials_mixed(const KParams &P, const double *_
         exp_arg4<NM>(ax, [&](int g, int t) { return b[g][t]; }, xg);
 #pragma unroll
         for (int g = 0; g < 4; ++g)
-            fOwn = fOwn + w[g] * exp(xg[g] - shift);
+            fOwn = fOwn + w[g] * mexp(xg[g] - shift);
     };
     if constexpr (NKS <= 4) {
         // two chunks at a time: their eight exponent chains and exps are independent work for
@@ -2771,8 +2806,8 @@ __device__ __forceinline__ void f_trials_mixed(const KParams &P, const double *_
                 double ea[4], eb[4];
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
-                    ea[g] = wA[g] * exp(xa[g] - shift);
-                    eb[g] = wB[g] * exp(xb[g] - shift);
+                    ea[g] = wA[g] * mexp(xa[g] - shift);
+                    eb[g] = wB[g] * mexp(xb[g] - shift);
                 }
                 if (ch + 2 < nch)
                     load(bA, wA, ch + 2);
