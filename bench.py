@@ -756,7 +756,8 @@ def step_workload(args, P, torch, dev):
         cpu = {"value": round(m ** 3 / secs, 1), "unit": "cell-updates/s", "cores": threads,
                "kind": "port", "sample": f"one SSP-RK2 step of the oracle on the same "
                                          f"problem at {m}^3, {secs:.1f} s"}
-    prec = ("mixed-precision closure (FP32 matrix-core Hessian, FP64 finish)" if mixed
+    prec = ("mixed-precision closure (split FP16 matrix-core Hessian, FP64 exponents, gradient "
+            "and stopping test)" if mixed
             else "FP64 closure")
     out = {
         "metric": f"M_{N} SSP-RK2 interior cell-updates/s (time stepping, one GPU)",

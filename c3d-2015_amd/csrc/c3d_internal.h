@@ -66,7 +66,7 @@ struct ClosureConsts {
 
 // Launch shapes of the closure kernels that a context may choose among (c3d_tune).
 struct Tune {
-    int mixedWaves = 3;  // waves per SIMD of the mixed sweep (2 or 3; nm = 25 always 2)
+    int mixedWaves = 3;  // C3D_LAUNCH_MIXED_WAVES, kept for the ABI: no sweep reads it
 };
 
 struct DevBuf {

@@ -56,8 +56,10 @@ extern "C" {
 #define C3D_PREC_FP64_REFORDER 1 /* FP64 in the reference's operation order with a
                                     deterministic exp/log: bit-reproducible on a CPU */
 #define C3D_PREC_MIXED 2         /* closure: the Hessian on the FP16 matrix cores as three
-                                    split products (about 22 bits, FP32 accumulation) under a
-                                    log-sum-exp shift; exponents, f, the gradient (on the FP64
+                                    split products (about 22 bits, FP32 accumulation; for
+                                    M_1..M_3 the cells' weights times an image of the basis
+                                    pair products) under a log-sum-exp shift; exponents, f,
+                                    the gradient (on the FP64
                                     matrix cores), Cholesky, the condition indicator and the
                                     stopping test in FP64; ill-conditioned cells continue on
                                     FP64 Hessians from their iterate near the optimum, or
@@ -268,8 +270,9 @@ int c3d_set_tail_wide(c3d_ctx ctx, int max_cells);
 
 /* Launch and hand-over parameters of the closure kernels, for tuning per (N, Q); results do not
  * depend on the launch shapes.  No counterpart in the reference.
- *   C3D_LAUNCH_MIXED_WAVES        waves per SIMD of the mixed sweep: 2 or 3 (default 3; the
- *                                 M_4 sweep always runs at 2)
+ *   C3D_LAUNCH_MIXED_WAVES        2 or 3, accepted and without effect: the mixed sweeps have
+ *                                 one launch shape each (the column sweep of M_1..M_3 eight
+ *                                 waves per block, the M_4 sweep two waves per SIMD)
  *   C3D_LAUNCH_MIXED_FINISH_COND  10 * log10 of the condition indicator above which a mixed
  *                                 cell with ||g|| <= switch_tol continues on FP64 Hessians
  *                                 (default 50, i.e. 1e5); changes which cells take that path
