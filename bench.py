@@ -161,7 +161,7 @@ def mixed_sweep_work(nm, nq):
 
 
 def mixed_sweep_kernel(nm):
-    return (f"k_sweep_cols<{nm}>" if nm <= 16 else "k_sweep_mixed<25, 2>")
+    return f"k_sweep_cols<{nm}>"
 
 
 def summarize_profile(paths, steps, warmup, write_traffic=None):

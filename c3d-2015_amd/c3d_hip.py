@@ -247,8 +247,6 @@ class Ctx:
         if t is None:
             return None
         self.set_tail(t["tail_cells"], t["tail_passes"], wide_cells=t["tail_wide"])
-        if "mixed_waves" in t:
-            self.set_launch(LAUNCH_MIXED_WAVES, t["mixed_waves"])
         return t
 
     def set_launch(self, param, value):

@@ -25,7 +25,7 @@ constexpr int kChunk = 64;  // ordinates per LDS chunk in the closure sweeps
 // the column sweep's lanes hold their weights).  Built on the host (capi.hip), read by the
 // mixed sweeps (closure_fp64.hip).
 constexpr double kPairScale = 64.0;  // lifts the small products out of FP16's subnormal range
-constexpr int kPairMaxNm = 16;
+constexpr int kPairMaxNm = 25;
 __host__ __device__ constexpr int pair_count(int nm) { return nm * (nm + 1) / 2; }
 __host__ __device__ constexpr int pair_tiles(int nm) { return (pair_count(nm) + 15) / 16; }
 __host__ __device__ constexpr int pair_chunk_halves(int nm)

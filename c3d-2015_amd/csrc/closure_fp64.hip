@@ -1145,88 +1145,22 @@ __global__ __launch_bounds__(kBlock) void k_sweep_ref(KParams P, const double *_
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------- mixed-precision sweep
-// Phase one of the mixed mode, on the FP16 and FP64 matrix cores.
-//   * FP64 on the vector units: the exponent alpha.m_q (FMA chain) and the shifted weight
-//     w_q e^{alpha.m_q - shift}, summed per lane in FP64 for f (the Armijo test keeps FP64
-//     resolution); the shift is log sum_q w e at the cell's last Newton iterate.
-//   * The FP64 gradient moments sum_q w_q e_q m_q on V_MFMA_F64_4X4X4_4B_F64: the wave's four
-//     cells are the rows, four 4-column blocks the moments, four ordinates per instruction.
-//   * The Hessian on V_MFMA_F32_16X16X32_F16 (32 ordinates per instruction, FP32
-//     accumulation) as three products of FP16 halves, a_hi b_hi + a_hi b_lo + a_lo b_hi, with
-//     b = m_j split once per chunk into b_hi + b_lo (round to nearest, then the rounded
-//     remainder) and a = v m_i formed from the halves of v = 2^12 w e (split once per ordinate
-//     and cell) in packed FP16: a_hi = v_hi b_hi rounded, and a_lo the exact rounding error of
-//     that product (a fused multiply-add) plus v_hi b_lo + v_lo b_hi.  About 21 significant
-//     bits per product; the fixed scale 2^12 and the range check below keep the operands in
-//     FP16's range.
+// Phase one of the mixed mode, on the FP16 and FP64 matrix cores (the column sweep below):
+//   * FP64: the exponents alpha.m_q and the gradient moments sum_q w_q e_q m_q on the FP64
+//     matrix cores, the shifted weights w_q e^{alpha.m_q - shift} (the shift is log sum_q w e
+//     at the cell's last Newton iterate) and their sum f (the Armijo test keeps FP64
+//     resolution) on the vector units;
+//   * the Hessian on V_MFMA_F32_16X16X32_F16 (32 ordinates per instruction, FP32
+//     accumulation) as three products of FP16 halves of v = 2^12 w e and of the basis pair
+//     products, v_hi P_hi + v_hi P_lo + v_lo P_hi (each split by rounding to nearest, then the
+//     rounded remainder); about 21 significant bits; the fixed scale 2^12 and the range check
+//     below keep the operands in FP16's range.
 // Operand layouts of the 16x16x32 FP16 instruction: A[i][k] in lane i + 16 (k / 8), element
 // k % 8; B[k][j] in lane j + 16 (k / 8), element k % 8; D[i][j] in lane j + 16 (i / 4), FP32
 // register i % 4.
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 
-// Row stride (doubles) of the FP64 weights a wave keeps per cell for the gradient products:
-// the four cells' rows start in different LDS banks.
-constexpr int kWE64S = kChunk + 4;
-
-template <int NM>
-struct H16Chunk {
-    static constexpr int NT = (NM + 15) / 16;      // 16-row tiles of H
-    static constexpr int kKB = kChunk / 32;        // 32-ordinate K blocks per chunk
-    // FP16 image [kb][t][g][hi, lo][j][8]: the ordinates q0..q0+7 (q0 = 32 kb + 8 g) of column
-    // 16 t + j as one 16-byte vector, the 16 vectors of a 16-lane row contiguous
-    static constexpr int kBHalves = kKB * NT * 4 * 2 * 16 * 8;
-    static constexpr int kLo = 16 * 8;  // the lo plane's offset from the hi plane
-    static __device__ __forceinline__ int bindex(int q, int col)
-    {
-        return ((((q >> 5) * NT + (col >> 4)) * 4 + ((q & 31) >> 3)) * 2 * 16 + (col & 15)) * 8 +
-               (q & 7);
-    }
-    // zero the columns >= NM of the image (once per kernel, any thread count)
-    static __device__ __forceinline__ void clear_pad(_Float16 *sBh, int tid, int nth)
-    {
-        constexpr int kPad = 16 * NT - NM;
-        if constexpr (kPad > 0) {
-            for (int e = tid; e < kChunk * kPad; e += nth) {
-                const int q = e / kPad, col = NM + e % kPad;
-                sBh[bindex(q, col)] = (_Float16)0.0f;
-                sBh[bindex(q, col) + kLo] = (_Float16)0.0f;
-            }
-        }
-    }
-    // element (q, col) of the chunk into the image: its FP16 halves (the FP32 difference
-    // m32 - hi is exact)
-    static __device__ __forceinline__ void put(_Float16 *sBh, int q, int col, double m)
-    {
-        const float m32 = (float)m;
-        const _Float16 hi = (_Float16)m32;
-        sBh[bindex(q, col)] = hi;
-        sBh[bindex(q, col) + kLo] = (_Float16)(m32 - (float)hi);
-    }
-    // elements (q, col) and (q + 1, col), q even, as one 32-bit store per plane
-    static __device__ __forceinline__ void put_pair(_Float16 *sBh, int q, int col, double m0,
-                                                    double m1)
-    {
-        const float a = (float)m0, b = (float)m1;
-        const _Float16 ha = (_Float16)a, hb = (_Float16)b;
-        const h2 hi = {ha, hb};
-        const h2 lo = {(_Float16)(a - (float)ha), (_Float16)(b - (float)hb)};
-        *reinterpret_cast<h2 *>(&sBh[bindex(q, col)]) = hi;
-        *reinterpret_cast<h2 *>(&sBh[bindex(q, col) + kLo]) = lo;
-    }
-};
-
-// Lane roles of V_MFMA_F64_4X4X4_4B_F64 (four 4x4x4 products, block b): A[i][k] of block b
-// in lane i + 4 b + 16 k, B[k][j] in lane j + 4 b + 16 k, D[i][j] in lane j + 4 b + 16 i.  With
-// the blocks as column groups (moment 4 b + j) and the same A in every block, a lane holds
-// the weight of cell lane & 3 at ordinate lane >> 4, the basis value of moment lane & 15 at
-// ordinate lane >> 4, and the sum of cell lane >> 4 for moment lane & 15.
-__device__ __forceinline__ int g4_a_row(int lane) { return lane & 3; }
-__device__ __forceinline__ int g4_k(int lane) { return lane >> 4; }
-__device__ __forceinline__ int g4_b_col(int lane) { return lane & 15; }
-__device__ __forceinline__ int g4_d_row(int lane) { return lane >> 4; }
-__device__ __forceinline__ int g4_d_col(int lane) { return lane & 15; }
 
 // e^x for the weights of the mixed sweeps (every mixed path computes them with it, so that they
 // agree bit for bit): x = k ln2 / 64 + r by Cody-Waite reduction with fused multiply-adds
@@ -1279,13 +1213,6 @@ __device__ __forceinline__ double mexp(double x)
 // iterate) marks its H invalid (see h16_range_ok); a cell accepting such a point re-centres
 // the shift there and sweeps again before its Newton iteration.
 constexpr float kWScale = 4096.0f;
-__device__ __forceinline__ float wave_max_f(float v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1)
-        v = fmaxf(v, __shfl_xor(v, o));
-    return v;
-}
 __device__ __forceinline__ bool h16_range_ok(float wmax)
 {
     const float m = wmax * kWScale;
@@ -1328,235 +1255,10 @@ __device__ __forceinline__ void exp_arg4(const ExpA<NM> &A, BF b, double (&x)[4]
             x[g] = __builtin_amdgcn_mfma_f64_4x4x4f64(A.a[s], b(g, s), x[g], 0, 0, 0);
 }
 
-// Phase 1 of a chunk for ordinate q = 16 g + (lane & 15) of row lane >> 4, given its
-// exponent x: the FP64 weight w_q e^{x - shift} (returned; rows < CW also store it in sWE64 for
-// the gradient products) and the FP16 halves of the scaled weight (the FP32 difference v - hi
-// is exact).  sWH holds, per row c, the halves: hi at [2 c][q], lo at [2 c + 1][q] (kChunk
-// halves each); lanes 2k and 2k + 1 (ordinates q, q + 1 of one row) store them as two 32-bit
-// words, the even lane the hi pair, the odd lane the lo pair (the partner's half by a DPP swap
-// within the pair, so every lane of the wave takes part).  wmaxOwn: the lane's running
-// maximum of its weights.
-template <int CW>
-__device__ __forceinline__ double h16_weight(double x, int q, const double *sWc, double shiftOwn,
-                                             float &wmaxOwn, _Float16 *sWH, double *sWE64,
-                                             int lane)
-{
-    const int row = lane >> 4;
-    const bool odd = lane & 1;
-    const double we = sWc[q] * mexp(x - shiftOwn);
-    const float wf = (float)we;
-    wmaxOwn = fmaxf(wmaxOwn, wf);
-    const float v = wf * kWScale;
-    const _Float16 hi = (_Float16)v;
-    const _Float16 lo = (_Float16)(v - (float)hi);
-    const unsigned hb = __builtin_bit_cast(unsigned short, hi);
-    const unsigned lb = __builtin_bit_cast(unsigned short, lo);
-    const unsigned give = odd ? hb : lb;
-    const unsigned got = (unsigned)__builtin_amdgcn_mov_dpp((int)give, 0xB1, 0xF, 0xF, false);
-    const unsigned word = odd ? (got | (lb << 16)) : (hb | (got << 16));
-    if (row < CW) {
-        sWE64[row * kWE64S + q] = we;
-        reinterpret_cast<unsigned *>(sWH)[(2 * row + (odd ? 1 : 0)) * (kChunk / 2) + (q >> 1)] =
-            word;
-    }
-    return we;
-}
-
-// Phase 2: the FP64 gradient moments of a chunk on the FP64 matrix cores, rows = the CW cells
-// (zero rows beyond), 4 ordinates a step, four independent accumulators (step s into
-// accumulator s % 4).
-template <int NM, int CW>
-__device__ __forceinline__ void h16_grad(const double *sMc, const double *sWE64,
-                                         double (&gacc)[(NM + 15) / 16][4], int lane)
-{
-    constexpr int NT = (NM + 15) / 16;
-    constexpr int NMS = NM | 1;
-    const int ar = g4_a_row(lane), kk = g4_k(lane), bc = g4_b_col(lane);
-#pragma unroll 4
-    for (int s = 0; s < kChunk / 4; ++s) {
-        const int q = 4 * s + kk;
-        const double a = ar < CW ? sWE64[ar * kWE64S + q] : 0.0;
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = 16 * t + bc;
-            const double b = col < NM ? sMc[q * NMS + col] : 0.0;
-            gacc[t][s & 3] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, gacc[t][s & 3], 0, 0, 0);
-        }
-    }
-}
-
-// Phase 3: the Hessian of a chunk as FP16 split products, 32 ordinates per instruction.  The A
-// operand of tile t (rows 16 t + i16, ordinates q0..q0+7 of the lane) holds the same basis
-// values as the B operand of tile t, so one image serves both.
-template <int NM, int CW>
-__device__ __forceinline__ void h16_hess(const _Float16 *sBh, const _Float16 *sWH,
-                                         f4 (&acc)[CW][(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
-                                         int lane)
-{
-    using HC = H16Chunk<NM>;
-    constexpr int NT = HC::NT;
-    const int i16 = lane & 15, g = lane >> 4;
-#pragma unroll 1
-    for (int kb = 0; kb < HC::kKB; ++kb) {
-        const int q0 = 32 * kb + 8 * g;
-        h8 bh[NT], bl[NT];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const _Float16 *bp = &sBh[HC::bindex(q0, 16 * t + i16)];
-            bh[t] = *reinterpret_cast<const h8 *>(bp);
-            bl[t] = *reinterpret_cast<const h8 *>(bp + HC::kLo);
-        }
-        h8 vhn = *reinterpret_cast<const h8 *>(&sWH[q0]);
-        h8 vln = *reinterpret_cast<const h8 *>(&sWH[kChunk + q0]);
-#pragma unroll
-        for (int c = 0; c < CW; ++c) {
-            const h8 vh = vhn, vl = vln;
-            if (c + 1 < CW) {  // the next cell's halves in flight
-                vhn = *reinterpret_cast<const h8 *>(&sWH[(2 * c + 2) * kChunk + q0]);
-                vln = *reinterpret_cast<const h8 *>(&sWH[(2 * c + 3) * kChunk + q0]);
-            }
-            h8 ah[NT], al[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                ah[t] = vh * bh[t];
-                h8 r = __builtin_elementwise_fma(vh, bh[t], -ah[t]);  // exact: vh bh - ah
-                r = __builtin_elementwise_fma(vh, bl[t], r);
-                al[t] = __builtin_elementwise_fma(vl, bh[t], r);
-            }
-            acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0], bh[0], acc[c][0], 0, 0, 0);
-            acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0], bl[0], acc[c][0], 0, 0, 0);
-            acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[0], bh[0], acc[c][0], 0, 0, 0);
-            if constexpr (NT == 2) {
-                acc[c][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1], bh[0], acc[c][1], 0, 0, 0);
-                acc[c][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1], bl[0], acc[c][1], 0, 0, 0);
-                acc[c][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[1], bh[0], acc[c][1], 0, 0, 0);
-                acc[c][2] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1], bh[1], acc[c][2], 0, 0, 0);
-                acc[c][2] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1], bl[1], acc[c][2], 0, 0, 0);
-                acc[c][2] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[1], bh[1], acc[c][2], 0, 0, 0);
-            }
-        }
-    }
-}
-
-// One chunk of the mixed sweep for the CW cells of a wave.  Lane L works on cell L >> 4 (rows
-// >= CW compute nothing that is kept) at ordinates 16 g + (L & 15), g = 0..3: fOwn and wmaxOwn
-// are its running sum and maximum of the weights, shiftOwn its cell's shift, sWc the chunk's
-// quadrature weights.
-template <int NM, int CW>
-__device__ __forceinline__ void chunk_h16(
-    const double *sMc, const _Float16 *sBh, const double *sWc, const ExpA<NM> &ax,
-    double shiftOwn, float &wmaxOwn, _Float16 *sWH, double *sWE64,
-    f4 (&acc)[CW][(NM + 15) / 16 * ((NM + 15) / 16 + 1) / 2],
-    double (&gacc)[(NM + 15) / 16][4], double &fOwn, int lane)
-{
-    constexpr int NMS = NM | 1;
-    // 1. per group of 16 ordinates: the exponents on the matrix cores, the weights
-    const int r16 = lane & 15;
-    double xg[4];
-    exp_arg4<NM>(ax, [&](int g, int s) {
-        const int m = 4 * s + (lane >> 4);
-        return (NM % 4 == 0 || m < NM) ? sMc[(16 * g + r16) * NMS + m] : 0.0;
-    }, xg);
-#pragma unroll
-    for (int g = 0; g < 4; ++g)
-        fOwn = fOwn + h16_weight<CW>(xg[g], 16 * g + r16, sWc, shiftOwn, wmaxOwn, sWH, sWE64,
-                                     lane);
-    wave_lds_sync();
-    __builtin_amdgcn_sched_barrier(0);
-    // 2. the FP64 gradient moments
-    h16_grad<NM, CW>(sMc, sWE64, gacc, lane);
-    __builtin_amdgcn_sched_barrier(0);
-    // 3. the Hessian
-    h16_hess<NM, CW>(sBh, sWH, acc, lane);
-    wave_lds_sync();  // sWH and sWE64 are rewritten by the next chunk
-}
-
-// The FP64 gradient moments of row r (cell r) of the 4x4x4 accumulators into gm[NM]:
-// gm_j = e^shift sum_q w_q e_q m_qj, the four partial sums added in a fixed order.
-template <int NM>
-__device__ __forceinline__ void store_gm(const double (&gacc)[(NM + 15) / 16][4], int r,
-                                         double es, double *gm, int lane)
-{
-#pragma unroll
-    for (int t = 0; t < (NM + 15) / 16; ++t) {
-        const int col = 16 * t + g4_d_col(lane);
-        if (g4_d_row(lane) == r && col < NM)
-            gm[col] = ((gacc[t][0] + gacc[t][1]) + (gacc[t][2] + gacc[t][3])) * es;
-    }
-}
-
-// The mixed sweep's chunk stager: each thread holds the elements (q, col) and (q + 1, col) of
-// its (ordinate pair, column) items, so that the FP16 image takes 32-bit stores; put() writes
-// the FP64 rows, the FP16 image and the weights.
-template <int NM>
-struct PairStager {
-    static constexpr int NMS = NM | 1;
-    static constexpr int kItems = kChunk / 2 * NM;
-    static constexpr int kPer = (kItems + kBlock - 1) / kBlock;
-    double pre[kPer][2];
-    double prew;
-    __device__ __forceinline__ void fetch(const double *__restrict__ Mg,
-                                          const double *__restrict__ wg, int ch, int tid)
-    {
-        const double *src = Mg + (size_t)ch * kChunk * NM;
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int e = tid + kBlock * k;
-            const int p = e / NM, col = e - p * NM;
-            pre[k][0] = e < kItems ? src[(2 * p) * NM + col] : 0.0;
-            pre[k][1] = e < kItems ? src[(2 * p + 1) * NM + col] : 0.0;
-        }
-        prew = tid < kChunk ? wg[(size_t)ch * kChunk + tid] : 0.0;
-    }
-    __device__ __forceinline__ void put(double *sM, _Float16 *sBh, double *sW, int tid) const
-    {
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int e = tid + kBlock * k;
-            if (e < kItems) {
-                const int p = e / NM, col = e - p * NM;
-                sM[(2 * p) * NMS + col] = pre[k][0];
-                sM[(2 * p + 1) * NMS + col] = pre[k][1];
-                H16Chunk<NM>::put_pair(sBh, 2 * p, col, pre[k][0], pre[k][1]);
-            }
-        }
-        if (tid < kChunk)
-            sW[tid] = prew;
-    }
-};
-
 // Row i of the packed lower triangle starts at i (i + 1) / 2.
 __host__ __device__ constexpr int tri(int i) { return i * (i + 1) / 2; }
 
-// H = e^shift * H32: PACKED, the lower triangle by rows (entry (i, j <= i) at tri(i) + j; the
-// closure's mixed-phase layout, which its step reads); else row-major, lower triangle
-// mirrored
-template <int NM, bool PACKED = true>
-__device__ __forceinline__ void store_H32(const f4 *acc, double es, double *H, int lane)
-{
-    constexpr int NT = (NM + 15) / 16;
-    constexpr int NB = NT * (NT + 1) / 2;
-    const int gq = lane >> 4, rq = lane & 15;
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        const int R = (b == 0) ? 0 : 1, Cb = (b == 2) ? 1 : 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int i = 16 * R + 4 * gq + k, j = 16 * Cb + rq;
-            if (i < NM && j < NM && j <= i) {
-                const double h = (double)acc[b][k] * es;
-                if constexpr (PACKED) {
-                    H[tri(i) + j] = h;
-                } else {
-                    H[i * NM + j] = h;
-                    H[j * NM + i] = h;
-                }
-            }
-        }
-    }
-}
-
-// ---------------------------------------------------------------- column sweep (nm <= 16)
+// ---------------------------------------------------------------- column sweep (nm <= 25)
 // The mixed sweep for nm <= kPairMaxNm with the cells as the columns of its matrix products,
 // 16 cells a wave, cell c in lanes l with l & 15 = c:
 //   exponents  X = M alpha^T on V_MFMA_F64_16X16X4_F64 per block of 16 ordinates (A = the M
@@ -1779,7 +1481,8 @@ struct RowImg {
 // over K in order (both V_MFMA_F64_16X16X4_F64 and V_MFMA_F64_4X4X4_4B_F64), so the cell's
 // exponents come from the cheaper four-row products (the cell in row 0: lanes 0..15 hold its
 // ordinates 16 b + lane) and its gradient from vector FMA chains in the bulk's order (lane
-// 16 b + j: moment j of block b's accumulator, K order r then k); the Hessian's pair products
+// 16 b + j: moments 16 t + j of block b's accumulators, K order r then k); the Hessian's pair
+// products
 // run on the FP16 matrix cores with the cell in row 0 (A rows 1..15 zero).  With four waves,
 // wave w computes the exponents of block w and the pair products of tiles w, w + 4, ..;
 // wave 1 the gradient.  f sums the weights per ordinate slot o = lane (blocks and chunks in
@@ -1806,8 +1509,12 @@ __device__ __forceinline__ double sweep_lone(int nqPad, const double *__restrict
     ExpA<NM> ax;
     ax.load(sA, 1, lane);
     const double shiftOwn = (lane >> 4) == 0 ? shift : 0.0;
+    constexpr int NT = (NM + 15) / 16;
     float wmaxOwn = 0.0f;
-    double fo = 0.0, gl = 0.0;
+    double fo = 0.0, gl[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+        gl[t] = 0.0;
     f4 acc[NA];
 #pragma unroll
     for (int i = 0; i < NA; ++i)
@@ -1844,14 +1551,17 @@ __device__ __forceinline__ double sweep_lone(int nqPad, const double *__restrict
             for (int b = 0; b < 4; ++b)
                 fo = fo + sWE[16 * b + o];
         }
-        if (wv == (NWV == 1 ? 0 : 1) && o < NM) {
+        if (wv == (NWV == 1 ? 0 : 1)) {
             const int bg = lane >> 4;
 #pragma unroll
             for (int r = 0; r < 4; ++r)
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const int q = 16 * bg + k + 4 * r;
-                    gl = fma(sWE[q], sM[q * NMS + o], gl);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        if (16 * t + o < NM)
+                            gl[t] = fma(sWE[q], sM[q * NMS + 16 * t + o], gl[t]);
                 }
         }
         h8 ah[2], al[2];
@@ -1877,7 +1587,9 @@ __device__ __forceinline__ double sweep_lone(int nqPad, const double *__restrict
         dma_units(Pimg, sP0, PH::kUnits, 0, 1, lane);
         for (int ch = 0; ch < nch; ++ch) {
             const bool more = ch + 1 < nch;
-            dma_wait_n<PH::kUnits>();  // the chunk's rows (its pair image may be in flight)
+            // the chunk's rows, issued before its pair image (of which up to 63 copies, the
+            // counter's limit, may still be in flight)
+            dma_wait_n<(PH::kUnits < 63 ? PH::kUnits : 63)>();
             wave_lds_sync();
             chunk(sR0, sP0, [&] {
                 lds_reads_done();  // the rows are read
@@ -1895,7 +1607,7 @@ __device__ __forceinline__ double sweep_lone(int nqPad, const double *__restrict
                 dma_units(Pimg + (size_t)(ch + 1) * PH::kHalves, sP0, PH::kUnits, 0, 1, lane);
             }
         }
-    } else {
+    } else if constexpr (NM <= 16) {
         __syncthreads();  // earlier readers of the buffers are done
         dma_units(Rimg, sR0, RI::kUnits, wv, 4, lane);
         dma_units(Pimg, sP0, PH::kUnits, wv, 4, lane);
@@ -1909,6 +1621,21 @@ __device__ __forceinline__ double sweep_lone(int nqPad, const double *__restrict
                           PH::kUnits, wv, 4, lane);
             }
             chunk((ch & 1) ? sR1 : sR0, (ch & 1) ? sP1 : sP0, [] {});
+        }
+    } else {  // nm = 25: one pair-image buffer, copied at the chunk's start (sP1 unused)
+        __syncthreads();  // earlier readers of the buffers are done
+        dma_units(Rimg, sR0, RI::kUnits, wv, 4, lane);
+        for (int ch = 0; ch < nch; ++ch) {
+            dma_wait();
+            __syncthreads();  // the chunk's rows; the previous chunk's readers are done
+            dma_units(Pimg + (size_t)ch * PH::kHalves, sP0, PH::kUnits, wv, 4, lane);
+            if (ch + 1 < nch)
+                dma_units(Rimg + (size_t)(ch + 1) * RI::kDoubles, (ch & 1) ? sR0 : sR1,
+                          RI::kUnits, wv, 4, lane);
+            chunk((ch & 1) ? sR1 : sR0, sP0, [] {
+                dma_wait();
+                __syncthreads();
+            });
         }
     }
     const float wRow = row16_max(wmaxOwn);
@@ -1931,25 +1658,38 @@ __device__ __forceinline__ double sweep_lone(int nqPad, const double *__restrict
         }
     }
     if (wv == (NWV == 1 ? 0 : 1)) {
-        const double g0 = __shfl(gl, o), g1 = __shfl(gl, 16 + o);
-        const double g2 = __shfl(gl, 32 + o), g3 = __shfl(gl, 48 + o);
-        if (lane < NM)
-            sG[lane] = ((g0 + g1) + (g2 + g3)) * es;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const double g0 = __shfl(gl[t], o), g1 = __shfl(gl[t], 16 + o);
+            const double g2 = __shfl(gl[t], 32 + o), g3 = __shfl(gl[t], 48 + o);
+            if (lane < 16 && 16 * t + lane < NM)
+                sG[16 * t + lane] = ((g0 + g1) + (g2 + g3)) * es;
+        }
     }
     return wv == 0 ? __shfl(row16_sum(fo), 0) * es : 0.0;
 }
 
-constexpr int kCWaves = 8;  // waves per block of the column sweep
-constexpr int kCBlock = kCWaves * kWave;
-constexpr int kCCells = 16;  // cells per wave
+constexpr int kCCells = 16;  // cells per wave of the column sweep
+// Launch shape of the column sweep: 8 waves per block (two per SIMD) with double-buffered
+// pair images up to nm = 16; at nm = 25 (84 KB of pair image per chunk, 84 accumulator
+// registers) 4 waves per block, one per SIMD, and a single pair-image buffer
+template <int NM>
+struct ColShape {
+    static constexpr int kWaves = NM <= 16 ? 8 : 4;
+    static constexpr int kBlock = kWaves * kWave;
+    static constexpr int kPairBufs = NM <= 16 ? 2 : 1;
+};
 
-// Sweep pass of the mixed phase for nm <= kPairMaxNm (the column sweep): 16 cells per wave, 128
-// per block sharing each chunk; the chunk's row image and pair image staged by LDS DMA, both
-// double-buffered (LDS 90 KB at nm = 16).  Writes H (packed lower
+// Sweep pass of the mixed phase (the column sweep): 16 cells per wave, the block's waves
+// (ColShape) sharing each chunk; the chunk's row image and pair image staged by LDS DMA, the
+// rows double-buffered, the pair image double-buffered (LDS 90 KB at nm = 16) or, at nm = 25,
+// copied at the chunk's start and waited for before its pair products (a second barrier;
+// LDS 110 KB).  Writes H (packed lower
 // triangle, first entry NaN where the weights left the FP16 range), sum_q w e (fdir) and the
 // FP64 gradient moments (g slot).
 template <int NM>
-__global__ __launch_bounds__(kCBlock) void k_sweep_cols(KParams P, const double *__restrict__ Rimg,
+__global__ __launch_bounds__(ColShape<NM>::kBlock) void k_sweep_cols(KParams P,
+                                                                     const double *__restrict__ Rimg,
                                                         const _Float16 *__restrict__ Pimg,
                                                         State S, const int *__restrict__ W,
                                                         const int *__restrict__ nWp)
@@ -1958,7 +1698,9 @@ __global__ __launch_bounds__(kCBlock) void k_sweep_cols(KParams P, const double 
     using RI = RowImg<NM>;
     constexpr int NT = (NM + 15) / 16;
     constexpr int PT = PH::PT;
-    __shared__ __align__(16) _Float16 sP[2][PH::kHalves];
+    constexpr int kCWaves = ColShape<NM>::kWaves;
+    constexpr int PB = ColShape<NM>::kPairBufs;
+    __shared__ __align__(16) _Float16 sP[PB][PH::kHalves];
     __shared__ __align__(16) double sR[2][RI::kDoubles];
     const int lane = threadIdx.x & 63, wv = uniform_int(threadIdx.x >> 6);
     const int i16 = lane & 15, g = lane >> 4;
@@ -1988,20 +1730,33 @@ __global__ __launch_bounds__(kCBlock) void k_sweep_cols(KParams P, const double 
             acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
         __syncthreads();  // the previous cells' last chunk is consumed
         dma_units(Rimg, sR[0], RI::kUnits, wv, kCWaves, lane);
-        dma_units(Pimg, sP[0], PH::kUnits, (wv + RI::kUnits) % kCWaves, kCWaves, lane);
+        if constexpr (PB == 2)
+            dma_units(Pimg, sP[0], PH::kUnits, (wv + RI::kUnits) % kCWaves, kCWaves, lane);
         for (int ch = 0; ch < nch; ++ch) {
             const int b = ch & 1;
             dma_wait();
-            __syncthreads();
+            __syncthreads();  // the chunk's rows; every wave is done with the previous chunk
+            if constexpr (PB == 1)
+                dma_units(Pimg + (size_t)ch * PH::kHalves, sP[0], PH::kUnits,
+                          (wv + RI::kUnits) % kCWaves, kCWaves, lane);
             if (ch + 1 < nch) {
                 dma_units(Rimg + (size_t)(ch + 1) * RI::kDoubles, sR[b ^ 1], RI::kUnits, wv,
                           kCWaves, lane);
-                dma_units(Pimg + (size_t)(ch + 1) * PH::kHalves, sP[b ^ 1], PH::kUnits,
-                          (wv + RI::kUnits) % kCWaves, kCWaves, lane);
+                if constexpr (PB == 2)
+                    dma_units(Pimg + (size_t)(ch + 1) * PH::kHalves, sP[b ^ 1], PH::kUnits,
+                              (wv + RI::kUnits) % kCWaves, kCWaves, lane);
             }
-            if (any)
-                chunk_cols<NM>(sR[b], sR[b] + RI::kW, sP[b], aT, shift, wmaxOwn, fb, gacc, acc,
-                               lane, [] {});
+            if constexpr (PB == 2) {
+                if (any)
+                    chunk_cols<NM>(sR[b], sR[b] + RI::kW, sP[b], aT, shift, wmaxOwn, fb, gacc,
+                                   acc, lane, [] {});
+            } else {  // every wave takes the barrier before the pair products
+                chunk_cols<NM>(sR[b], sR[b] + RI::kW, sP[0], aT, shift, wmaxOwn, fb, gacc, acc,
+                               lane, [] {
+                                   dma_wait();
+                                   __syncthreads();
+                               });
+            }
         }
         if (!any)
             continue;
@@ -2043,104 +1798,6 @@ __global__ __launch_bounds__(kCBlock) void k_sweep_cols(KParams P, const double 
                         gm[col] = col_g(gacc[t], r) * esr;
                 }
             }
-        }
-    }
-}
-
-// Sweep pass of the mixed phase: 4 cells per wave, 16 per block sharing each LDS chunk (its
-// FP64 rows for the exponents and the gradient, its FP32 transpose and FP16 halves for the
-// Hessian).  Writes H, sum_q w e (fdir) and the FP64 gradient moments (the cell's g slot).
-// WPE waves per SIMD (the register budget 512 / WPE; LDS 49 KB per block at nm = 16, 76 KB
-// at nm = 25), chosen per context (c3d_tune).
-template <int NM, int WPE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_sweep_mixed(KParams P, const double *__restrict__ Mg,
-                                                         const double *__restrict__ wg, State S,
-                                                         const int *__restrict__ W,
-                                                         const int *__restrict__ nWp)
-{
-    using HC = H16Chunk<NM>;
-    constexpr int NT = HC::NT;
-    constexpr int NB = NT * (NT + 1) / 2;
-    constexpr int NMS = NM | 1;
-    constexpr int CW = kCellsPerWave;
-    __shared__ double sM[2][kChunk * NMS];
-    __shared__ __align__(16) _Float16 sBh[2][HC::kBHalves];
-    __shared__ __align__(16) _Float16 sWH[kWavesPerBlock][2 * CW * kChunk];
-    __shared__ __align__(16) double sWE64[kWavesPerBlock][CW * kWE64S];
-    __shared__ double sW[2][kChunk];
-    __shared__ double sA[kWavesPerBlock][CW * NM];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int nch = P.nqPad / kChunk;
-    const int nW = uniform_int(*nWp);
-    const int groups = (nW + CW - 1) / CW;
-    const int nBlocksNeeded = (groups + kWavesPerBlock - 1) / kWavesPerBlock;
-    HC::clear_pad(sBh[0], tid, kBlock);
-    HC::clear_pad(sBh[1], tid, kBlock);
-    for (int blk = blockIdx.x; blk < nBlocksNeeded; blk += gridDim.x) {
-        const int grp = blk * kWavesPerBlock + wv;
-        int cells[CW];
-        double shift[CW];
-#pragma unroll
-        for (int c = 0; c < CW; ++c) {
-            const int slot = grp * CW + c;
-            cells[c] = (grp < groups && slot < nW) ? W[slot] : -1;
-            shift[c] = cells[c] >= 0 ? S.scal[(size_t)cells[c] * kScal + kSshift] : 0.0;
-            if (lane < NM)
-                sA[wv][c * NM + lane] =
-                    cells[c] >= 0 ? S.vec[(size_t)cells[c] * kVec * NM + kVa * NM + lane] : 0.0;
-        }
-        f4 acc[CW][NB];
-        double gacc[NT][4];
-#pragma unroll
-        for (int c = 0; c < CW; ++c)
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-                acc[c][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int t = 0; t < NT; ++t)
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                gacc[t][k] = 0.0;
-        // the lane's row: cell lane >> 4
-        double shiftOwn = 0.0, fOwn = 0.0;
-        float wmaxOwn = 0.0f;
-#pragma unroll
-        for (int c = 0; c < CW; ++c)
-            if ((lane >> 4) == c)
-                shiftOwn = shift[c];
-        wave_lds_sync();  // sA
-        ExpA<NM> ax;
-        ax.load(sA[wv], CW, lane);
-        const bool any = cells[0] >= 0;
-        PairStager<NM> st;
-        st.fetch(Mg, wg, 0, tid);
-        __syncthreads();  // the previous cells' last chunk is consumed
-        for (int ch = 0; ch < nch; ++ch) {
-            const int b = ch & 1;
-            st.put(sM[b], sBh[b], sW[b], tid);
-            __syncthreads();
-            if (ch + 1 < nch)
-                st.fetch(Mg, wg, ch + 1, tid);
-            if (any)
-                chunk_h16<NM, CW>(sM[b], sBh[b], sW[b], ax, shiftOwn, wmaxOwn, sWH[wv],
-                                  sWE64[wv], acc, gacc, fOwn, lane);
-        }
-        const double fRow = row16_sum(fOwn);
-        const float wRow = row16_max(wmaxOwn);
-#pragma unroll
-        for (int c = 0; c < CW; ++c) {
-            const double fs = __shfl(fRow, 16 * c);
-            const bool ok = h16_range_ok(__shfl(wRow, 16 * c));
-            if (cells[c] < 0)
-                continue;
-            const double es = exp(shift[c]);
-            double *H = S.H + (size_t)cells[c] * NM * NM;
-            if (lane == 0)
-                S.fdir[cells[c]] = fs * es;
-            store_H32<NM>(acc[c], es / kWScale, H, lane);
-            store_gm<NM>(gacc, c, es, S.vec + (size_t)cells[c] * kVec * NM + kVg * NM, lane);
-            if (!ok && lane == 0)
-                H[0] = __builtin_nan("");
         }
     }
 }
@@ -2672,90 +2329,7 @@ __device__ __forceinline__ void f_trials(const KParams &P, const double *__restr
     }
 }
 
-// The mixed sweep of one cell by the four waves of a tail block (256 threads), with exactly the
-// operations a wave of k_sweep_mixed performs for that cell (CW = 1), so the cell's H, f and
-// gradient are the same bits: all waves stage each chunk (PairStager); wave g computes the
-// exponents and weights of ordinate group g (its exponent chain is exp_arg4's chain g); then
-// wave 0 adds the weights into its lanes' running sums in group order and runs the Hessian
-// products while wave 1 runs the gradient products (each accumulator's instruction sequence
-// unchanged).  Results: H (scaled by e^shift, invalid marked as k_sweep_mixed marks it) in sH
-// and the gradient moments in sG; the return value is f (wave 0).  ax: the cell's exponent
-// operand (every wave); sRed: 4 floats.
-template <int NM>
-__device__ __forceinline__ double sweep_mixed_block(const KParams &P, const double *__restrict__ Mg,
-                                                    const double *__restrict__ wg,
-                                                    const ExpA<NM> &ax, double shift, double *sM,
-                                                    _Float16 *sBh, _Float16 *sWH, double *sWE64,
-                                                    double *sW0, double *sH, double *sG,
-                                                    float *sRed, int lane, int wv)
-{
-    using HC = H16Chunk<NM>;
-    constexpr int NT = HC::NT;
-    constexpr int NB = NT * (NT + 1) / 2;
-    constexpr int NMS = NM | 1;
-    const int tid = threadIdx.x, r16 = lane & 15;
-    const double shiftOwn = (lane >> 4) == 0 ? shift : 0.0;
-    f4 acc[1][NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-        acc[0][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    double gacc[NT][4];
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            gacc[t][k] = 0.0;
-    double fOwn = 0.0;
-    float wmaxOwn = 0.0f;
-    const int nch = P.nqPad / kChunk;
-    PairStager<NM> st;
-    st.fetch(Mg, wg, 0, tid);
-    for (int ch = 0; ch < nch; ++ch) {
-        __syncthreads();  // the previous chunk's readers are done
-        st.put(sM, sBh, sW0, tid);
-        __syncthreads();
-        if (ch + 1 < nch)
-            st.fetch(Mg, wg, ch + 1, tid);
-        {
-            const int g = wv;
-            double x = 0.0;
-#pragma unroll
-            for (int s = 0; s < ExpA<NM>::NKS; ++s) {
-                const int m = 4 * s + (lane >> 4);
-                const double b = (NM % 4 == 0 || m < NM) ? sM[(16 * g + r16) * NMS + m] : 0.0;
-                x = __builtin_amdgcn_mfma_f64_4x4x4f64(ax.a[s], b, x, 0, 0, 0);
-            }
-            (void)h16_weight<1>(x, 16 * g + r16, sW0, shiftOwn, wmaxOwn, sWH, sWE64, lane);
-        }
-        __syncthreads();
-        if (wv == 0) {
-#pragma unroll
-            for (int g = 0; g < 4; ++g)
-                fOwn = fOwn + sWE64[16 * g + r16];
-            h16_hess<NM, 1>(sBh, sWH, acc, lane);
-        } else if (wv == 1) {
-            h16_grad<NM, 1>(sM, sWE64, gacc, lane);
-        }
-    }
-    const float wRow = row16_max(wmaxOwn);
-    if (lane == 0)
-        sRed[wv] = wRow;
-    __syncthreads();
-    const double es = exp(shift);
-    double fdir = 0.0;
-    if (wv == 0) {
-        const bool ok = h16_range_ok(fmaxf(fmaxf(sRed[0], sRed[1]), fmaxf(sRed[2], sRed[3])));
-        fdir = __shfl(row16_sum(fOwn), 0) * es;
-        store_H32<NM>(acc[0], es / kWScale, sH, lane);
-        if (!ok && lane == 0)
-            sH[0] = __builtin_nan("");
-    } else if (wv == 1) {
-        store_gm<NM>(gacc, 0, es, sG, lane);
-    }
-    return fdir;
-}
-
-// f at up to four trial points in the mixed phase, as chunk_h16 evaluates it: the points are
+// f at up to four trial points in the mixed phase, as the column sweep evaluates it: the points are
 // the rows of the exponent products (alpha of point j at sA + j * NM, j < kw), lane L takes
 // point L >> 4 at ordinates 16 g + (L & 15) of each chunk and sums its weights in chunk and
 // group order, then the 16-lane row sum and the e^shift rescale.  M is read from its
@@ -2841,7 +2415,7 @@ __device__ __forceinline__ void f_trials_mixed(const KParams &P, const double *_
 // For the last few cells: one block per cell runs up to `passes` rounds in one launch, with
 // the cell's state in registers and its trial points, M chunks, H and the transposed factor
 // in LDS.  The sweep performs the same operations in the same order as k_sweep /
-// k_sweep_ref / k_sweep_mixed do for that cell, so a cell's result does not depend on which
+// k_sweep_ref / k_sweep_cols do for that cell, so a cell's result does not depend on which
 // kernel finished it.  A line-search trial needs f alone (H only once a trial is accepted).
 // The TW waves of a block evaluate f at the next trial points of the backtracking search
 // (tls, tls/2, tls/4, ...: one per wave, or KW per wave once the search has backtracked), so
@@ -2875,7 +2449,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     constexpr bool COLS = MIXED && NM <= kPairMaxNm;
     constexpr bool COLS4 = COLS && TW == 4;
     __shared__ __align__(16) double sR[COLS4 ? 2 : 1][COLS ? RowImg<NM>::kDoubles : 1];
-    __shared__ __align__(16) _Float16 sP[COLS4 ? 2 : 1][COLS ? PairHess<NM>::kHalves : 8];
+    __shared__ __align__(16) _Float16 sP[COLS4 && NM <= 16 ? 2 : 1][COLS ? PairHess<NM>::kHalves : 8];
     __shared__ double sMo[COLS ? 1 : kChunk * NMS];
     double *const sM = COLS ? &sR[0][0] : sMo;
     __shared__ double sH[NM * NM];
@@ -2890,11 +2464,6 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     __shared__ double sA[TW][KWA * NM];
     __shared__ int sOk[TW * KWA];       // Armijo verdict at each trial point
     __shared__ double sF[1];            // f of the sweep
-    // the mixed sweep's images for nm > kPairMaxNm: H16Chunk's image, the weights' halves and
-    // FP64 weights
-    __shared__ __align__(16) _Float16 sBh[MIXED && !COLS ? H16Chunk<NM>::kBHalves : 8];
-    __shared__ __align__(16) _Float16 sWH[MIXED && !COLS ? 2 * kChunk : 8];
-    __shared__ __align__(16) double sWE64[MIXED && !COLS ? kWE64S : 2];
     __shared__ double sWEl[COLS ? kChunk : 1];  // the lone-cell sweep's weights
     __shared__ double sG[NM];
     __shared__ float sRed[4];
@@ -2904,10 +2473,6 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     if ((int)blockIdx.x >= nW)
         return;
     const int li = lane & (G - 1), row = lane / G;
-    if constexpr (MIXED && !COLS) {
-        H16Chunk<NM>::clear_pad(sBh, threadIdx.x, TW * kWave);
-        __syncthreads();
-    }
     for (int slot = blockIdx.x; slot < nW; slot += gridDim.x) {
         const int c = uniform_int(list_cell(Lc, slot, nWm));
         Cell<NM, G> C;
@@ -2985,17 +2550,9 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                     if (wv == 0 && lane < NM)
                         sA[0][lane] = C.a;
                     __syncthreads();
-                    double fdir;
-                    if constexpr (COLS) {
-                        fdir = sweep_lone<NM, 4>(P.nqPad, Rimg, Pimg, sA[0], C.shift, sR[0],
-                                                 sR[COLS4 ? 1 : 0], sP[0], sP[COLS4 ? 1 : 0], sWEl,
-                                                 sRed, sH, sG, true, lane, wv);
-                    } else {
-                        ExpA<NM> ax;
-                        ax.load(sA[0], 1, lane);
-                        fdir = sweep_mixed_block<NM>(P, Mg, wg, ax, C.shift, sM, sBh, sWH, sWE64,
-                                                     sW[0], sH, sG, sRed, lane, wv);
-                    }
+                    const double fdir = sweep_lone<NM, 4>(
+                        P.nqPad, Rimg, Pimg, sA[0], C.shift, sR[0], sR[COLS4 ? 1 : 0], sP[0],
+                        sP[COLS4 && NM <= 16 ? 1 : 0], sWEl, sRed, sH, sG, true, lane, wv);
                     if (wv == 0) {
                         if (lane == 0)
                             sF[0] = fdir;
@@ -3008,17 +2565,10 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 double fdir;
                 if (lane < NM)
                     sA[0][lane] = C.a;
-                bool colSwept = false;
-                if constexpr (COLS) {
-                    if (mx) {
-                        wave_lds_sync();  // sA[0]
-                        fdir = sweep_lone<NM, 1>(P.nqPad, Rimg, Pimg, sA[0], C.shift, sR[0],
-                                                 sR[0], sP[0], sP[0], sWEl, sRed, sH, sG, true,
-                                                 lane, 0);
-                        colSwept = true;
-                    }
-                }
-                if (colSwept) {
+                if (mx) {  // the mixed phase's lone-cell sweep
+                    wave_lds_sync();  // sA[0]
+                    fdir = sweep_lone<NM, 1>(P.nqPad, Rimg, Pimg, sA[0], C.shift, sR[0], sR[0],
+                                             sP[0], sP[0], sWEl, sRed, sH, sG, true, lane, 0);
                 } else if constexpr (REF) {
                     const RefEntries<NM> E(lane);
                     double h[RefEntries<NM>::NS];
@@ -3042,29 +2592,12 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                     fdir = f;
                 } else {
                     d4 acc[1][NB];
-                    f4 acc32[1][NB];
 #pragma unroll
-                    for (int b = 0; b < NB; ++b) {
+                    for (int b = 0; b < NB; ++b)
                         acc[0][b] = d4{0.0, 0.0, 0.0, 0.0};
-                        acc32[0][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-                    }
                     double fpart[1] = {0.0};
-                    double gacc[NT][4];
-                    float wmaxOwn = 0.0f;
-                    double fOwn = 0.0;
-                    const double shiftOwn = (lane >> 4) == 0 ? C.shift : 0.0;
-#pragma unroll
-                    for (int t = 0; t < NT; ++t)
-#pragma unroll
-                        for (int k = 0; k < 4; ++k)
-                            gacc[t][k] = 0.0;
-                    ExpA<NM> ax;
-                    if (mx) {
-                        wave_lds_sync();  // sA[0]
-                        ax.load(sA[0], 1, lane);
-                    }
                     double pre[NM];
-                    const int qo = mx ? lane : mfma_ordinate(lane);  // the lane's weight
+                    const int qo = mfma_ordinate(lane);  // the lane's weight
                     double prew = wg[qo];
 #pragma unroll
                     for (int k = 0; k < NM; ++k)
@@ -3076,13 +2609,8 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                             const int e = lane + kWave * k;
                             const int rr = e / NM, cc = e - rr * NM;
                             sM[rr * NMS + cc] = pre[k];
-                            if constexpr (MIXED && !COLS)
-                                if (mx)
-                                    H16Chunk<NM>::put(sBh, rr, cc, pre[k]);
                         }
                         const double wl = prew;
-                        if (mx)
-                            sW[0][lane] = wl;  // the chunk's weights (qo = lane)
                         wave_lds_sync();
                         if (ch + 1 < nch) {
                             const size_t base = (size_t)(ch + 1) * kChunk;
@@ -3091,27 +2619,11 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                             for (int k = 0; k < NM; ++k)
                                 pre[k] = Mg[base * NM + lane + kWave * k];
                         }
-                        if constexpr (MIXED && !COLS) {
-                            if (mx)
-                                chunk_h16<NM, 1>(sM, sBh, sW[0], ax, shiftOwn, wmaxOwn, sWH,
-                                                 sWE64, acc32, gacc, fOwn, lane);
-                        }
-                        if (!mx)
-                            chunk_mfma<NM, 1>(sM, wl, sA[0], acc, fpart, lane);
+                        chunk_mfma<NM, 1>(sM, wl, sA[0], acc, fpart, lane);
                     }
                     wave_lds_sync();
-                    if (mx) {
-                        const double es = exp(C.shift);
-                        fdir = __shfl(row16_sum(fOwn), 0) * es;
-                        const bool ok = h16_range_ok(__shfl(row16_max(wmaxOwn), 0));
-                        store_H32<NM>(acc32[0], es / kWScale, sH, lane);
-                        store_gm<NM>(gacc, 0, es, sG, lane);
-                        if (!ok && lane == 0)
-                            sH[0] = __builtin_nan("");
-                    } else {
-                        fdir = wave_sum(fpart[0]);
-                        store_H<NM>(acc[0], sH, lane);
-                    }
+                    fdir = wave_sum(fpart[0]);
+                    store_H<NM>(acc[0], sH, lane);
                 }
                 if (lane == 0)
                     sF[0] = fdir;
@@ -3521,92 +3033,6 @@ __global__ __launch_bounds__(kWave) void k_hessian_cols(int64_t ncells, int nqPa
     }
 }
 
-// H only, mixed mode (FP32 matrix cores, zero shift): one wave per cell.
-template <int NM>
-__global__ __launch_bounds__(kBlock) void k_hessian_mixed(int64_t ncells, int nqPad,
-                                                         const double *__restrict__ Mg,
-                                                         const double *__restrict__ wg,
-                                                         const double *__restrict__ alpha_g,
-                                                         double *hs)
-{
-    using HC = H16Chunk<NM>;
-    constexpr int NT = HC::NT;
-    constexpr int NB = NT * (NT + 1) / 2;
-    constexpr int NMS = NM | 1;
-    __shared__ double sM[kChunk * NMS];
-    __shared__ __align__(16) _Float16 sBh[HC::kBHalves];
-    __shared__ __align__(16) _Float16 sWH[kWavesPerBlock][2 * kChunk];
-    __shared__ __align__(16) double sWE64[kWavesPerBlock][kWE64S];
-    __shared__ double sW[kChunk];
-    __shared__ double sA[kWavesPerBlock][NM];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    HC::clear_pad(sBh, tid, kBlock);
-    const int64_t cell = (int64_t)blockIdx.x * kWavesPerBlock + wv;
-    const bool have = cell < ncells;
-    if (lane < NM)
-        sA[wv][lane] = have ? alpha_g[cell * NM + lane] : 0.0;
-    f4 acc[1][NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-        acc[0][b] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    double gacc[NT][4];
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            gacc[t][k] = 0.0;
-    double fOwn = 0.0;
-    float wmaxOwn = 0.0f;
-    // a first pass for the shift: the largest exponent and weight, so that every weight of the
-    // second pass is at most one and the largest at least min w / max w
-    double xmax = -DBL_MAX, wtop = 0.0;
-    for (int c = 0; c < nqPad / kChunk; ++c) {
-        __syncthreads();
-        for (int e = tid; e < kChunk * NM; e += kBlock) {
-            int rr = e / NM, cc = e - rr * NM;
-            sM[rr * NMS + cc] = Mg[(size_t)c * kChunk * NM + e];
-        }
-        if (tid < kChunk)
-            sW[tid] = wg[(size_t)c * kChunk + tid];
-        __syncthreads();
-        if (have && sW[lane] > 0.0) {
-            double x = 0.0;
-#pragma unroll
-            for (int i = 0; i < NM; ++i)
-                x = fma(sA[wv][i], sM[lane * NMS + i], x);
-            xmax = fmax(xmax, x);
-            wtop = fmax(wtop, sW[lane]);
-        }
-    }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        xmax = fmax(xmax, __shfl_xor(xmax, o));
-        wtop = fmax(wtop, __shfl_xor(wtop, o));
-    }
-    const double shift = have ? xmax + log(wtop) : 0.0;
-    const double shiftOwn = (lane >> 4) == 0 ? shift : 0.0;
-    ExpA<NM> ax;
-    ax.load(sA[wv], 1, lane);  // sA was published by the first pass's barriers
-    for (int c = 0; c < nqPad / kChunk; ++c) {
-        __syncthreads();
-        for (int e = tid; e < kChunk * NM; e += kBlock) {
-            int rr = e / NM, cc = e - rr * NM;
-            const double m = Mg[(size_t)c * kChunk * NM + e];
-            sM[rr * NMS + cc] = m;
-            HC::put(sBh, rr, cc, m);
-        }
-        if (tid < kChunk)
-            sW[tid] = wg[(size_t)c * kChunk + tid];
-        __syncthreads();
-        if (have)
-            chunk_h16<NM, 1>(sM, sBh, sW, ax, shiftOwn, wmaxOwn, sWH[wv], sWE64[wv], acc, gacc,
-                             fOwn, lane);
-    }
-    if (!have)
-        return;
-    store_H32<NM, false>(acc[0], exp(shift) / kWScale, hs + cell * NM * NM, lane);
-}
-
 // The dense part of one Newton iteration for a batch of given Hessians (one wave each):
 // info = dpotrf, rcond = dpocon(norm1(H)), d = -H^{-1} g (custom or LAPACK solve order).
 template <int NM>
@@ -3815,14 +3241,9 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
             return hipErrorOutOfMemory;
         if (ev)
             (void)hipEventRecord(ev[0], s);
-        if (mixed) {
-            if constexpr (NM <= kPairMaxNm)
-                hipLaunchKernelGGL(k_sweep_cols<NM>, dim3(grid_cap(ctx, 1)), dim3(kCBlock), 0, s,
-                                   P, ctx->d_R, (const _Float16 *)ctx->d_P16, S, Lc.Wm, Lc.nm);
-            else
-                hipLaunchKernelGGL((k_sweep_mixed<NM, 2>), dim3(grid_cap(ctx, 8)), dim3(kBlock),
-                                   0, s, P, ctx->d_M, ctx->d_wpad, S, Lc.Wm, Lc.nm);
-        }
+        if (mixed)
+            hipLaunchKernelGGL(k_sweep_cols<NM>, dim3(grid_cap(ctx, 1)), dim3(ColShape<NM>::kBlock),
+                               0, s, P, ctx->d_R, (const _Float16 *)ctx->d_P16, S, Lc.Wm, Lc.nm);
         if (ev)
             (void)hipEventRecord(ev[1], s);
         if (ref)
@@ -3860,8 +3281,8 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
             C3D_TAIL(true, false, 4);
         else if (ref)
             C3D_TAIL(true, false, 1);
-        else if (mixed && wide)
-            C3D_TAIL(false, true, 4);
+        else if (mixed && (wide || NM > 16))  // nm = 25: the one-wave kernel's 110 KB of LDS
+            C3D_TAIL(false, true, 4);         // leave one block per CU either way
         else if (mixed)
             C3D_TAIL(false, true, 1);
         else if (wide)
@@ -4021,10 +3442,7 @@ hipError_t launch_hessian_fp64(c3d_ctx_s *ctx, int64_t ncells, const double *alp
         case 4: C3D_HC(4); break;
         case 9: C3D_HC(9); break;
         case 16: C3D_HC(16); break;
-        case 25:
-            hipLaunchKernelGGL(k_hessian_mixed<25>, grid, dim3(kBlock), 0, s, ncells, ctx->nqPad,
-                               ctx->d_M, ctx->d_wpad, alpha, hs);
-            break;
+        case 25: C3D_HC(25); break;
         default: return hipErrorInvalidValue;
         }
 #undef C3D_HC

@@ -11,7 +11,6 @@ runs in the benchmark) under each candidate of a small grid of:
   tail_passes  rounds per tail launch (c3d_set_tail)
   tail_wide    cells left when the tail evaluates four line-search trials per round
                (c3d_set_tail_wide)
-  mixed_waves  waves per SIMD of the mixed sweep (c3d_set_launch, mixed with nm > 16)
 
 and writes the fastest to a JSON table keyed "N,Q,precision", which Ctx.apply_tuning() (and
 through it bench.py) reads.  Usage (on the GPU):
@@ -48,8 +47,6 @@ def front_end():
 def candidates(precision, nm):
     grid = {"tail_cells": (1024, 2048, 4096, 8192, 32768), "tail_passes": (32, 128),
             "tail_wide": (256, 512, 2048, 8192)}
-    if precision == "mixed" and nm > 16:  # the column sweep (nm <= 16) has one launch shape
-        grid["mixed_waves"] = (2, 3)
     keys = list(grid)
     for vals in itertools.product(*(grid[k] for k in keys)):
         c = dict(zip(keys, vals))
@@ -80,8 +77,6 @@ def tune_shape(P, torch, dev, N, Q, precision, cells, seed, reps, log):
 
     def run(c):
         ctx.set_tail(c["tail_cells"], c["tail_passes"], wide_cells=c["tail_wide"])
-        if "mixed_waves" in c:
-            ctx.set_launch(P.LAUNCH_MIXED_WAVES, c["mixed_waves"])
         best = float("inf")
         for r in range(reps + 1):
             u.copy_(u0)
@@ -103,7 +98,7 @@ def tune_shape(P, torch, dev, N, Q, precision, cells, seed, reps, log):
     results.sort(key=lambda x: x[0])
     best_t, best = results[0]
     default = next((t for t, c in results if c["tail_cells"] == 8192 and c["tail_passes"] == 32
-                    and c["tail_wide"] == 512 and c.get("mixed_waves", 3) == 3), None)
+                    and c["tail_wide"] == 512), None)
     ctx.close()
     return {**best, "ms": round(1e3 * best_t, 3),
             "default_ms": None if default is None else round(1e3 * default, 3),

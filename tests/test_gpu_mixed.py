@@ -77,6 +77,9 @@ def test_closure_mixed_against_oracle_and_fp64(P, O, dev, N, Q, n):
     um, am, sm, rc_m = _run(P, dev, T, u, a0, P.closure_opts(), P.PREC_MIXED)
     uf, af, sf, rc_f = _run(P, dev, T, u, a0, P.closure_opts(), P.PREC_FP64)
     fb = (sm["flags"] & P.FLAG_FP64_FALLBACK) != 0
+    # the mixed phase solves nearly every cell itself (a fallback restarts from the input on
+    # the FP64 path, whose results would pass every bar below)
+    assert fb.mean() <= 0.01, fb.mean()
     # cells handed over end exactly as the FP64 mode ends them
     assert np.array_equal(am[fb], af[fb]) and np.array_equal(um[fb], uf[fb])
     for k in ("totalIter", "iterGamma", "regIndex", "r", "err", "gamma"):
@@ -105,11 +108,12 @@ def test_closure_mixed_against_oracle_and_fp64(P, O, dev, N, Q, n):
     assert rc_m == rc_f
 
 
-def test_closure_mixed_result_independent_of_worklist_strategy(P, O, dev):
-    T = O.Tables(3, 24)
-    n = 2000
+@pytest.mark.parametrize("N,Q,n", [(3, 24, 2000), (4, 12, 600)])
+def test_closure_mixed_result_independent_of_worklist_strategy(P, O, dev, N, Q, n):
+    T = O.Tables(N, Q)
     a_star, u, a0 = O.closure_batch_input(T, n, 0xC3D2015 + 77, 0.5, 8.0)
     outs = [_run(P, dev, T, u, a0, P.closure_opts(), P.PREC_MIXED, tail=t) for t in TAILS]
+    assert ((outs[0][2]["flags"] & P.FLAG_FP64_FALLBACK) != 0).mean() <= 0.01
     for ug, ag, sg, rc in outs[1:]:
         assert rc == outs[0][3]
         assert np.array_equal(ag, outs[0][1]) and np.array_equal(ug, outs[0][0])

@@ -61,9 +61,8 @@ def test_deck_reader_rules(tmp_path):
 
 def test_tuning_table_lookup_and_candidates(tmp_path):
     """The tuner's candidate grid keeps the four-wave threshold at or below the tail switch
-    and covers the wave counts of the mixed sweep of order 4 (the column sweep of orders 1-3
-    has one launch shape); the front end reads entries by (N, Q, precision) and reports
-    absent ones as None."""
+    and holds the tail knobs only (the column sweep has one launch shape at every order); the
+    front end reads entries by (N, Q, precision) and reports absent ones as None."""
     import importlib.util
     import json
     import os
@@ -76,12 +75,11 @@ def test_tuning_table_lookup_and_candidates(tmp_path):
     cols = list(tune.candidates("mixed", 16))
     fp64 = list(tune.candidates("fp64", 16))
     assert all(c["tail_wide"] <= c["tail_cells"] for c in mixed + cols + fp64)
-    assert {c["mixed_waves"] for c in mixed} == {2, 3}
-    assert all("mixed_waves" not in c for c in cols + fp64)
+    assert all(set(c) == {"tail_cells", "tail_passes", "tail_wide"} for c in mixed + cols + fp64)
     P = product()
     path = tmp_path / "t.json"
     path.write_text(json.dumps({"3,24,mixed": {"tail_cells": 8192, "tail_passes": 32,
-                                               "tail_wide": 512, "mixed_waves": 3}}))
-    assert P.tuning_for(3, 24, "mixed", str(path))["mixed_waves"] == 3
+                                               "tail_wide": 512}}))
+    assert P.tuning_for(3, 24, "mixed", str(path))["tail_wide"] == 512
     assert P.tuning_for(3, 24, "fp64", str(path)) is None
     assert P.tuning_for(3, 24, "mixed", str(tmp_path / "absent.json")) is None
