@@ -270,9 +270,9 @@ int c3d_set_tail_wide(c3d_ctx ctx, int max_cells);
 
 /* Launch and hand-over parameters of the closure kernels, for tuning per (N, Q); results do not
  * depend on the launch shapes.  No counterpart in the reference.
- *   C3D_LAUNCH_MIXED_WAVES        2 or 3, accepted and without effect: the mixed sweeps have
- *                                 one launch shape each (the column sweep of M_1..M_3 eight
- *                                 waves per block, the M_4 sweep two waves per SIMD)
+ *   C3D_LAUNCH_MIXED_WAVES        2 or 3, accepted and without effect: the mixed sweep (the
+ *                                 column sweep) has one launch shape per order (eight waves
+ *                                 per block for M_1..M_3, four for M_4)
  *   C3D_LAUNCH_MIXED_FINISH_COND  10 * log10 of the condition indicator above which a mixed
  *                                 cell with ||g|| <= switch_tol continues on FP64 Hessians
  *                                 (default 50, i.e. 1e5); changes which cells take that path
