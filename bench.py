@@ -270,8 +270,6 @@ def main():
     ap.add_argument("--tail", type=int, nargs=3, default=None, metavar=("CELLS", "ROUNDS", "WIDE"),
                     help="the closure's tail thresholds (c3d_set_tail, c3d_set_tail_wide), "
                          "overriding the tuning table")
-    ap.add_argument("--mixed-waves", type=int, choices=(2, 3), default=None,
-                    help="waves per SIMD of the mixed sweep (default: the library's)")
     ap.add_argument("--finish-cond", type=int, default=None,
                     help="10 log10 of the condition indicator above which a mixed cell "
                          "finishes on FP64 Hessians (default: the library's)")
@@ -315,8 +313,6 @@ def main():
     n = args.cells
     ctx = P.Ctx(N, Q)
     ctx.set_mixed(args.mixed_cond, args.mixed_iter)
-    if args.mixed_waves is not None:
-        ctx.set_launch(P.LAUNCH_MIXED_WAVES, args.mixed_waves)
     if args.finish_cond is not None:
         ctx.set_launch(P.LAUNCH_MIXED_FINISH_COND, args.finish_cond)
     nm, nq = ctx.nm, ctx.nq
@@ -361,8 +357,6 @@ def main():
         ctx.set_warmstart(path.endswith("+nn"))
         if not args.no_tuning:
             tuning[path] = ctx.apply_tuning("mixed" if path.startswith("mixed") else "fp64")
-        if args.mixed_waves is not None:  # explicit values win over the table
-            ctx.set_launch(P.LAUNCH_MIXED_WAVES, args.mixed_waves)
         if args.tail is not None:
             ctx.set_tail(args.tail[0], args.tail[1], wide_cells=args.tail[2])
         ctx.set_launch(P.LAUNCH_ASYNC, args.async_passes)

@@ -964,6 +964,92 @@ __device__ __forceinline__ void store_H(const d4 *acc, double *H, int lane)
     }
 }
 
+// One cell's FP64 sweep for nm > 16 on a block of four waves, with chunk_mfma's bits: wave
+// b < 3 runs the matrix-core chain of H's accumulator block b only (rows 16 R.., columns
+// 16 Cb.., as store_H numbers them), every working wave evaluates the chunk's exponents and
+// weights as chunk_mfma does, and wave 0 sums f.  The chunks are the row image's (rows with
+// leading dimension nm | 1, then the chunk's weights), staged by the whole block into two LDS
+// buffers with one barrier per chunk.  Every thread of the block calls it; it returns f in
+// wave 0 (0 elsewhere) and leaves H in sH, published by its last barrier.
+template <int NM>
+__device__ double sweep_fp64_blocks(int nqPad, const double *__restrict__ Rimg,
+                                    const double *sAc, double *sR0, double *sR1, double *sH,
+                                    int lane, int wv)
+{
+    static_assert(NM > 16, "one accumulator block per wave");
+    constexpr int NMS = NM | 1;
+    constexpr int KD = row_chunk_doubles(NM);
+    constexpr int kPer = (KD + kBlock - 1) / kBlock;
+    const int tid = wv * kWave + lane;
+    const int nch = nqPad / kChunk;
+    const int gq = lane >> 4, rq = lane & 15;
+    const bool lowRows = wv == 0, highCols = wv == 2;  // block (R, Cb) = (0,0), (1,0), (1,1)
+    double pre[kPer];
+    auto fetch = [&](int ch) {
+        const double *src = Rimg + (size_t)ch * KD;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int e = tid + kBlock * k;
+            pre[k] = e < KD ? src[e] : 0.0;
+        }
+    };
+    auto put = [&](double *dst) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int e = tid + kBlock * k;
+            if (e < KD)
+                dst[e] = pre[k];
+        }
+    };
+    fetch(0);
+    put(sR0);
+    __syncthreads();
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    double fpart = 0.0;
+    for (int ch = 0; ch < nch; ++ch) {
+        const double *sMc = (ch & 1) ? sR1 : sR0;
+        if (ch + 1 < nch)
+            fetch(ch + 1);
+        if (wv < 3) {
+            const int qo = mfma_ordinate(lane);
+            const double wl = sMc[kChunk * NMS + qo];
+            const double *rowp = &sMc[qo * NMS];
+            double x = 0.0;
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                x = fma(sAc[i], rowp[i], x);
+            const double we = wl * exp(x);
+            fpart = fpart + we;
+#pragma unroll
+            for (int k = 0; k < kChunk / 4; ++k) {
+                const int ql = 4 * k + gq;
+                const double *mr = &sMc[ql * NMS];
+                const double b0 = mr[rq];
+                const double b1 = 16 + rq < NM ? mr[16 + rq] : 0.0;
+                const double wq = __shfl(we, ql);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wq * (lowRows ? b0 : b1),
+                                                           highCols ? b1 : b0, acc, 0, 0, 0);
+            }
+        }
+        if (ch + 1 < nch)
+            put((ch & 1) ? sR0 : sR1);
+        __syncthreads();
+    }
+    if (wv < 3) {
+        const int R = lowRows ? 0 : 1, Cb = highCols ? 1 : 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = 16 * R + gq + 4 * k, j = 16 * Cb + rq;
+            if (i < NM && j < NM && j <= i) {
+                sH[i * NM + j] = acc[k];
+                sH[j * NM + i] = acc[k];
+            }
+        }
+    }
+    __syncthreads();
+    return wv == 0 ? wave_sum(fpart) : 0.0;
+}
+
 // Entry-per-lane assignment of the lower triangle for the reference-order sweep.
 template <int NM>
 struct RefEntries {
@@ -2448,10 +2534,13 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     // else single; the first rows buffer also serves as the other paths' chunk rows (sM)
     constexpr bool COLS = MIXED && NM <= kPairMaxNm;
     constexpr bool COLS4 = COLS && TW == 4;
-    __shared__ __align__(16) double sR[COLS4 ? 2 : 1][COLS ? RowImg<NM>::kDoubles : 1];
+    // FP64 Newton sweeps of nm > 16 with four waves: H's accumulator blocks over the waves,
+    // the chunks staged through both row buffers
+    constexpr bool SPLIT = !REF && TW == 4 && NB > 1;
+    __shared__ __align__(16) double sR[COLS4 || SPLIT ? 2 : 1][COLS || SPLIT ? RowImg<NM>::kDoubles : 1];
     __shared__ __align__(16) _Float16 sP[COLS4 && NM <= 16 ? 2 : 1][COLS ? PairHess<NM>::kHalves : 8];
-    __shared__ double sMo[COLS ? 1 : kChunk * NMS];
-    double *const sM = COLS ? &sR[0][0] : sMo;
+    __shared__ double sMo[COLS || SPLIT ? 1 : kChunk * NMS];
+    double *const sM = COLS || SPLIT ? &sR[0][0] : sMo;
     __shared__ double sH[NM * NM];
     // the state machine runs on 16-lane rows for NM <= 16 (DPP broadcasts; the four rows of a
     // wave hold identical copies), else on the whole wave
@@ -2553,6 +2642,21 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                     const double fdir = sweep_lone<NM, 4>(
                         P.nqPad, Rimg, Pimg, sA[0], C.shift, sR[0], sR[COLS4 ? 1 : 0], sP[0],
                         sP[COLS4 && NM <= 16 ? 1 : 0], sWEl, sRed, sH, sG, true, lane, wv);
+                    if (wv == 0) {
+                        if (lane == 0)
+                            sF[0] = fdir;
+                        ++sweeps;
+                    }
+                    swept = true;
+                }
+            }
+            if constexpr (SPLIT) {
+                if (!mx && !swept) {  // the FP64 sweep, H's accumulator blocks over the waves
+                    if (wv == 0 && lane < NM)
+                        sA[0][lane] = C.a;
+                    __syncthreads();
+                    const double fdir = sweep_fp64_blocks<NM>(P.nqPad, Rimg, sA[0], sR[0],
+                                                              sR[SPLIT ? 1 : 0], sH, lane, wv);
                     if (wv == 0) {
                         if (lane == 0)
                             sF[0] = fdir;
