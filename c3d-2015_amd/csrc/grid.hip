@@ -202,16 +202,31 @@ __global__ __launch_bounds__(kTB) void k_stencil(StencilArgs S, const double *__
 #pragma unroll
     for (int m = 0; m < NM; ++m)
         fl[m] = 0.0;
+    // the 13 ansatz values of the cell's stencil at ordinate q: the centre, then -2, -1, +1,
+    // +2 along x, y and z; the next ordinate's are in flight while this one's are used
+    const ptrdiff_t off[13] = {0,
+                               -2 * (ptrdiff_t)sx, -(ptrdiff_t)sx, (ptrdiff_t)sx, 2 * (ptrdiff_t)sx,
+                               -2 * (ptrdiff_t)sy, -(ptrdiff_t)sy, (ptrdiff_t)sy, 2 * (ptrdiff_t)sy,
+                               -2, -1, 1, 2};
+    double v[13], nv[13];
+#pragma unroll
+    for (int e = 0; e < 13; ++e)
+        v[e] = A[c + off[e]];
     for (int q = 0; q < S.nq; ++q) {
-        const double *Aq = A + (size_t)q * ng;
-        const double a0 = Aq[c];
+        if (q + 1 < S.nq) {
+            const double *An = A + (size_t)(q + 1) * ng + c;
+#pragma unroll
+            for (int e = 0; e < 13; ++e)
+                nv[e] = An[off[e]];
+        }
+        const double a0 = v[0];
         const double wq = w[q];
-        const double fX = axis_flux<TH1>(Aq[c - 2 * sx], Aq[c - sx], a0, Aq[c + sx],
-                                         Aq[c + 2 * sx], xi[q], wq, S.theta);
-        const double fY = axis_flux<TH1>(Aq[c - 2 * sy], Aq[c - sy], a0, Aq[c + sy],
-                                         Aq[c + 2 * sy], eta[q], wq, S.theta);
-        const double fZ = axis_flux<TH1>(Aq[c - 2], Aq[c - 1], a0, Aq[c + 1], Aq[c + 2], mu[q],
-                                         wq, S.theta);
+        const double fX = axis_flux<TH1>(v[1], v[2], a0, v[3], v[4], xi[q], wq, S.theta);
+        const double fY = axis_flux<TH1>(v[5], v[6], a0, v[7], v[8], eta[q], wq, S.theta);
+        const double fZ = axis_flux<TH1>(v[9], v[10], a0, v[11], v[12], mu[q], wq, S.theta);
+#pragma unroll
+        for (int e = 0; e < 13; ++e)
+            v[e] = nv[e];
         double s;
         if constexpr (P2)
             s = fX * S.rdx + fY * S.rdy + fZ * S.rdz;
