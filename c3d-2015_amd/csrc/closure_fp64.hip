@@ -3387,8 +3387,10 @@ hipError_t run_closure(c3d_ctx_s *ctx, const KParams &P, double *u, double *alph
             C3D_TAIL(true, false, 1);
         else if (mixed && (wide || NM > 16))  // nm = 25: the one-wave kernel's 110 KB of LDS
             C3D_TAIL(false, true, 4);         // leave one block per CU either way
-        else if (mixed)
-            C3D_TAIL(false, true, 1);
+        else if (mixed) {
+            if constexpr (NM <= 16)
+                C3D_TAIL(false, true, 1);
+        }
         else if (wide)
             C3D_TAIL(false, false, 4);
         else
