@@ -2655,8 +2655,8 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                     if (wv == 0 && lane < NM)
                         sA[0][lane] = C.a;
                     __syncthreads();
-                    const double fdir = sweep_fp64_blocks<NM>(P.nqPad, Rimg, sA[0], sR[0],
-                                                              sR[SPLIT ? 1 : 0], sH, lane, wv);
+                    const double fdir =
+                        sweep_fp64_blocks<NM>(P.nqPad, Rimg, sA[0], sR[0], sR[1], sH, lane, wv);
                     if (wv == 0) {
                         if (lane == 0)
                             sF[0] = fdir;
