@@ -1278,7 +1278,10 @@ __constant__ double kExp2Tab[64] = {
     0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0,
     0x1.fa7c1819e90d8p+0,
 };
-__device__ __forceinline__ double mexp(double x)
+// tab: the table's 64 entries, kExp2Tab or a copy of it in LDS.  Where LDS DMA copies or
+// prefetched loads are in flight, the copy in LDS keeps the lookup from waiting for them (the
+// vector-memory counter drains in order).
+__device__ __forceinline__ double mexp(double x, const double *tab)
 {
     const double kf = __builtin_rint(x * 0x1.71547652b82fep+6);  // 64 / ln2
     double r = fma(kf, -0x1.62e42fefa39efp-7, x);                // ln2 / 64, high part
@@ -1289,8 +1292,9 @@ __device__ __forceinline__ double mexp(double x)
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
-    return ldexp(kExp2Tab[k & 63] * p, k >> 6);
+    return ldexp(tab[k & 63] * p, k >> 6);
 }
+__device__ __forceinline__ double mexp(double x) { return mexp(x, kExp2Tab); }
 
 // The FP16 operands carry the weights scaled by 2^12.  At the shift's own iterate the weights
 // sum to one, so the largest lies in [1/nq, 1] and every operand in FP16's range with full
@@ -1584,7 +1588,7 @@ __device__ __forceinline__ double sweep_lone(int nqPad, const double *__restrict
                                              double shift, double *sR0, double *sR1,
                                              _Float16 *sP0, _Float16 *sP1, double *sWE,
                                              float *sRed, double *sH, double *sG, bool mark,
-                                             int lane, int wv)
+                                             int lane, int wv, const double *xtab)
 {
     using PH = PairHess<NM>;
     using RI = RowImg<NM>;
@@ -1626,7 +1630,7 @@ __device__ __forceinline__ double sweep_lone(int nqPad, const double *__restrict
                 const double bm = (NM % 4 == 0 || m < NM) ? sM[(16 * b + o) * NMS + m] : 0.0;
                 x = __builtin_amdgcn_mfma_f64_4x4x4f64(ax.a[s], bm, x, 0, 0, 0);
             }
-            const double we = sW[16 * b + o] * mexp(x - shiftOwn);
+            const double we = sW[16 * b + o] * mexp(x - shiftOwn, xtab);
             wmaxOwn = fmaxf(wmaxOwn, (float)we);
             if (lane < 16)
                 sWE[16 * b + o] = we;
@@ -2423,7 +2427,8 @@ __device__ __forceinline__ void f_trials(const KParams &P, const double *__restr
 template <int NM>
 __device__ __forceinline__ void f_trials_mixed(const KParams &P, const double *__restrict__ MT,
                                                const double *__restrict__ wg, const double *sA,
-                                               int kw, double shift, int lane, double (&out)[4])
+                                               int kw, double shift, int lane, double (&out)[4],
+                                               const double *xtab)
 {
     constexpr int NKS = ExpA<NM>::NKS;
     const int nch = P.nqPad / kChunk;
@@ -2449,7 +2454,7 @@ __device__ __forceinline__ void f_trials_mixed(const KParams &P, const double *_
         exp_arg4<NM>(ax, [&](int g, int t) { return b[g][t]; }, xg);
 #pragma unroll
         for (int g = 0; g < 4; ++g)
-            fOwn = fOwn + w[g] * mexp(xg[g] - shift);
+            fOwn = fOwn + w[g] * mexp(xg[g] - shift, xtab);
     };
     if constexpr (NKS <= 4) {
         // two chunks at a time: their eight exponent chains and exps are independent work for
@@ -2466,8 +2471,8 @@ __device__ __forceinline__ void f_trials_mixed(const KParams &P, const double *_
                 double ea[4], eb[4];
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
-                    ea[g] = wA[g] * mexp(xa[g] - shift);
-                    eb[g] = wB[g] * mexp(xb[g] - shift);
+                    ea[g] = wA[g] * mexp(xa[g] - shift, xtab);
+                    eb[g] = wB[g] * mexp(xb[g] - shift, xtab);
                 }
                 if (ch + 2 < nch)
                     load(bA, wA, ch + 2);
@@ -2556,11 +2561,17 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     __shared__ double sWEl[COLS ? kChunk : 1];  // the lone-cell sweep's weights
     __shared__ double sG[NM];
     __shared__ float sRed[4];
+    __shared__ double sX[MIXED ? 64 : 1];  // the mixed weights' exp table
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int nWm = uniform_int(*Lc.nm);
     const int nW = nWm + uniform_int(*Lc.nf);
     if ((int)blockIdx.x >= nW)
         return;
+    if constexpr (MIXED) {
+        for (int e = threadIdx.x; e < 64; e += TW * kWave)
+            sX[e] = kExp2Tab[e];
+        __syncthreads();
+    }
     const int li = lane & (G - 1), row = lane / G;
     for (int slot = blockIdx.x; slot < nW; slot += gridDim.x) {
         const int c = uniform_int(list_cell(Lc, slot, nWm));
@@ -2594,7 +2605,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                     double ft[KWA];
                     if (mx) {  // the mixed sweeps' f: the same bits as the sweep's
                         double f4p[4];
-                        f_trials_mixed<NM>(P, MT, wg, sA[wv], kw, C.shift, lane, f4p);
+                        f_trials_mixed<NM>(P, MT, wg, sA[wv], kw, C.shift, lane, f4p, sX);
 #pragma unroll
                         for (int j = 0; j < KWA; ++j)
                             ft[j] = j < 4 ? f4p[j] : 0.0;
@@ -2641,7 +2652,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                     __syncthreads();
                     const double fdir = sweep_lone<NM, 4>(
                         P.nqPad, Rimg, Pimg, sA[0], C.shift, sR[0], sR[COLS4 ? 1 : 0], sP[0],
-                        sP[COLS4 && NM <= 16 ? 1 : 0], sWEl, sRed, sH, sG, true, lane, wv);
+                        sP[COLS4 && NM <= 16 ? 1 : 0], sWEl, sRed, sH, sG, true, lane, wv, sX);
                     if (wv == 0) {
                         if (lane == 0)
                             sF[0] = fdir;
@@ -2672,7 +2683,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                 if (mx) {  // the mixed phase's lone-cell sweep
                     wave_lds_sync();  // sA[0]
                     fdir = sweep_lone<NM, 1>(P.nqPad, Rimg, Pimg, sA[0], C.shift, sR[0], sR[0],
-                                             sP[0], sP[0], sWEl, sRed, sH, sG, true, lane, 0);
+                                             sP[0], sP[0], sWEl, sRed, sH, sG, true, lane, 0, sX);
                 } else if constexpr (REF) {
                     const RefEntries<NM> E(lane);
                     double h[RefEntries<NM>::NS];
@@ -3092,10 +3103,12 @@ __global__ __launch_bounds__(kWave) void k_hessian_cols(int64_t ncells, int nqPa
     __shared__ double sH[PairHess<NM>::NP];
     __shared__ double sG[NM];
     __shared__ double sWE[kChunk];
+    __shared__ double sX[64];  // the exp table
     const int lane = threadIdx.x;
     const int64_t cell = blockIdx.x;
     if (cell >= ncells)
         return;
+    sX[lane] = kExp2Tab[lane];
     if (lane < NM)
         sA[lane] = alpha_g[cell * NM + lane];
     double xmax = -DBL_MAX, wtop = 0.0;
@@ -3124,7 +3137,7 @@ __global__ __launch_bounds__(kWave) void k_hessian_cols(int64_t ncells, int nqPa
     const double shift = xmax + log(wtop);
     wave_lds_sync();
     (void)sweep_lone<NM, 1>(nqPad, Rimg, Pimg, sA, shift, sR, sR, sP, sP, sWE, nullptr, sH, sG,
-                            false, lane, 0);
+                            false, lane, 0, sX);
     wave_lds_sync();
     double *H = hs + cell * NM * NM;
     for (int p = lane; p < PairHess<NM>::NP; p += kWave) {
