@@ -943,6 +943,84 @@ __device__ __forceinline__ void chunk_mfma(const double *sMc, double wl, const d
     }
 }
 
+// chunk_mfma for four cells and nm <= 9 on V_MFMA_F64_4X4X4_4B_F64 (as fast per flop as the
+// 16x16 instruction): the lower-triangle 4x4 blocks (R, C) of H, one instruction per block
+// with the four cells in its four block slots, in place of a 16x16 tile per cell of which
+// nm^2 entries are used.  Lane l: cell (l >> 2) & 3, row or column l & 3 of the block, K index
+// l >> 4 (ordinate 4 k + (l >> 4) of K step k, as in chunk_mfma).  Each element's chain of
+// fused multiply-adds over the ordinates, and its operands w_q m_qi (rounded) and m_qj, are
+// chunk_mfma's, so H has the same bits; the exponents and f are chunk_mfma's code.
+template <int NM>
+struct Blk4 {
+    static constexpr int NR = (NM + 3) / 4;
+    static constexpr int NBLK = NR * (NR + 1) / 2;
+};
+template <int NM>
+__device__ __forceinline__ void chunk_mfma4(const double *sMc, double wl, const double *sAc,
+                                            double (&acc)[Blk4<NM>::NBLK], double (&fpart)[4],
+                                            int lane)
+{
+    static_assert(NM <= 9, "the 16x16 tile serves larger orders");
+    constexpr int NR = Blk4<NM>::NR;
+    constexpr int NMS = NM | 1;
+    const double *rowp = &sMc[mfma_ordinate(lane) * NMS];
+    double we[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double x = 0.0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i)
+            x = fma(sAc[c * NM + i], rowp[i], x);
+        we[c] = wl * exp(x);
+        fpart[c] = fpart[c] + we[c];
+    }
+    const int ii = lane & 3, kq = lane >> 4, cb = (lane >> 2) & 3;
+#pragma unroll
+    for (int k = 0; k < kChunk / 4; ++k) {
+        const int ql = 4 * k + kq;
+        const double *mr = &sMc[ql * NMS];
+        double mv[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            mv[r] = (4 * r + ii < NM) ? mr[4 * r + ii] : 0.0;
+        double w4[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            w4[c] = __shfl(we[c], ql);
+        const double wq = cb == 0 ? w4[0] : (cb == 1 ? w4[1] : (cb == 2 ? w4[2] : w4[3]));
+        int p = 0;
+#pragma unroll
+        for (int R = 0; R < NR; ++R) {
+            const double a = wq * mv[R];
+#pragma unroll
+            for (int C = 0; C <= R; ++C) {
+                acc[p] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, mv[C], acc[p], 0, 0, 0);
+                ++p;
+            }
+        }
+    }
+}
+
+// H of the lane's cell from chunk_mfma4's blocks: element (4 R + (l >> 4), 4 C + (l & 3)),
+// lower triangle, mirrored (store_H's writes).
+template <int NM>
+__device__ __forceinline__ void store_H4(const double (&acc)[Blk4<NM>::NBLK], double *H, int lane)
+{
+    const int i0 = lane >> 4, j0 = lane & 3;
+    int p = 0;
+#pragma unroll
+    for (int R = 0; R < Blk4<NM>::NR; ++R)
+#pragma unroll
+        for (int C = 0; C <= R; ++C) {
+            const int i = 4 * R + i0, j = 4 * C + j0;
+            if (i < NM && j < NM && j <= i) {
+                H[i * NM + j] = acc[p];
+                H[j * NM + i] = acc[p];
+            }
+            ++p;
+        }
+}
+
 // H from one cell's accumulators: lower triangle, mirrored (fobj.cpp:96-101).
 template <int NM>
 __device__ __forceinline__ void store_H(const d4 *acc, double *H, int lane)
@@ -1144,12 +1222,18 @@ __global__ __launch_bounds__(kBlock) void k_sweep(KParams P, const double *__res
             if (lane < NM)
                 sA[wv][c * NM + lane] =
                     cells[c] >= 0 ? S.vec[(size_t)cells[c] * kVec * NM + kVa * NM + lane] : 0.0;
-        d4 acc[CW][NB];
+        constexpr bool B4 = NM <= 9;  // 4x4 blocks, the four cells in the block slots
+        static_assert(!B4 || CW == 4, "one cell per block slot");
+        d4 acc[B4 ? 1 : CW][B4 ? 1 : NB];
+        double acc4[Blk4<B4 ? NM : 4>::NBLK];
 #pragma unroll
-        for (int c = 0; c < CW; ++c)
+        for (int c = 0; c < (B4 ? 1 : CW); ++c)
 #pragma unroll
-            for (int b = 0; b < NB; ++b)
+            for (int b = 0; b < (B4 ? 1 : NB); ++b)
                 acc[c][b] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int p = 0; p < Blk4<B4 ? NM : 4>::NBLK; ++p)
+            acc4[p] = 0.0;
         double fpart[CW];
 #pragma unroll
         for (int c = 0; c < CW; ++c)
@@ -1164,8 +1248,13 @@ __global__ __launch_bounds__(kBlock) void k_sweep(KParams P, const double *__res
             __syncthreads();
             if (ch + 1 < nch)
                 st.fetch(Mg, wg, ch + 1, tid);
-            if (any)
-                chunk_mfma<NM, CW>(sM[b], sW[b][mfma_ordinate(lane)], sA[wv], acc, fpart, lane);
+            if (any) {
+                if constexpr (B4)
+                    chunk_mfma4<NM>(sM[b], sW[b][mfma_ordinate(lane)], sA[wv], acc4, fpart, lane);
+                else
+                    chunk_mfma<NM, CW>(sM[b], sW[b][mfma_ordinate(lane)], sA[wv], acc, fpart,
+                                       lane);
+            }
         }
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
@@ -1174,7 +1263,13 @@ __global__ __launch_bounds__(kBlock) void k_sweep(KParams P, const double *__res
                 continue;
             if (lane == 0)
                 S.fdir[cells[c]] = fs;
-            store_H<NM>(acc[c], S.H + (size_t)cells[c] * NM * NM, lane);
+            if constexpr (!B4)
+                store_H<NM>(acc[c], S.H + (size_t)cells[c] * NM * NM, lane);
+        }
+        if constexpr (B4) {
+            const int cb = (lane >> 2) & 3;
+            if (cells[cb] >= 0)
+                store_H4<NM>(acc4, S.H + (size_t)cells[cb] * NM * NM, lane);
         }
     }
 }
