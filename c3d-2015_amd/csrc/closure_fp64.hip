@@ -842,9 +842,10 @@ __global__ __launch_bounds__(kBlock) void k_init(KParams P, const double *__rest
 
 // ---------------------------------------------------------------- sweep building blocks
 // FP32 copy of an M chunk for the matrix-core operands, laid out so that lane (rq, gq) of
-// the MFMA finds M[4k + gq][rq] for k = 0..15 in one 16-float row: row (col * 4 + gq),
-// element k, rows kRS floats apart (16-byte aligned, spread over the banks).  Rows of
-// columns nm..16 NT-1 stay zero.
+// the MFMA finds M[4k + gq][rq] for k = 0..15 in one 16-float row: row (col * 4 + gq), rows
+// kRS floats apart (16-byte aligned), element k in 4-float group (k >> 2) ^ ((col >> 2) & 3)
+// of the row, so that the 16 lanes of a group reading group kb of their rows cover all 64
+// banks.  Rows of columns nm..16 NT-1 stay zero.
 constexpr int kRS = 20;
 template <int NM>
 struct MixedChunk {
@@ -852,7 +853,8 @@ struct MixedChunk {
     static constexpr int kFloats = 16 * NT * 4 * kRS;
     static __device__ __forceinline__ int index(int rr, int cc)
     {
-        return (cc * 4 + (rr & 3)) * kRS + (rr >> 2);
+        const int k = rr >> 2;
+        return (cc * 4 + (rr & 3)) * kRS + 4 * ((k >> 2) ^ ((cc >> 2) & 3)) + (k & 3);
     }
     // zero the padding columns of a buffer (once per kernel, any number of threads)
     static __device__ __forceinline__ void clear_pad(float *sMfT, int tid, int nthreads)
@@ -2905,7 +2907,10 @@ __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double 
     __shared__ __align__(16) float sMfT[2][MixedChunk<NM>::kFloats];
     __shared__ float sW[2][kChunk];
     __shared__ float sA[kWavesPerBlock][NV * NM];
-    __shared__ __align__(16) float sWE[kWavesPerBlock][NV * kChunk];
+    // weights by point, rows kChunk + 4 floats apart: the A operand's 16 rows fall in distinct
+    // banks
+    constexpr int kWES = kChunk + 4;
+    __shared__ __align__(16) float sWE[kWavesPerBlock][NV * kWES];
     __shared__ float sN[kWavesPerBlock][NV];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int gq = lane >> 4, rq = lane & 15;
@@ -2990,7 +2995,7 @@ __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double 
 #pragma unroll
                 for (int v = h * NV / 4; v < (h + 1) * NV / 4; ++v) {
                     if ((v & 1) == 0 && (iso >> (v >> 1)) & 1u) {  // wave-uniform
-                        sWE[wv][v * kChunk + (lane & 3) * 16 + (lane >> 2)] = 0.0f;
+                        sWE[wv][v * kWES + (lane & 3) * 16 + (lane >> 2)] = 0.0f;
                         continue;
                     }
                     float x = 0.0f;
@@ -2999,22 +3004,24 @@ __global__ __launch_bounds__(kBlock) void k_warm_select(KParams P, const double 
                         x = fmaf(sA[wv][v * NM + i], rowp[i], x);
                     const float we = wl * __expf(x);  // hardware exp2: ranking needs no more
                     fpart[v] += we;
-                    sWE[wv][v * kChunk + (lane & 3) * 16 + (lane >> 2)] = we;
+                    sWE[wv][v * kWES + (lane & 3) * 16 + (lane >> 2)] = we;
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
             wave_lds_sync();
             const float *m0 = &sMfT[b][(rq * 4 + gq) * RS];
-            const float *a0 = &sWE[wv][(rq < NV ? rq : 0) * kChunk + gq * 16];
+            const float *a0 = &sWE[wv][(rq < NV ? rq : 0) * kWES + gq * 16];
+            const int swz = (rq >> 2) & 3;  // the rows' group swizzle (MixedChunk)
 #pragma unroll
             for (int kb = 0; kb < kChunk / 16; ++kb) {
                 f4 av = *reinterpret_cast<const f4 *>(a0 + 4 * kb);
                 if (rq >= NV)
                     av = f4{0.0f, 0.0f, 0.0f, 0.0f};
-                const f4 b0v = *reinterpret_cast<const f4 *>(m0 + 4 * kb);
+                const f4 b0v = *reinterpret_cast<const f4 *>(m0 + 4 * (kb ^ swz));
                 f4 b1v = f4{0.0f, 0.0f, 0.0f, 0.0f};
                 if constexpr (NT == 2)
-                    b1v = *reinterpret_cast<const f4 *>(&sMfT[b][((16 + rq) * 4 + gq) * RS + 4 * kb]);
+                    b1v = *reinterpret_cast<const f4 *>(
+                        &sMfT[b][((16 + rq) * 4 + gq) * RS + 4 * (kb ^ swz)]);
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) {
                     acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], b0v[kk], acc[0], 0, 0, 0);
