@@ -220,10 +220,11 @@ def test_closure_reference_order_bit_exact(P, O, dev, N, Q, n, smax, tail):
     assert rc_o == 0 and rc_g == 0
 
 
-@pytest.mark.parametrize("N,Q,n", [(3, 24, 2500), (4, 12, 500), (1, 6, 700)])
+@pytest.mark.parametrize("N,Q,n", [(3, 24, 2500), (4, 12, 500), (1, 6, 700), (2, 8, 1500)])
 def test_closure_result_independent_of_worklist_strategy(P, O, dev, N, Q, n):
     """The matrix-core mode gives the same bits per cell whether a cell finishes in the
-    launch-pair passes, in the fused tail kernel, or moves between them."""
+    launch-pair passes, in the fused tail kernel, or moves between them (for nm <= 9 the
+    passes' sweep forms H from 4x4 blocks, the tail's from a 16x16 tile)."""
     T = _tables(O, N, Q)
     a_star, u, a0 = O.closure_batch_input(T, n, 0xC3D2015 + 3 * N, 0.5, 8.0)
     outs = [_run_gpu_closure(P, dev, T, u, a0, P.closure_opts(), tail=t) for t in TAILS]
