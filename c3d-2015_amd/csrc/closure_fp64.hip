@@ -2680,8 +2680,10 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
         for (int p = 0; p < passes; ++p) {
             const bool mx = MIXED && C.mixed(P);
             if (C.phase == kPhaseLS) {
-                // points k = wv * kw + j at tls / 2^k, formed as ls_next() forms them
-                const int kw = C.tls < 1.0 ? (mx ? KWM : KW) : 1;
+                // points k = wv * kw + j at tls / 2^k, formed as ls_next() forms them; the
+                // mixed phase's points are the rows of one exponent product, so its four cost
+                // what one does and are taken from the first trial on
+                const int kw = mx ? KWM : (C.tls < 1.0 ? KW : 1);
                 double tk = C.tls;
                 for (int k = 0; k < wv * kw; ++k)
                     tk = 0.5 * tk;
