@@ -2629,7 +2629,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
     // evaluation's (f costs nq (NM + exp) per point)
     constexpr int KW = REF ? 1 : (NM <= 9 ? 8 : (NM <= 16 ? 2 : 1));
     // in the mixed phase the points are the rows of the exponent products: four per wave
-    constexpr int KWM = NM <= 16 ? 4 : 2;
+    constexpr int KWM = 4;
     constexpr int KWA = MIXED && KWM > KW ? KWM : KW;
     static_assert(!(REF && MIXED), "one arithmetic mode");
     // the column sweep's images (nm <= kPairMaxNm, mixed): double-buffered with four waves,
