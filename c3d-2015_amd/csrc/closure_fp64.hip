@@ -2441,7 +2441,7 @@ enum : int { kFp64 = 0, kFRef = 1 };
 template <int NM, int MODE, int KW>
 __device__ __forceinline__ void f_trials(const KParams &P, const double *__restrict__ MT,
                                          const double *__restrict__ wg, const double *sA,
-                                         int kw, double shift, double *sW, double *sE, int lane,
+                                         int kw, double *sW, double *sE, int lane,
                                          double (&out)[KW])
 {
     static_assert(MODE != kFRef || KW == 1, "reference order: one point per wave");
@@ -2710,7 +2710,7 @@ __global__ __launch_bounds__(TW * kWave) void k_tail(KParams P, const double *__
                             ft[j] = j < 4 ? f4p[j] : 0.0;
                     } else {
                         double fk[KW];
-                        f_trials<NM, REF ? kFRef : kFp64, KW>(P, MT, wg, sA[wv], kw, 0.0, sW[wv],
+                        f_trials<NM, REF ? kFRef : kFp64, KW>(P, MT, wg, sA[wv], kw, sW[wv],
                                                               sE[wv], lane, fk);
 #pragma unroll
                         for (int j = 0; j < KWA; ++j)
